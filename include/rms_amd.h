@@ -1,0 +1,166 @@
+/*
+ * rms_amd.h -- C ABI of the MI355X RMS random-walk engine (librms_amd.so).
+ *
+ * The reference engine has no library interface: hpc_code/lib/main.cu and
+ * main_PGSE.cu are single programs whose main() reads text files from the
+ * working directory, launches propagate<<<>>> once and writes text files back
+ * (main.cu:388-621, main_PGSE.cu:414-675).  This ABI is what that process
+ * boundary becomes; the drop-in CLIs rms_main / rms_main_pgse are thin
+ * clients of it.  Each entry point names the reference code it replaces.
+ *
+ * Conventions: caller-owned host buffers, device buffers owned by the
+ * context, one context per host thread (contexts are not internally
+ * synchronised), each context owns one HIP stream, every call returns a
+ * status and rms_last_error() describes the last failure.  No C++
+ * exceptions cross the ABI.
+ */
+#ifndef RMS_AMD_H
+#define RMS_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RMS_NC_MAX 4        /* classes per geometry; reference Nc_max = 3 (main.cu:37) */
+#define RMS_NITE 4          /* face iterations per step (main.cu:36) */
+#define RMS_REF_TIMEPOINTS 1000 /* records per run in the reference (main.cu:35) */
+#define RMS_NPARAM 16       /* adjoint parameters: dD[4], dT2[4], dkappa, pad */
+
+typedef enum {
+    RMS_OK = 0,
+    RMS_ERR_INVALID_ARG = 1,
+    RMS_ERR_HIP = 2,
+    RMS_ERR_OUT_OF_MEMORY = 3,
+    RMS_ERR_NO_DEVICE = 4,
+    RMS_ERR_STATE = 5,
+    RMS_ERR_UNSUPPORTED = 6
+} rms_status;
+
+enum { RMS_PREC_F32 = 0, RMS_PREC_F64 = 1 };
+
+typedef struct rms_ctx rms_ctx;
+typedef struct rms_plan rms_plan;
+
+typedef struct {
+    int32_t device;       /* HIP device ordinal */
+    int32_t reserved;
+} rms_device_opts;
+
+/* A voxelised geometry with per-class physics.  Replaces the reads of
+ * fiber.txt (main.cu:460-466), diffusivity.txt (:421-427), T2.txt (:429-436)
+ * and the Pij construction (:444-458). */
+typedef struct {
+    int32_t nx, ny, nz;          /* NPix1, NPix2, NPix3 */
+    int32_t label_bytes;         /* 1 = uint8, 2 = uint16 labels */
+    const void *labels;          /* host, nx*ny*nz, index (x*ny + y)*nz + z */
+    const uint8_t *label_class;  /* label -> class map, NULL = label is the class */
+    int32_t nlabel;              /* entries of label_class */
+    int32_t nclass;              /* Nc; class 0 is the null (impermeable) class */
+    double res;                  /* voxel size, um */
+    double D[RMS_NC_MAX];        /* um^2/ms */
+    double T2[RMS_NC_MAX];       /* ms; INFINITY = no relaxation */
+    /* permeability between classes (um/ms), row-major [i][j];
+     * < 0 selects the reference rule P_ij = min(1, sqrt(D_j/D_i)) */
+    double kappa[RMS_NC_MAX * RMS_NC_MAX];
+} rms_geometry;
+
+/* One simulation on one geometry (one reference setup directory). */
+typedef struct {
+    int32_t geom_id;             /* from rms_add_geometry */
+    int64_t nwalk;               /* NPar */
+    uint64_t seed;               /* Philox key (replaces /dev/random, main.cu:491-496) */
+    int32_t nbvec;               /* narrow-pulse b-table (bval.txt/bvec.txt), 0 = none */
+    const double *bval;          /* [nbvec] ms/um^2 */
+    const double *bvec;          /* [nbvec*3] */
+    int32_t ngrad;               /* PGSE table (gradient_time.txt), 0 = none */
+    const double *grad;          /* [ngrad*6] rows: Delta, delta, |g|, gx, gy, gz */
+} rms_job;
+
+typedef struct {
+    double dt;                   /* ms */
+    int64_t TN;                  /* steps */
+    int32_t timepoints;          /* records (reference: 1000) */
+    int32_t precision;           /* RMS_PREC_F32 (FP32 offsets) or RMS_PREC_F64 */
+    int32_t replicas;            /* batch-means replicas for standard errors (>= 1) */
+    int64_t chunk_steps;         /* steps per launch, 0 = all in one launch */
+    int32_t use_graph;           /* capture the launch sequence in a hipGraph */
+    int32_t block_threads;       /* walkers per workgroup, 0 = auto (256) */
+    int32_t grid_blocks;         /* workgroups, 0 = auto (resident limit) */
+    int32_t xcd_split;           /* 1 = give each XCD a contiguous batch range */
+} rms_run_params;
+
+/* Per-job outputs (caller-owned host buffers; NULL pointers are skipped).
+ * Record layout per time point: [sig0, NPar_count[nclass], dx2[6], dx4[15],
+ * sigRe[nbvec]] -- the sig0.txt, NPar_count.txt, dx2.txt, dx4.txt and
+ * sigRe.txt columns of main.cu:549-604. */
+typedef struct {
+    double *rec;                 /* [timepoints * nval] */
+    double *rec_rep;             /* [replicas * timepoints * nval] */
+    double *sig;                 /* [ngrad] PGSE signal (sig.txt, main_PGSE.cu:648-651) */
+    double *sig_rep;             /* [replicas * ngrad] */
+    uint64_t counters[5];        /* crossings, reflections, edge mirrors, truncated steps, exchange draws */
+    /* optional final walker state, axis-major [3][nwalk] etc. */
+    int32_t *fin_g;
+    double *fin_o;
+    int32_t *fin_gi;
+    double *fin_oi;
+    int32_t *fin_c;
+    uint32_t *fin_cnt;           /* [5][nwalk] */
+} rms_job_out;
+
+typedef struct {
+    double walk_ms;              /* hipEvent time of the walk launches */
+    double reduce_ms;            /* hipEvent time of the ordered reductions */
+    double total_ms;             /* hipEvent time of the whole execute */
+    int32_t grid;                /* workgroups used */
+    int32_t block;               /* threads per workgroup */
+    int32_t launches;            /* walk launches */
+    int32_t reserved;
+} rms_timing;
+
+/* Adjoint outputs: derivatives of sum(cot_rec * rec) + sum(cot_sig * sig)
+ * with respect to D[c] (index c), T2[c] (4 + c) and kappa (8). */
+typedef struct {
+    const double *cot_rec;       /* [timepoints * nval] or NULL */
+    const double *cot_sig;       /* [ngrad] or NULL */
+    double *grad;                /* [RMS_NPARAM] output */
+    double *grad_rep;            /* [replicas * RMS_NPARAM] or NULL */
+} rms_job_adj;
+
+/* ---- context (replaces the implicit CUDA context + default stream) ---- */
+rms_status rms_create(const rms_device_opts *opts, rms_ctx **out);
+rms_status rms_destroy(rms_ctx *ctx);
+const char *rms_last_error(const rms_ctx *ctx);
+const char *rms_version(void);
+
+/* Upload a geometry; replaces thrust::device_vector<int> d_APix (main.cu:532). */
+rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id);
+
+/* Derived per-class quantities of a geometry at time step dt, as the
+ * reference host computes them (main.cu:429-458): step[c] in voxels,
+ * T2s[c] in steps, P[i*RMS_NC_MAX + j]. */
+rms_status rms_derive(const rms_geometry *geom, double dt, double *step, double *T2s, double *P);
+
+/* Record row width: 1 + nclass + 6 + 15 + nbvec. */
+int32_t rms_nval(int32_t nclass, int32_t nbvec);
+
+/* ---- plans: validate + upload tables once, execute many times ---- */
+rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_run_params *params,
+                           rms_plan **out);
+/* Runs the walk (setup_kernel + propagate + copies, main.cu:498-547). */
+rms_status rms_plan_execute(rms_plan *plan, rms_job_out *outs, rms_timing *timing);
+rms_status rms_plan_destroy(rms_plan *plan);
+
+/* One-shot convenience: plan_create + plan_execute + plan_destroy. */
+rms_status rms_run(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_run_params *params,
+                   rms_job_out *outs, rms_timing *timing);
+
+/* Pathwise/score-function derivatives (no reference counterpart: the
+ * reference has no gradients).  Recomputes every trajectory from its seed. */
+rms_status rms_plan_adjoint(rms_plan *plan, const rms_job_adj *adj, rms_timing *timing);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

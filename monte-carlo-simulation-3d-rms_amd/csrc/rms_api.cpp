@@ -1,0 +1,775 @@
+// rms_api.cpp -- the C ABI of librms_amd.so (include/rms_amd.h).
+//
+// Host side of the engine: validation, the derived parameters of
+// hpc_code/lib/main.cu:429-458 and :484-487, PGSE lobe tables
+// (main_PGSE.cu:341-345), device buffers, launch sequencing (optionally as a
+// captured hipGraph) and the copies back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/rms_amd.h"
+#include "rms_internal.h"
+#include "rms_types.h"
+
+using namespace rms;
+
+struct GeomDev {
+    rms_geometry meta;     // labels / label_class pointers cleared
+    void *d_labels = nullptr;
+    uint8_t *d_lclass = nullptr;
+    int64_t nvox = 0;
+    bool has_walkable = false;
+};
+
+struct rms_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int ncu = 0;
+    std::string err;
+    std::vector<GeomDev> geoms;
+};
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+struct JobHost {
+    int32_t geom_id;
+    int64_t nwalk, nbatch, batch0, wbase;
+    int32_t nbvec, ngrad, nval, nbound, ntriple;
+    std::vector<int32_t> gperm;
+};
+
+}  // namespace
+
+struct rms_plan {
+    rms_ctx *ctx = nullptr;
+    rms_run_params prm{};
+    std::vector<JobHost> jobs;
+    int64_t nbatch = 0, ws = 0;
+    int sig = 0;
+    int grid = 0, block = 256;
+    int xcd_split = 0;
+    std::vector<void *> allocs;
+    JobDesc *d_jobs = nullptr;
+    int32_t *d_batch_job = nullptr;
+    ReduceDesc *d_red = nullptr;
+    std::vector<ReduceDesc> red;   // host copy (device pointers)
+    int64_t max_nq = 0;
+    unsigned long long *d_counters = nullptr;
+    StateBuf st{};
+    double *d_snap = nullptr;
+    FinalOut fin{};
+    bool fin_alloc = false;
+    std::vector<JobDesc> hjobs;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+};
+
+static const char *kVersion = "rms_amd 0.1 (gfx950)";
+
+#define HIPCHK(ctx, call)                                                                          \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) {                                                                    \
+            (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                        \
+            return e_ == hipErrorOutOfMemory ? RMS_ERR_OUT_OF_MEMORY : RMS_ERR_HIP;               \
+        }                                                                                          \
+    } while (0)
+
+static rms_status fail(rms_ctx *ctx, rms_status s, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    return s;
+}
+
+extern "C" const char *rms_version(void) { return kVersion; }
+
+extern "C" int32_t rms_nval(int32_t nclass, int32_t nbvec) { return 1 + nclass + 6 + 15 + nbvec; }
+
+extern "C" rms_status rms_create(const rms_device_opts *opts, rms_ctx **out)
+{
+    if (!out) return RMS_ERR_INVALID_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return RMS_ERR_NO_DEVICE;
+    int dev = opts ? opts->device : 0;
+    if (dev < 0 || dev >= ndev) return RMS_ERR_INVALID_ARG;
+    rms_ctx *c = new rms_ctx();
+    c->device = dev;
+    if (hipSetDevice(dev) != hipSuccess) {
+        delete c;
+        return RMS_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        delete c;
+        return RMS_ERR_HIP;
+    }
+    c->ncu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return RMS_ERR_HIP;
+    }
+    *out = c;
+    return RMS_OK;
+}
+
+extern "C" rms_status rms_destroy(rms_ctx *ctx)
+{
+    if (!ctx) return RMS_ERR_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    for (auto &g : ctx->geoms) {
+        if (g.d_labels) hipFree(g.d_labels);
+        if (g.d_lclass) hipFree(g.d_lclass);
+    }
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RMS_OK;
+}
+
+extern "C" const char *rms_last_error(const rms_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+// main.cu:429-458 (+ the kappa extension); identical double arithmetic to the
+// host restatement used for parity.
+static bool derive(const rms_geometry &g, double dt, double step[NC], double inv_step[NC], double T2s[NC],
+                   double P[NC * NC], int Pcat[NC * NC], std::string &err)
+{
+    if (g.nclass < 1 || g.nclass > NC) {
+        err = "nclass must be in 1..4";
+        return false;
+    }
+    for (int i = 0; i < NC; i++) {
+        step[i] = inv_step[i] = 0.0;
+        T2s[i] = INFINITY;
+    }
+    for (int i = 0; i < NC * NC; i++) {
+        P[i] = 0.0;
+        Pcat[i] = 0;
+    }
+    for (int i = 0; i < g.nclass; i++) {
+        T2s[i] = g.T2[i] / dt;
+        step[i] = std::sqrt(6.0 * dt * g.D[i]) / g.res;
+        inv_step[i] = (step[i] > 0.0) ? 1.0 / step[i] : 0.0;
+    }
+    for (int i = 0; i < g.nclass; i++) {
+        for (int j = 0; j < g.nclass; j++) {
+            double p;
+            if ((i == 0) || (j == 0)) {
+                p = 0.0;
+            } else if (i == j) {
+                p = 1.0;
+            } else {
+                p = std::fmin(1.0, std::sqrt(g.D[j] / g.D[i]));
+                double kap = g.kappa[i * NC + j];
+                if (kap >= 0.0) {
+                    double smin = std::fmin(step[i], step[j]);
+                    double pi0 = 4.0 * kap * dt / (g.res * smin);
+                    p = p * (pi0 / (1.0 + pi0));
+                }
+            }
+            P[i * NC + j] = p;
+            Pcat[i * NC + j] = (p > 0.999999) ? 1 : ((p < 0.000001) ? 0 : 2);
+        }
+    }
+    return true;
+}
+
+extern "C" rms_status rms_derive(const rms_geometry *geom, double dt, double *step, double *T2s, double *P)
+{
+    if (!geom || !(dt > 0.0)) return RMS_ERR_INVALID_ARG;
+    double st[NC], ist[NC], t2[NC], pp[NC * NC];
+    int pc[NC * NC];
+    std::string err;
+    if (!derive(*geom, dt, st, ist, t2, pp, pc, err)) return RMS_ERR_INVALID_ARG;
+    if (step) std::memcpy(step, st, sizeof(st));
+    if (T2s) std::memcpy(T2s, t2, sizeof(t2));
+    if (P) std::memcpy(P, pp, sizeof(pp));
+    return RMS_OK;
+}
+
+extern "C" rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id)
+{
+    if (!ctx || !geom || !geom_id) return fail(ctx, RMS_ERR_INVALID_ARG, "null argument");
+    const rms_geometry &g = *geom;
+    if (g.nx < 1 || g.ny < 1 || g.nz < 1) return fail(ctx, RMS_ERR_INVALID_ARG, "lattice dimensions must be >= 1");
+    int64_t nvox = (int64_t)g.nx * g.ny * g.nz;
+    if (nvox > 2147483647LL)
+        return fail(ctx, RMS_ERR_INVALID_ARG, "lattice has >= 2^31 voxels (the reference int index overflows too)");
+    if (g.label_bytes != 1 && g.label_bytes != 2) return fail(ctx, RMS_ERR_INVALID_ARG, "label_bytes must be 1 or 2");
+    if (!g.labels) return fail(ctx, RMS_ERR_INVALID_ARG, "labels is null");
+    if (g.nclass < 1 || g.nclass > NC) return fail(ctx, RMS_ERR_INVALID_ARG, "nclass must be in 1..4");
+    if (!(g.res > 0.0)) return fail(ctx, RMS_ERR_INVALID_ARG, "voxel size must be positive");
+    if (g.label_class) {
+        if (g.nlabel < 1) return fail(ctx, RMS_ERR_INVALID_ARG, "nlabel must be >= 1 with a class map");
+        for (int i = 0; i < g.nlabel; i++)
+            if (g.label_class[i] >= g.nclass) return fail(ctx, RMS_ERR_INVALID_ARG, "class map value >= nclass");
+    }
+    // scan labels: range and presence of a walkable voxel
+    uint32_t maxlab = 0;
+    bool walk = false;
+    for (int64_t i = 0; i < nvox; i++) {
+        uint32_t lab = g.label_bytes == 1 ? ((const uint8_t *)g.labels)[i] : ((const uint16_t *)g.labels)[i];
+        if (lab > maxlab) maxlab = lab;
+        uint32_t c = g.label_class ? (lab < (uint32_t)g.nlabel ? g.label_class[lab] : 0u) : lab;
+        if (c != 0) walk = true;
+    }
+    if (g.label_class ? maxlab >= (uint32_t)g.nlabel : maxlab >= (uint32_t)g.nclass)
+        return fail(ctx, RMS_ERR_INVALID_ARG, "label value out of range of the class table");
+    if (!walk)
+        return fail(ctx, RMS_ERR_INVALID_ARG,
+                    "lattice has no non-null voxel (the reference initialisation loop would never end)");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    GeomDev gd;
+    gd.meta = g;
+    gd.meta.labels = nullptr;
+    gd.meta.label_class = nullptr;
+    gd.nvox = nvox;
+    gd.has_walkable = walk;
+    size_t bytes = (size_t)nvox * g.label_bytes;
+    HIPCHK(ctx, hipMalloc(&gd.d_labels, bytes));
+    HIPCHK(ctx, hipMemcpy(gd.d_labels, g.labels, bytes, hipMemcpyHostToDevice));
+    if (g.label_class) {
+        HIPCHK(ctx, hipMalloc((void **)&gd.d_lclass, (size_t)g.nlabel));
+        HIPCHK(ctx, hipMemcpy(gd.d_lclass, g.label_class, (size_t)g.nlabel, hipMemcpyHostToDevice));
+    }
+    ctx->geoms.push_back(gd);
+    *geom_id = (int32_t)ctx->geoms.size() - 1;
+    return RMS_OK;
+}
+
+// ---------------------------------------------------------------- plans
+template <typename T>
+static rms_status palloc(rms_plan *p, T **ptr, size_t bytes)
+{
+    *ptr = nullptr;
+    if (bytes == 0) return RMS_OK;
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) {
+        p->ctx->err = std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? RMS_ERR_OUT_OF_MEMORY : RMS_ERR_HIP;
+    }
+    p->allocs.push_back(q);
+    *ptr = (T *)q;
+    return RMS_OK;
+}
+
+template <typename T>
+static rms_status pupload(rms_plan *p, T **ptr, const T *src, size_t n)
+{
+    rms_status s = palloc(p, ptr, n * sizeof(T));
+    if (s != RMS_OK || n == 0) return s;
+    hipError_t e = hipMemcpy(*ptr, src, n * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        p->ctx->err = std::string("hipMemcpy: ") + hipGetErrorString(e);
+        return RMS_ERR_HIP;
+    }
+    return RMS_OK;
+}
+
+static int64_t first_false_lt(double dt, int64_t TN, double thr)
+{
+    int64_t lo = 0, hi = TN;
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if ((double)(mid + 1) * dt < thr)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+static int64_t first_true_ge(double dt, int64_t TN, double thr)
+{
+    int64_t lo = 0, hi = TN;
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if ((double)(mid + 1) * dt >= thr)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+static void plan_free(rms_plan *p)
+{
+    if (!p) return;
+    hipSetDevice(p->ctx->device);
+    if (p->gexec) hipGraphExecDestroy(p->gexec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    for (auto &e : p->ev)
+        if (e) hipEventDestroy(e);
+    for (void *q : p->allocs) hipFree(q);
+    delete p;
+}
+
+extern "C" rms_status rms_plan_destroy(rms_plan *plan)
+{
+    if (!plan) return RMS_ERR_INVALID_ARG;
+    plan_free(plan);
+    return RMS_OK;
+}
+
+extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_run_params *prm,
+                                      rms_plan **out)
+{
+    if (!ctx || !jobs || njobs < 1 || !prm || !out) return fail(ctx, RMS_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    const rms_run_params &P = *prm;
+    if (!(P.dt > 0.0)) return fail(ctx, RMS_ERR_INVALID_ARG, "dt must be positive");
+    if (P.timepoints < 1) return fail(ctx, RMS_ERR_INVALID_ARG, "timepoints must be >= 1");
+    if (P.TN < P.timepoints)
+        return fail(ctx, RMS_ERR_INVALID_ARG,
+                    "TN must be >= timepoints (the reference divides by TN/timepoints, main.cu:68,333)");
+    if (P.TN > 2147483647LL) return fail(ctx, RMS_ERR_INVALID_ARG, "TN must be < 2^31");
+    if (P.precision != RMS_PREC_F32 && P.precision != RMS_PREC_F64)
+        return fail(ctx, RMS_ERR_INVALID_ARG, "precision must be RMS_PREC_F32 or RMS_PREC_F64");
+    const int R = P.replicas < 1 ? 1 : P.replicas;
+    const int64_t T = P.timepoints;
+    const int64_t Tstep = P.TN / P.timepoints;
+
+    rms_plan *p = new rms_plan();
+    p->ctx = ctx;
+    p->prm = P;
+    p->prm.replicas = R;
+    if (hipSetDevice(ctx->device) != hipSuccess) {
+        delete p;
+        return fail(ctx, RMS_ERR_HIP, "hipSetDevice failed");
+    }
+    rms_status st = RMS_OK;
+#define TRY(x)                   \
+    do {                         \
+        st = (x);                \
+        if (st != RMS_OK) {      \
+            plan_free(p);        \
+            return st;           \
+        }                        \
+    } while (0)
+#define BAD(msg) TRY(fail(ctx, RMS_ERR_INVALID_ARG, msg))
+
+    int sig = 0;
+    int64_t nbatch = 0;
+    int maxbound = 0;
+    p->hjobs.resize(njobs);
+    p->jobs.resize(njobs);
+    std::vector<std::vector<double>> qscale(njobs);
+    for (int j = 0; j < njobs; j++) {
+        const rms_job &J = jobs[j];
+        if (J.geom_id < 0 || J.geom_id >= (int32_t)ctx->geoms.size()) BAD("job " + std::to_string(j) + ": bad geom_id");
+        if (J.nwalk < 1) BAD("job " + std::to_string(j) + ": nwalk must be >= 1");
+        if (J.nbvec < 0 || (J.nbvec > 0 && (!J.bval || !J.bvec))) BAD("job: bad b-table");
+        if (J.ngrad < 0 || (J.ngrad > 0 && !J.grad)) BAD("job: bad gradient table");
+        if (J.nbvec > 0) sig |= 1;
+        if (J.ngrad > 0) sig |= 2;
+    }
+    p->sig = sig;
+    for (int j = 0; j < njobs; j++) {
+        const rms_job &J = jobs[j];
+        const GeomDev &G = ctx->geoms[J.geom_id];
+        JobDesc &D = p->hjobs[j];
+        std::memset(&D, 0, sizeof(D));
+        double step[NC], inv_step[NC], T2s[NC], Pp[NC * NC];
+        int Pcat[NC * NC];
+        std::string err;
+        if (!derive(G.meta, P.dt, step, inv_step, T2s, Pp, Pcat, err)) BAD(err);
+        for (int c = 1; c < G.meta.nclass; c++) {
+            if (!(G.meta.D[c] > 0.0)) BAD("non-null classes need a positive diffusivity");
+            if (!(step[c] < 1.0))
+                BAD("step size sqrt(6*D*dt) must be below one voxel (main.cu:210-214 assumes it)");
+            if (!(G.meta.T2[c] > 0.0)) BAD("T2 must be positive");
+        }
+        D.labels = G.d_labels;
+        D.lclass = G.d_lclass;
+        D.nx = G.meta.nx;
+        D.ny = G.meta.ny;
+        D.nz = G.meta.nz;
+        D.label_bytes = G.meta.label_bytes;
+        D.nclass = G.meta.nclass;
+        D.nbvec = J.nbvec;
+        D.ngrad = J.ngrad;
+        D.nwalk = J.nwalk;
+        D.key0 = (uint32_t)J.seed;
+        D.key1 = (uint32_t)(J.seed >> 32);
+        D.res = G.meta.res;
+        uint32_t pcat = 0;
+        for (int i = 0; i < NC * NC; i++) pcat |= (uint32_t)(Pcat[i] & 3) << (2 * i);
+        D.pcat = pcat;
+        for (int c = 0; c < NC; c++) {
+            D.step_f[c] = (float)step[c];
+            D.inv_step_f[c] = (float)inv_step[c];
+            D.step_d[c] = step[c];
+            D.inv_step_d[c] = inv_step[c];
+            D.T2s[c] = T2s[c];
+        }
+        for (int i = 0; i < NC * NC; i++) {
+            D.P_f[i] = (float)Pp[i];
+            D.P_d[i] = Pp[i];
+        }
+        JobHost &H = p->jobs[j];
+        H.geom_id = J.geom_id;
+        H.nwalk = J.nwalk;
+        H.nbatch = (J.nwalk + WAVE - 1) / WAVE;
+        H.batch0 = nbatch;
+        H.wbase = nbatch * WAVE;
+        H.nbvec = J.nbvec;
+        H.ngrad = J.ngrad;
+        H.nval = 1 + G.meta.nclass + 21 + ((sig & 1) ? J.nbvec : 0);
+        H.nbound = 0;
+        H.ntriple = 0;
+        D.batch0 = H.batch0;
+        D.wbase = H.wbase;
+        nbatch += H.nbatch;
+        // narrow-pulse q scale table, sqrt(bval/TD) (main.cu:375)
+        if (J.nbvec > 0) {
+            for (int b = 0; b < J.nbvec; b++)
+                if (!(J.bval[b] >= 0.0)) BAD("b-values must be >= 0");
+            qscale[j].resize((size_t)T * J.nbvec);
+            for (int64_t r = 0; r < T; r++) {
+                double TD = (double)(r * Tstep + 1) * P.dt;  // main.cu:486
+                for (int b = 0; b < J.nbvec; b++) qscale[j][(size_t)r * J.nbvec + b] = std::sqrt(J.bval[b] / TD);
+            }
+        }
+    }
+    p->nbatch = nbatch;
+    p->ws = nbatch * WAVE;
+
+    // PGSE tables (window sums), per job
+    for (int j = 0; j < njobs; j++) {
+        const rms_job &J = jobs[j];
+        JobDesc &D = p->hjobs[j];
+        JobHost &H = p->jobs[j];
+        if (J.ngrad <= 0) continue;
+        std::vector<std::array<int64_t, 3>> abe(J.ngrad);
+        std::vector<int64_t> bl;
+        bl.push_back(P.TN);
+        for (int g = 0; g < J.ngrad; g++) {
+            const double *gr = J.grad + 6 * g;
+            int64_t a = first_false_lt(P.dt, P.TN, gr[1]);
+            int64_t b = first_true_ge(P.dt, P.TN, gr[0]);
+            int64_t e = first_false_lt(P.dt, P.TN, gr[0] + gr[1]);
+            if (b < a) b = a;
+            if (e < b) e = b;
+            abe[g] = {a, b, e};
+            bl.push_back(a);
+            bl.push_back(b);
+            bl.push_back(e);
+        }
+        std::sort(bl.begin(), bl.end());
+        bl.erase(std::unique(bl.begin(), bl.end()), bl.end());
+        std::vector<int64_t> bounds(1, 0);
+        for (int64_t v : bl)
+            if (v != 0) bounds.push_back(v);
+        if ((int)bounds.size() - 1 > MAX_BOUNDS) BAD("too many distinct PGSE lobe boundaries (max 64)");
+        H.nbound = (int)bounds.size() - 1;
+        maxbound = std::max(maxbound, H.nbound);
+        auto kidx = [&](int64_t v) {
+            return (int32_t)(std::lower_bound(bounds.begin(), bounds.end(), v) - bounds.begin());
+        };
+        std::vector<std::tuple<int32_t, int32_t, int32_t, int32_t>> order;
+        for (int g = 0; g < J.ngrad; g++) order.emplace_back(kidx(abe[g][0]), kidx(abe[g][1]), kidx(abe[g][2]), g);
+        std::stable_sort(order.begin(), order.end());
+        std::vector<double> gs((size_t)J.ngrad * 6);
+        std::vector<int32_t> perm(J.ngrad), trip;
+        for (int g = 0; g < J.ngrad; g++) {
+            int src = std::get<3>(order[g]);
+            perm[g] = src;
+            std::memcpy(&gs[(size_t)g * 6], J.grad + 6 * src, 6 * sizeof(double));
+            bool newt = g == 0 || std::get<0>(order[g]) != std::get<0>(order[g - 1]) ||
+                        std::get<1>(order[g]) != std::get<1>(order[g - 1]) ||
+                        std::get<2>(order[g]) != std::get<2>(order[g - 1]);
+            if (newt) {
+                trip.push_back(std::get<0>(order[g]));
+                trip.push_back(std::get<1>(order[g]));
+                trip.push_back(std::get<2>(order[g]));
+                trip.push_back(g);
+            }
+        }
+        H.ntriple = (int)trip.size() / 4;
+        trip.push_back(0);
+        trip.push_back(0);
+        trip.push_back(0);
+        trip.push_back(J.ngrad);  // sentinel
+        H.gperm = perm;
+        D.ntriple = H.ntriple;
+        D.nbound = H.nbound;
+        double *dg;
+        int32_t *dp, *dt3;
+        int64_t *db;
+        TRY(pupload(p, &dg, gs.data(), gs.size()));
+        TRY(pupload(p, &dp, perm.data(), perm.size()));
+        TRY(pupload(p, &dt3, trip.data(), trip.size()));
+        TRY(pupload(p, &db, bounds.data(), bounds.size()));
+        D.grad = dg;
+        D.gperm = dp;
+        D.triple = dt3;
+        D.bounds = db;
+    }
+    for (int j = 0; j < njobs; j++) {
+        JobDesc &D = p->hjobs[j];
+        if (jobs[j].nbvec > 0) {
+            double *dq, *dbv;
+            TRY(pupload(p, &dq, qscale[j].data(), qscale[j].size()));
+            TRY(pupload(p, &dbv, jobs[j].bvec, (size_t)jobs[j].nbvec * 3));
+            D.qscale = dq;
+            D.bvec = dbv;
+        }
+    }
+    // outputs: per-batch partials, replicas, totals
+    TRY(palloc(p, &p->d_counters, sizeof(unsigned long long) * 5 * njobs));
+    for (int j = 0; j < njobs; j++) {
+        JobDesc &D = p->hjobs[j];
+        JobHost &H = p->jobs[j];
+        D.counters = p->d_counters + 5 * j;
+        double *part, *rep, *tot;
+        int64_t nq = T * H.nval;
+        TRY(palloc(p, &part, sizeof(double) * H.nbatch * nq));
+        TRY(palloc(p, &rep, sizeof(double) * R * nq));
+        TRY(palloc(p, &tot, sizeof(double) * nq));
+        D.part = part;
+        p->red.push_back(ReduceDesc{part, rep, tot, H.nbatch, nq});
+        p->max_nq = std::max(p->max_nq, nq);
+        if (H.ngrad > 0) {
+            double *ps, *sr, *stt;
+            TRY(palloc(p, &ps, sizeof(double) * H.nbatch * H.ngrad));
+            TRY(palloc(p, &sr, sizeof(double) * R * H.ngrad));
+            TRY(palloc(p, &stt, sizeof(double) * H.ngrad));
+            D.psig = ps;
+            p->red.push_back(ReduceDesc{ps, sr, stt, H.nbatch, H.ngrad});
+            p->max_nq = std::max(p->max_nq, (int64_t)H.ngrad);
+        }
+    }
+    if (sig & 2) {
+        TRY(palloc(p, &p->d_snap, sizeof(double) * (size_t)(maxbound + 1) * 3 * p->ws));
+        for (int j = 0; j < njobs; j++) p->hjobs[j].snap = p->d_snap;
+    }
+    // chunked launches keep walker state in HBM between launches
+    const bool chunked = P.chunk_steps > 0 && P.chunk_steps < P.TN;
+    p->st.ws = p->ws;
+    if (chunked) {
+        const int64_t ws = p->ws;
+        TRY(palloc(p, &p->st.g, sizeof(int32_t) * 3 * ws));
+        TRY(palloc(p, &p->st.gi, sizeof(int32_t) * 3 * ws));
+        TRY(palloc(p, &p->st.c, sizeof(int32_t) * ws));
+        if (P.precision == RMS_PREC_F32) {
+            TRY(palloc(p, &p->st.of, sizeof(float) * 6 * ws));
+            TRY(palloc(p, &p->st.tcf, sizeof(float) * ws));
+        } else {
+            TRY(palloc(p, &p->st.od, sizeof(double) * 6 * ws));
+            TRY(palloc(p, &p->st.tcd, sizeof(double) * ws));
+        }
+        TRY(palloc(p, &p->st.tsum, sizeof(double) * NC * ws));
+        TRY(palloc(p, &p->st.cnt, sizeof(uint32_t) * 5 * ws));
+        if (sig & 2) TRY(palloc(p, &p->st.ps, sizeof(double) * 3 * ws));
+    }
+    // batch -> job table
+    std::vector<int32_t> bj(nbatch);
+    for (int j = 0; j < njobs; j++)
+        for (int64_t b = 0; b < p->jobs[j].nbatch; b++) bj[p->jobs[j].batch0 + b] = j;
+    TRY(pupload(p, &p->d_batch_job, bj.data(), bj.size()));
+    TRY(pupload(p, &p->d_jobs, p->hjobs.data(), p->hjobs.size()));
+    TRY(pupload(p, &p->d_red, p->red.data(), p->red.size()));
+
+    // launch shape: persistent grid at the resident limit
+    p->block = P.block_threads > 0 ? P.block_threads : 256;
+    if (p->block % WAVE != 0 || p->block > 256) BAD("block_threads must be a multiple of 64 and <= 256");
+    int bpc = 0;
+    if (walk_occupancy(P.precision == RMS_PREC_F64, sig, p->block, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+    const int wpb = p->block / WAVE;
+    int64_t need = (nbatch + wpb - 1) / wpb;
+    int64_t resident = (int64_t)bpc * ctx->ncu;
+    int64_t grid = P.grid_blocks > 0 ? P.grid_blocks : std::min(need, resident);
+    if (grid < 1) grid = 1;
+    int xs = P.xcd_split;
+    bool split = xs > 0 || (xs == 0 && njobs > 1);
+    if (split && grid >= 8) {
+        grid = (grid / 8) * 8;
+        p->xcd_split = 1;
+    } else {
+        p->xcd_split = 0;
+    }
+    p->grid = (int)grid;
+    for (auto &e : p->ev) {
+        if (hipEventCreate(&e) != hipSuccess) TRY(fail(ctx, RMS_ERR_HIP, "hipEventCreate failed"));
+    }
+    *out = p;
+    return RMS_OK;
+#undef TRY
+#undef BAD
+}
+
+static LaunchArgs make_args(rms_plan *p, int64_t s0, int64_t s1)
+{
+    LaunchArgs a{};
+    a.jobs = p->d_jobs;
+    a.batch_job = p->d_batch_job;
+    a.nbatch = p->nbatch;
+    a.TN = p->prm.TN;
+    a.step0 = s0;
+    a.step1 = s1;
+    a.Tstep = p->prm.TN / p->prm.timepoints;
+    a.timepoints = p->prm.timepoints;
+    a.dt = p->prm.dt;
+    a.st = p->st;
+    a.fin = p->fin;
+    a.xcd_split = p->xcd_split;
+    return a;
+}
+
+// Enqueue the whole run: zero counters, walk chunks, ordered reductions.
+static rms_status enqueue(rms_plan *p, bool with_events)
+{
+    rms_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    HIPCHK(ctx, launch_fill_zero(p->d_counters, sizeof(unsigned long long) * 5 * (int64_t)p->jobs.size(), s));
+    if (with_events) HIPCHK(ctx, hipEventRecord(p->ev[0], s));
+    const int64_t TN = p->prm.TN;
+    const int64_t ch = (p->prm.chunk_steps > 0 && p->prm.chunk_steps < TN) ? p->prm.chunk_steps : TN;
+    for (int64_t s0 = 0; s0 < TN; s0 += ch) {
+        int64_t s1 = std::min(TN, s0 + ch);
+        LaunchArgs a = make_args(p, s0, s1);
+        HIPCHK(ctx, launch_walk(a, p->prm.precision == RMS_PREC_F64, p->sig, p->grid, p->block, s));
+    }
+    if (with_events) HIPCHK(ctx, hipEventRecord(p->ev[1], s));
+    HIPCHK(ctx, launch_reduce(p->d_red, (int32_t)p->red.size(), p->max_nq, p->prm.replicas, s));
+    if (with_events) HIPCHK(ctx, hipEventRecord(p->ev[2], s));
+    return RMS_OK;
+}
+
+extern "C" rms_status rms_plan_execute(rms_plan *p, rms_job_out *outs, rms_timing *timing)
+{
+    if (!p) return RMS_ERR_INVALID_ARG;
+    rms_ctx *ctx = p->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int nj = (int)p->jobs.size();
+    // final-state outputs requested?
+    bool want_fin = false;
+    if (outs)
+        for (int j = 0; j < nj; j++) want_fin = want_fin || outs[j].fin_g;
+    if (want_fin && !p->fin_alloc) {
+        const int64_t ws = p->ws;
+        rms_status st;
+        if ((st = palloc(p, &p->fin.g, sizeof(int32_t) * 3 * ws)) != RMS_OK) return st;
+        if ((st = palloc(p, &p->fin.o, sizeof(double) * 3 * ws)) != RMS_OK) return st;
+        if ((st = palloc(p, &p->fin.gi, sizeof(int32_t) * 3 * ws)) != RMS_OK) return st;
+        if ((st = palloc(p, &p->fin.oi, sizeof(double) * 3 * ws)) != RMS_OK) return st;
+        if ((st = palloc(p, &p->fin.c, sizeof(int32_t) * ws)) != RMS_OK) return st;
+        if ((st = palloc(p, &p->fin.cnt, sizeof(uint32_t) * 5 * ws)) != RMS_OK) return st;
+        p->fin_alloc = true;
+        if (p->gexec) {  // the captured graph does not write the final state
+            hipGraphExecDestroy(p->gexec);
+            hipGraphDestroy(p->graph);
+            p->gexec = nullptr;
+            p->graph = nullptr;
+        }
+    }
+    hipStream_t s = ctx->stream;
+    if (p->prm.use_graph) {
+        if (!p->gexec) {
+            HIPCHK(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            rms_status st = enqueue(p, false);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(s, &g);
+            if (st != RMS_OK) {
+                if (g) hipGraphDestroy(g);
+                return st;
+            }
+            HIPCHK(ctx, e);
+            p->graph = g;
+            HIPCHK(ctx, hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+        }
+        HIPCHK(ctx, hipEventRecord(p->ev[0], s));
+        HIPCHK(ctx, hipGraphLaunch(p->gexec, s));
+        HIPCHK(ctx, hipEventRecord(p->ev[2], s));
+    } else {
+        rms_status st = enqueue(p, true);
+        if (st != RMS_OK) return st;
+    }
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    if (timing) {
+        std::memset(timing, 0, sizeof(*timing));
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, p->ev[0], p->ev[2]);
+        timing->total_ms = ms;
+        if (!p->prm.use_graph) {
+            hipEventElapsedTime(&ms, p->ev[0], p->ev[1]);
+            timing->walk_ms = ms;
+            hipEventElapsedTime(&ms, p->ev[1], p->ev[2]);
+            timing->reduce_ms = ms;
+        } else {
+            timing->walk_ms = timing->total_ms;
+            timing->reduce_ms = 0.0;
+        }
+        timing->grid = p->grid;
+        timing->block = p->block;
+        const int64_t TN = p->prm.TN;
+        const int64_t ch = (p->prm.chunk_steps > 0 && p->prm.chunk_steps < TN) ? p->prm.chunk_steps : TN;
+        timing->launches = (int32_t)((TN + ch - 1) / ch);
+    }
+    if (!outs) return RMS_OK;
+    const int R = p->prm.replicas;
+    std::vector<unsigned long long> cnt(5 * nj);
+    HIPCHK(ctx, hipMemcpy(cnt.data(), p->d_counters, sizeof(unsigned long long) * 5 * nj, hipMemcpyDeviceToHost));
+    size_t ri = 0;
+    for (int j = 0; j < nj; j++) {
+        const JobHost &H = p->jobs[j];
+        rms_job_out &O = outs[j];
+        const ReduceDesc &rd = p->red[ri++];
+        if (O.rec) HIPCHK(ctx, hipMemcpy(O.rec, rd.tot, sizeof(double) * rd.nq, hipMemcpyDeviceToHost));
+        if (O.rec_rep) HIPCHK(ctx, hipMemcpy(O.rec_rep, rd.rep, sizeof(double) * rd.nq * R, hipMemcpyDeviceToHost));
+        if (H.ngrad > 0) {
+            const ReduceDesc &rs = p->red[ri++];
+            if (O.sig) HIPCHK(ctx, hipMemcpy(O.sig, rs.tot, sizeof(double) * rs.nq, hipMemcpyDeviceToHost));
+            if (O.sig_rep)
+                HIPCHK(ctx, hipMemcpy(O.sig_rep, rs.rep, sizeof(double) * rs.nq * R, hipMemcpyDeviceToHost));
+        }
+        for (int k = 0; k < 5; k++) O.counters[k] = cnt[5 * j + k];
+        if (O.fin_g && p->fin_alloc) {
+            const int64_t ws = p->ws, n = H.nwalk, b = H.wbase;
+            for (int k = 0; k < 3; k++) {
+                HIPCHK(ctx, hipMemcpy(O.fin_g + k * n, p->fin.g + k * ws + b, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+                HIPCHK(ctx, hipMemcpy(O.fin_o + k * n, p->fin.o + k * ws + b, sizeof(double) * n, hipMemcpyDeviceToHost));
+                HIPCHK(ctx, hipMemcpy(O.fin_gi + k * n, p->fin.gi + k * ws + b, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+                HIPCHK(ctx, hipMemcpy(O.fin_oi + k * n, p->fin.oi + k * ws + b, sizeof(double) * n, hipMemcpyDeviceToHost));
+            }
+            HIPCHK(ctx, hipMemcpy(O.fin_c, p->fin.c + b, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+            for (int k = 0; k < 5; k++)
+                HIPCHK(ctx, hipMemcpy(O.fin_cnt + k * n, p->fin.cnt + k * ws + b, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+        }
+    }
+    return RMS_OK;
+}
+
+extern "C" rms_status rms_run(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_run_params *params,
+                              rms_job_out *outs, rms_timing *timing)
+{
+    rms_plan *p = nullptr;
+    rms_status s = rms_plan_create(ctx, jobs, njobs, params, &p);
+    if (s != RMS_OK) return s;
+    s = rms_plan_execute(p, outs, timing);
+    rms_plan_destroy(p);
+    return s;
+}
+
+extern "C" rms_status rms_plan_adjoint(rms_plan *plan, const rms_job_adj *adj, rms_timing *timing)
+{
+    (void)adj;
+    (void)timing;
+    if (!plan) return RMS_ERR_INVALID_ARG;
+    return fail(plan->ctx, RMS_ERR_UNSUPPORTED, "adjoint pass not built yet");
+}

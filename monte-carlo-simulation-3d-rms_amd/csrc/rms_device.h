@@ -1,0 +1,199 @@
+// rms_device.h -- device-side random numbers and elementary functions for the
+// RMS walker kernels (gfx950).
+//
+// Every routine here is a fixed sequence of IEEE-754 basic operations (add,
+// mul, fma, floor, correctly rounded div/sqrt) so that the host restatement
+// reproduces it bit for bit.  The kernels are compiled with
+// -ffp-contract=off: no a*b+c is fused unless written as fma().
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rms {
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel Random Numbers: As Easy
+// as 1, 2, 3", SC'11).  Replaces the per-thread cuRAND state of
+// hpc_code/lib/main.cu:58-61,66: counters are (index, stream, walker lo, hi).
+struct U4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                     uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t lo0 = 0xD2511F53u * c0;
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// Streams of the counter layout.
+enum : uint32_t { STREAM_DIR = 0, STREAM_EXCH_A = 1, STREAM_EXCH_B = 2, STREAM_INIT_XY = 3, STREAM_INIT_Z = 4 };
+
+__device__ __forceinline__ U4 rng_block(uint32_t key0, uint32_t key1, uint32_t idx, uint32_t stream, uint64_t wid)
+{
+    return philox(idx, stream, (uint32_t)wid, (uint32_t)(wid >> 32), key0, key1);
+}
+
+// Open-interval uniforms: (2k+1)*2^-53 with a 52-bit k, (2k+1)*2^-24 with 23 bits.
+__device__ __forceinline__ double u01_f64(uint32_t a, uint32_t b)
+{
+    double hi = (double)(a >> 6);
+    double lo = (double)(b >> 6);
+    double k = hi * 67108864.0 + lo;
+    return (2.0 * k + 1.0) * 1.1102230246251565e-16;
+}
+
+__device__ __forceinline__ float u01_f32(uint32_t w)
+{
+    float k = (float)(w >> 9);
+    return (2.0f * k + 1.0f) * 5.9604644775390625e-08f;
+}
+
+// ---------------------------------------------------------------- sin / cos
+// Taylor polynomials on |x| <= pi/4 (terms to x^16 in FP64, x^10 in FP32).
+__device__ __forceinline__ void sc_poly(double x, double &s, double &c)
+{
+    double z = x * x;
+    double p = -7.647163731819816e-13;
+    p = __builtin_fma(p, z, 1.6059043836821613e-10);
+    p = __builtin_fma(p, z, -2.505210838544172e-08);
+    p = __builtin_fma(p, z, 2.7557319223985893e-06);
+    p = __builtin_fma(p, z, -0.0001984126984126984);
+    p = __builtin_fma(p, z, 0.008333333333333333);
+    p = __builtin_fma(p, z, -0.16666666666666666);
+    s = __builtin_fma(x * z, p, x);
+    double q = 4.779477332387385e-14;
+    q = __builtin_fma(q, z, -1.1470745597729725e-11);
+    q = __builtin_fma(q, z, 2.08767569878681e-09);
+    q = __builtin_fma(q, z, -2.755731922398589e-07);
+    q = __builtin_fma(q, z, 2.48015873015873e-05);
+    q = __builtin_fma(q, z, -0.001388888888888889);
+    q = __builtin_fma(q, z, 0.041666666666666664);
+    q = __builtin_fma(q, z, -0.5);
+    c = __builtin_fma(z, q, 1.0);
+}
+
+__device__ __forceinline__ void sc_poly(float x, float &s, float &c)
+{
+    float z = x * x;
+    float p = 2.7557318844628753e-06f;
+    p = __builtin_fmaf(p, z, -0.00019841270113829523f);
+    p = __builtin_fmaf(p, z, 0.008333333767950535f);
+    p = __builtin_fmaf(p, z, -0.1666666716337204f);
+    s = __builtin_fmaf(x * z, p, x);
+    float q = -2.755731998149713e-07f;
+    q = __builtin_fmaf(q, z, 2.4801587642286904e-05f);
+    q = __builtin_fmaf(q, z, -0.0013888889225199819f);
+    q = __builtin_fmaf(q, z, 0.0416666679084301f);
+    q = __builtin_fmaf(q, z, -0.5f);
+    c = __builtin_fmaf(z, q, 1.0f);
+}
+
+template <typename R>
+__device__ __forceinline__ void quadrant(int q, R s, R c, R &so, R &co)
+{
+    q &= 3;
+    R a = (q & 1) ? c : s;   // sin(x + q*pi/2)
+    R b = (q & 1) ? s : c;   // cos(x + q*pi/2) up to sign
+    so = (q >= 2) ? -a : a;
+    co = (q == 1 || q == 2) ? -b : b;
+}
+
+// sin(2*pi*t), cos(2*pi*t), t in [0,1].
+__device__ __forceinline__ void sincos_turns(double t, double &s, double &c)
+{
+    double k = __builtin_floor(4.0 * t + 0.5);
+    double r = t - k * 0.25;
+    double x = r * 6.283185307179586;
+    double sx, cx;
+    sc_poly(x, sx, cx);
+    quadrant<double>((int)k, sx, cx, s, c);
+}
+
+__device__ __forceinline__ void sincos_turns(float t, float &s, float &c)
+{
+    float k = __builtin_floorf(4.0f * t + 0.5f);
+    float r = t - k * 0.25f;
+    float x = r * 6.2831854820251465f;
+    float sx, cx;
+    sc_poly(x, sx, cx);
+    quadrant<float>((int)k, sx, cx, s, c);
+}
+
+// exp(x), x <= 0: Cody-Waite reduction by ln 2, Taylor to r^13.
+__device__ __forceinline__ double exp_neg(double x)
+{
+    if (!(x > -708.0)) return 0.0;
+    double k = __builtin_floor(x * 1.4426950408889634 + 0.5);
+    double r = __builtin_fma(-k, 0.6931471803691238, x);
+    r = __builtin_fma(-k, 1.9082149292705877e-10, r);
+    double p = 1.6059043836821613e-10;
+    p = __builtin_fma(p, r, 2.08767569878681e-09);
+    p = __builtin_fma(p, r, 2.505210838544172e-08);
+    p = __builtin_fma(p, r, 2.755731922398589e-07);
+    p = __builtin_fma(p, r, 2.7557319223985893e-06);
+    p = __builtin_fma(p, r, 2.48015873015873e-05);
+    p = __builtin_fma(p, r, 0.0001984126984126984);
+    p = __builtin_fma(p, r, 0.001388888888888889);
+    p = __builtin_fma(p, r, 0.008333333333333333);
+    p = __builtin_fma(p, r, 0.041666666666666664);
+    p = __builtin_fma(p, r, 0.16666666666666666);
+    p = __builtin_fma(p, r, 0.5);
+    p = __builtin_fma(p, r, 1.0);
+    p = __builtin_fma(p, r, 1.0);
+    long long e = (long long)k + 1023;
+    double scale = __longlong_as_double(e << 52);
+    return p * scale;
+}
+
+// sin and cos of an arbitrary argument: 3-part Cody-Waite reduction by pi/2.
+__device__ __forceinline__ void sincos_any(double x, double &s, double &c)
+{
+    double k = __builtin_floor(x * 0.6366197723675814 + 0.5);
+    double r = __builtin_fma(-k, 1.5707963267341256, x);
+    r = __builtin_fma(-k, 6.077100506303966e-11, r);
+    r = __builtin_fma(-k, 2.0222662487959506e-21, r);
+    double sx, cx;
+    sc_poly(r, sx, cx);
+    quadrant<double>((int)(((long long)k) & 3), sx, cx, s, c);
+}
+
+__device__ __forceinline__ double cos_any(double x)
+{
+    double s, c;
+    sincos_any(x, s, c);
+    return c;
+}
+
+// ---------------------------------------------------------------- wave sums
+// Butterfly over the 64 lanes: every lane ends with the same pairwise tree
+// ((x0+x1)+(x2+x3))+... (a+b == b+a bit for bit), which is the order the host
+// restatement sums a batch in.
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) x = x + __shfl_xor(x, m, 64);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x)
+{
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) x = x + (uint32_t)__shfl_xor((int)x, m, 64);
+    return x;
+}
+
+}  // namespace rms

@@ -1,0 +1,29 @@
+// rms_internal.h -- kernel launch interface used by the host library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rms_types.h"
+
+namespace rms {
+
+// One ordered reduction: part[nb][nq] -> rep[R][nq] -> tot[nq].
+struct ReduceDesc {
+    const double *part;
+    double *rep;
+    double *tot;
+    int64_t nb;
+    int64_t nq;
+};
+
+hipError_t launch_walk(const LaunchArgs &a, int prec_f64, int sig, int grid, int block, hipStream_t s);
+hipError_t walk_occupancy(int prec_f64, int sig, int block, int *blocks_per_cu);
+hipError_t launch_reduce(const ReduceDesc *rd_dev, int32_t nred, int64_t max_nq, int32_t R, hipStream_t s);
+hipError_t launch_fill_zero(void *p, int64_t bytes, hipStream_t s);
+
+// Tangent (adjoint) pass: same walk with forward-mode tangents, contracted
+// with the record/signal cotangents per walker.
+hipError_t launch_adjoint(const LaunchArgs &a, int prec_f64, int sig, int grid, int block, hipStream_t s);
+hipError_t adjoint_occupancy(int prec_f64, int sig, int block, int *blocks_per_cu);
+
+}  // namespace rms

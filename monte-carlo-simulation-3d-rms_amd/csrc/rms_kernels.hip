@@ -1,0 +1,726 @@
+// rms_kernels.hip -- the RMS random-walk engine for gfx950 (MI355X).
+//
+// One lane = one walker (hpc_code/lib/main.cu:64-70), but the grid is
+// persistent: each wavefront walks batches of 64 walkers taken from a batch
+// list that can span many geometries (jobs).  Walker state lives in VGPRs for
+// the whole chunk of steps; the lattice is read as uint8/uint16 labels through
+// L2 / Infinity Cache.  Per-record moments are summed across the wavefront
+// with a fixed butterfly and written as per-batch partials; a second kernel
+// sums the partials in a fixed order, so results are bit-reproducible and
+// identical to the host restatement.
+//
+// Reference anchors:
+//   propagate (narrow pulse)  main.cu:63-384
+//   propagate (PGSE)          main_PGSE.cu:65-410
+//   atomAdd CAS loop          main.cu:40-56   -> butterfly + ordered partials
+//   setup_kernel              main.cu:58-61   -> counter-based Philox, no state
+#include <hip/hip_runtime.h>
+
+#include "rms_device.h"
+#include "rms_internal.h"
+#include "rms_types.h"
+
+namespace rms {
+
+// Axis-indexed triples with named fields: code indexes them only through
+// compile-time axes (for3) or value selects (pick), so walker state never
+// becomes an addressable array and stays in VGPRs.
+template <typename T>
+struct V3 {
+    T x, y, z;
+};
+template <int K, typename T>
+__device__ __forceinline__ T &at(V3<T> &v)
+{
+    if constexpr (K == 0)
+        return v.x;
+    else if constexpr (K == 1)
+        return v.y;
+    else
+        return v.z;
+}
+template <typename T>
+__device__ __forceinline__ T pick(T a, T b, T c, int k)
+{
+    return k == 0 ? a : (k == 1 ? b : c);
+}
+template <int K>
+struct Ax {
+    static constexpr int value = K;
+};
+template <typename F>
+__device__ __forceinline__ void for3(F &&f)
+{
+    f(Ax<0>{});
+    f(Ax<1>{});
+    f(Ax<2>{});
+}
+
+template <typename R>
+struct Walker {
+    V3<int32_t> g;   // voxel
+    V3<R> o;         // offset inside the voxel, [0,1]
+    V3<int32_t> gi;  // origin voxel (mirror-unfolded)
+    V3<R> oi;        // origin offset
+    int32_t c;       // class
+    int32_t lin;     // linear voxel index of g
+    R tcur;          // steps spent in class c since the last flush
+    double ts0, ts1, ts2, ts3;  // steps per class
+    uint32_t ncross, nrefl, nmirror, ntrunc, ndraw;
+};
+
+// Per-job constants held in registers for the whole batch.
+template <typename R>
+struct JobLocal {
+    const void *labels;
+    const uint8_t *lclass;
+    int32_t nx, ny, nz, sx;  // sx = ny*nz
+    int32_t label_bytes, nclass;
+    uint32_t key0, key1, pcat;
+    R st0, st1, st2, st3;    // step per class (voxels)
+    R is0, is1, is2, is3;    // 1/step
+    double T0, T1, T2, T3;   // T2 in steps
+    double res;
+    const R *P;              // permeation probabilities [NC*NC]
+};
+
+template <typename R>
+__device__ __forceinline__ R sel4(int c, R a0, R a1, R a2, R a3)
+{
+    return c == 0 ? a0 : (c == 1 ? a1 : (c == 2 ? a2 : a3));
+}
+
+template <typename R>
+__device__ __forceinline__ JobLocal<R> job_local(const JobDesc &J)
+{
+    JobLocal<R> L;
+    L.labels = J.labels;
+    L.lclass = J.lclass;
+    L.nx = J.nx;
+    L.ny = J.ny;
+    L.nz = J.nz;
+    L.sx = J.ny * J.nz;
+    L.label_bytes = J.label_bytes;
+    L.nclass = J.nclass;
+    L.key0 = J.key0;
+    L.key1 = J.key1;
+    L.pcat = J.pcat;
+    if constexpr (sizeof(R) == 4) {
+        L.st0 = J.step_f[0];
+        L.st1 = J.step_f[1];
+        L.st2 = J.step_f[2];
+        L.st3 = J.step_f[3];
+        L.is0 = J.inv_step_f[0];
+        L.is1 = J.inv_step_f[1];
+        L.is2 = J.inv_step_f[2];
+        L.is3 = J.inv_step_f[3];
+        L.P = J.P_f;
+    } else {
+        L.st0 = J.step_d[0];
+        L.st1 = J.step_d[1];
+        L.st2 = J.step_d[2];
+        L.st3 = J.step_d[3];
+        L.is0 = J.inv_step_d[0];
+        L.is1 = J.inv_step_d[1];
+        L.is2 = J.inv_step_d[2];
+        L.is3 = J.inv_step_d[3];
+        L.P = J.P_d;
+    }
+    L.T0 = J.T2s[0];
+    L.T1 = J.T2s[1];
+    L.T2 = J.T2s[2];
+    L.T3 = J.T2s[3];
+    L.res = J.res;
+    return L;
+}
+
+template <typename R>
+struct PiRef;
+template <>
+struct PiRef<float> {
+    static constexpr float v = 1.0f;
+};
+template <>
+struct PiRef<double> {
+    static constexpr double v = 0.9999999988573335;  // Pi = 3.14159265 (main.cu:34) over pi
+};
+
+template <typename R>
+__device__ __forceinline__ int32_t class_at(const JobLocal<R> &L, int32_t lin)
+{
+    uint32_t lab = (L.label_bytes == 1) ? (uint32_t)((const uint8_t *)L.labels)[lin]
+                                        : (uint32_t)((const uint16_t *)L.labels)[lin];
+    return L.lclass ? (int32_t)L.lclass[lab] : (int32_t)lab;
+}
+
+// Move the time spent in the current class into its FP64 total.  Every class
+// total is rewritten (x + 0.0 == x for the non-negative totals) so no store
+// goes through a data-dependent address.
+template <typename R>
+__device__ __forceinline__ void flush(Walker<R> &w)
+{
+    const double t = (double)w.tcur;
+    w.ts0 = w.ts0 + (w.c == 0 ? t : 0.0);
+    w.ts1 = w.ts1 + (w.c == 1 ? t : 0.0);
+    w.ts2 = w.ts2 + (w.c == 2 ? t : 0.0);
+    w.ts3 = w.ts3 + (w.c == 3 ? t : 0.0);
+    w.tcur = (R)0;
+}
+
+// Rejection-sampled start voxel, main.cu:117-132 (uniforms over (0,1)).
+template <typename R>
+__device__ __forceinline__ void init_walker(Walker<R> &w, const JobLocal<R> &L, uint64_t wid)
+{
+    const double nx = (double)L.nx, ny = (double)L.ny, nz = (double)L.nz;
+    w.g = {0, 0, 0};
+    w.o = {(R)0.5, (R)0.5, (R)0.5};
+    w.c = 0;
+    w.lin = 0;
+    for (uint32_t n = 0; n < INIT_TRIALS; n++) {
+        U4 a = rng_block(L.key0, L.key1, n, STREAM_INIT_XY, wid);
+        U4 b = rng_block(L.key0, L.key1, n, STREAM_INIT_Z, wid);
+        double x0 = u01_f64(a.x, a.y) * nx;
+        double x1 = u01_f64(a.z, a.w) * ny;
+        double x2 = u01_f64(b.x, b.y) * nz;
+        int32_t g0 = (int32_t)__builtin_floor(x0);
+        int32_t g1 = (int32_t)__builtin_floor(x1);
+        int32_t g2 = (int32_t)__builtin_floor(x2);
+        g0 = g0 > L.nx - 1 ? L.nx - 1 : g0;
+        g1 = g1 > L.ny - 1 ? L.ny - 1 : g1;
+        g2 = g2 > L.nz - 1 ? L.nz - 1 : g2;
+        int32_t lin = g0 * L.sx + g1 * L.nz + g2;
+        int32_t c = class_at(L, lin);
+        if (c != 0) {
+            w.g = {g0, g1, g2};
+            w.o = {(R)(x0 - (double)g0), (R)(x1 - (double)g1), (R)(x2 - (double)g2)};
+            w.c = c;
+            w.lin = lin;
+            break;
+        }
+    }
+    // lattices without a non-null voxel are rejected on the host
+    w.gi = w.g;
+    w.oi = w.o;
+}
+
+template <typename R>
+__device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, int64_t i, int j, U4 &blk)
+{
+    if constexpr (sizeof(R) == 4) {
+        if (j == 0) blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A, wid);
+        uint32_t word = j == 0 ? blk.x : (j == 1 ? blk.y : (j == 2 ? blk.z : blk.w));
+        return u01_f32(word);
+    } else {
+        if ((j & 1) == 0) blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A + (uint32_t)(j >> 1), wid);
+        return (j & 1) ? u01_f64(blk.z, blk.w) : u01_f64(blk.x, blk.y);
+    }
+}
+
+// One step i of the walk: direction (main.cu:140-151) and face march
+// (main.cu:153-300).  `dblk` is the cached direction block (f32 layout:
+// one Philox block serves steps 2m and 2m+1).
+template <typename R>
+__device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, uint64_t wid, int64_t i,
+                                            bool fresh_block, U4 &dblk)
+{
+    R u1, u2;
+    if constexpr (sizeof(R) == 4) {
+        if (fresh_block) dblk = rng_block(L.key0, L.key1, (uint32_t)(i >> 1), STREAM_DIR, wid);
+        const bool odd = (i & 1) != 0;
+        u1 = u01_f32(odd ? dblk.z : dblk.x);
+        u2 = u01_f32(odd ? dblk.w : dblk.y);
+    } else {
+        U4 r = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_DIR, wid);
+        u1 = u01_f64(r.x, r.y);
+        u2 = u01_f64(r.z, r.w);
+    }
+    const R cth = (R)1 - (R)2 * u1;
+    const R sth = (R)2 * sqrt(u1 * ((R)1 - u1));
+    R sp, cp;
+    sincos_turns(u2 * PiRef<R>::v, sp, cp);
+    V3<R> v = {sth * cp, sth * sp, cth};
+    V3<R> iv = {(R)1 / v.x, (R)1 / v.y, (R)1 / v.z};
+    R f = (R)1;
+    int nd = 0;
+    U4 eblk = {0, 0, 0, 0};
+    for (int j = 0; (j < NITE) && (f > (R)0); j++) {
+        // nearest face within one unit of path, main.cu:199-219
+        R tmin = (R)2;
+        int hit = -1;
+        for3([&](auto K) __attribute__((always_inline)) {
+            constexpr int k = decltype(K)::value;
+            const R vk = at<k>(v);
+            const R ok = at<k>(w.o);
+            const R tmp = vk > (R)0 ? ((R)1 - ok) : (vk < (R)0 ? -ok : (R)2);
+            R q = tmp * at<k>(iv);
+            q = (q > (R)0) ? q : (R)0;
+            const R tt = (fabs(tmp) > fabs(vk)) ? (R)2 : q;
+            if (tt < tmin) {
+                tmin = tt;
+                hit = k;
+            }
+        });
+        const R sc = sel4(w.c, L.st0, L.st1, L.st2, L.st3);
+        if (f * sc >= tmin) {
+            // reach the face, main.cu:231-236
+            const R fr = tmin * sel4(w.c, L.is0, L.is1, L.is2, L.is3);
+            f = f - fr;
+            w.tcur = w.tcur + fr;
+            for3([&](auto K) __attribute__((always_inline)) {
+                constexpr int k = decltype(K)::value;
+                const R moved = at<k>(w.o) + tmin * at<k>(v);
+                const R snapv = at<k>(v) > (R)0 ? (R)1 : (R)0;
+                at<k>(w.o) = (k == hit) ? snapv : moved;
+            });
+        } else {
+            // finish inside the voxel, main.cu:237-243
+            const R ds = sc * f;
+            for3([&](auto K) __attribute__((always_inline)) {
+                constexpr int k = decltype(K)::value;
+                at<k>(w.o) = at<k>(w.o) + ds * at<k>(v);
+            });
+            w.tcur = w.tcur + f;
+            f = (R)0;
+            break;
+        }
+        const R vh = pick(v.x, v.y, v.z, hit);
+        const int32_t gh = pick(w.g.x, w.g.y, w.g.z, hit);
+        const int32_t Nh = pick(L.nx, L.ny, L.nz, hit);
+        const int32_t Sh = pick(L.sx, L.nz, 1, hit);
+        const bool up = vh > (R)0;
+        const int32_t nk = gh + (up ? 1 : -1);
+        if (nk < 0 || nk >= Nh) {
+            // lattice-edge mirror, main.cu:249-265; origin unfold main.cu:302-310
+            for3([&](auto K) __attribute__((always_inline)) {
+                constexpr int k = decltype(K)::value;
+                const bool h = (k == hit);
+                at<k>(v) = h ? -at<k>(v) : at<k>(v);
+                at<k>(iv) = h ? -at<k>(iv) : at<k>(iv);
+                const int32_t gm = (nk < 0) ? -at<k>(w.gi) : 2 * Nh - at<k>(w.gi);
+                at<k>(w.gi) = h ? gm : at<k>(w.gi);
+                at<k>(w.oi) = h ? -at<k>(w.oi) : at<k>(w.oi);
+            });
+            w.nmirror++;
+            continue;
+        }
+        const int32_t lin_n = w.lin + (up ? Sh : -Sh);
+        const int32_t cn = class_at(L, lin_n);  // label gather, main.cu:272
+        const int pidx = w.c * NC + cn;
+        const int cat = (int)((L.pcat >> (2 * pidx)) & 3u);
+        bool cross;
+        if (cat == 2) {
+            const R u = exch_uniform<R>(L, wid, i, nd, eblk);
+            nd++;
+            cross = u < L.P[pidx];
+        } else {
+            cross = (cat == 1);
+        }
+        if (cross) {
+            for3([&](auto K) __attribute__((always_inline)) {
+                constexpr int k = decltype(K)::value;
+                const bool h = (k == hit);
+                at<k>(w.g) = h ? nk : at<k>(w.g);
+                at<k>(w.o) = h ? (up ? (R)0 : (R)1) : at<k>(w.o);
+            });
+            w.lin = lin_n;
+            if (cn != w.c) {
+                flush(w);
+                w.c = cn;
+            }
+            w.ncross++;
+        } else {
+            for3([&](auto K) __attribute__((always_inline)) {
+                constexpr int k = decltype(K)::value;
+                const bool h = (k == hit);
+                at<k>(v) = h ? -at<k>(v) : at<k>(v);
+                at<k>(iv) = h ? -at<k>(iv) : at<k>(iv);
+            });
+            w.nrefl++;
+        }
+    }
+    if (f > (R)0) w.ntrunc++;
+    w.ndraw += (uint32_t)nd;
+}
+
+template <typename R>
+__device__ __forceinline__ V3<double> displacement(const Walker<R> &w, double res)
+{
+    V3<double> d;
+    d.x = ((double)(w.g.x - w.gi.x) + ((double)w.o.x - (double)w.oi.x)) * res;
+    d.y = ((double)(w.g.y - w.gi.y) + ((double)w.o.y - (double)w.oi.y)) * res;
+    d.z = ((double)(w.g.z - w.gi.z) + ((double)w.o.z - (double)w.oi.z)) * res;
+    return d;
+}
+
+// T2 weight, main.cu:334-338.
+template <typename R>
+__device__ __forceinline__ double t2_weight(Walker<R> &w, const JobLocal<R> &L)
+{
+    flush(w);
+    double s = 0.0;
+    s = s + (w.ts0 / L.T0);
+    if (L.nclass > 1) s = s + (w.ts1 / L.T1);
+    if (L.nclass > 2) s = s + (w.ts2 / L.T2);
+    if (L.nclass > 3) s = s + (w.ts3 / L.T3);
+    return exp_neg(-1.0 * s);
+}
+
+// Wave-sum one record value and store it from lane 0.  Lanes past the end of
+// the job contribute an exact +0.0.
+__device__ __forceinline__ void put(double v, bool active, double *dst, int lane)
+{
+    double t = wave_sum(active ? v : 0.0);
+    if (lane == 0) *dst = t;
+}
+
+template <typename R>
+__device__ __forceinline__ void load_state(Walker<R> &w, const StateBuf &st, int64_t s)
+{
+    const int64_t ws = st.ws;
+    w.g = {st.g[s], st.g[ws + s], st.g[2 * ws + s]};
+    w.gi = {st.gi[s], st.gi[ws + s], st.gi[2 * ws + s]};
+    if constexpr (sizeof(R) == 4) {
+        w.o = {st.of[s], st.of[ws + s], st.of[2 * ws + s]};
+        w.oi = {st.of[3 * ws + s], st.of[4 * ws + s], st.of[5 * ws + s]};
+        w.tcur = st.tcf[s];
+    } else {
+        w.o = {st.od[s], st.od[ws + s], st.od[2 * ws + s]};
+        w.oi = {st.od[3 * ws + s], st.od[4 * ws + s], st.od[5 * ws + s]};
+        w.tcur = st.tcd[s];
+    }
+    w.c = st.c[s];
+    w.ts0 = st.tsum[s];
+    w.ts1 = st.tsum[ws + s];
+    w.ts2 = st.tsum[2 * ws + s];
+    w.ts3 = st.tsum[3 * ws + s];
+    w.ncross = st.cnt[s];
+    w.nrefl = st.cnt[ws + s];
+    w.nmirror = st.cnt[2 * ws + s];
+    w.ntrunc = st.cnt[3 * ws + s];
+    w.ndraw = st.cnt[4 * ws + s];
+}
+
+template <typename R>
+__device__ __forceinline__ void save_state(const Walker<R> &w, const StateBuf &st, int64_t s)
+{
+    const int64_t ws = st.ws;
+    st.g[s] = w.g.x;
+    st.g[ws + s] = w.g.y;
+    st.g[2 * ws + s] = w.g.z;
+    st.gi[s] = w.gi.x;
+    st.gi[ws + s] = w.gi.y;
+    st.gi[2 * ws + s] = w.gi.z;
+    if constexpr (sizeof(R) == 4) {
+        st.of[s] = w.o.x;
+        st.of[ws + s] = w.o.y;
+        st.of[2 * ws + s] = w.o.z;
+        st.of[3 * ws + s] = w.oi.x;
+        st.of[4 * ws + s] = w.oi.y;
+        st.of[5 * ws + s] = w.oi.z;
+        st.tcf[s] = w.tcur;
+    } else {
+        st.od[s] = w.o.x;
+        st.od[ws + s] = w.o.y;
+        st.od[2 * ws + s] = w.o.z;
+        st.od[3 * ws + s] = w.oi.x;
+        st.od[4 * ws + s] = w.oi.y;
+        st.od[5 * ws + s] = w.oi.z;
+        st.tcd[s] = w.tcur;
+    }
+    st.c[s] = w.c;
+    st.tsum[s] = w.ts0;
+    st.tsum[ws + s] = w.ts1;
+    st.tsum[2 * ws + s] = w.ts2;
+    st.tsum[3 * ws + s] = w.ts3;
+    st.cnt[s] = w.ncross;
+    st.cnt[ws + s] = w.nrefl;
+    st.cnt[2 * ws + s] = w.nmirror;
+    st.cnt[3 * ws + s] = w.ntrunc;
+    st.cnt[4 * ws + s] = w.ndraw;
+}
+
+// The persistent walk kernel.  SIG: bit 0 narrow-pulse b-table signal,
+// bit 1 PGSE signal.
+template <typename R, int SIG>
+__global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
+{
+    const int lane = (int)(threadIdx.x & 63u);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int wpb = (int)(blockDim.x >> 6);
+    int64_t first, stride, hi;
+    if (a.xcd_split) {
+        // workgroups are dispatched round-robin over the 8 XCDs: give each
+        // XCD a contiguous range of batches so a geometry stays in one L2
+        const int xcd = (int)(blockIdx.x & 7u);
+        const int64_t bx = (int64_t)(blockIdx.x >> 3);
+        const int64_t nbx = (int64_t)(gridDim.x >> 3);
+        const int64_t lo = a.nbatch * xcd / 8;
+        hi = a.nbatch * (xcd + 1) / 8;
+        first = lo + bx * wpb + wv;
+        stride = nbx * wpb;
+    } else {
+        first = (int64_t)blockIdx.x * wpb + wv;
+        stride = (int64_t)gridDim.x * wpb;
+        hi = a.nbatch;
+    }
+    for (int64_t b = first; b < hi; b += stride) {
+        const int jid = __builtin_amdgcn_readfirstlane(a.batch_job[b]);
+        const JobDesc &J = a.jobs[jid];
+        const JobLocal<R> L = job_local<R>(J);
+        const int64_t lb = b - J.batch0;                   // batch within the job
+        const uint64_t wid = (uint64_t)(lb * WAVE + lane);  // walker id within the job
+        const bool active = (int64_t)wid < J.nwalk;
+        const int64_t slot = J.wbase + lb * WAVE + lane;    // state slot
+        const int nbvec = (SIG & 1) ? J.nbvec : 0;
+        const int nval = 1 + L.nclass + 21 + nbvec;
+        const int nbound = (SIG & 2) ? J.nbound : 0;
+        const int64_t ws = a.st.ws;
+
+        Walker<R> w;
+        w.ts0 = w.ts1 = w.ts2 = w.ts3 = 0.0;
+        w.ncross = w.nrefl = w.nmirror = w.ntrunc = w.ndraw = 0;
+        w.tcur = (R)0;
+        V3<double> ps = {0.0, 0.0, 0.0};
+        if (a.step0 == 0) {
+            init_walker<R>(w, L, wid);
+        } else {
+            load_state<R>(w, a.st, slot);
+            w.lin = w.g.x * L.sx + w.g.y * L.nz + w.g.z;
+            if (SIG & 2) ps = {a.st.ps[slot], a.st.ps[ws + slot], a.st.ps[2 * ws + slot]};
+        }
+        // next record step and next PGSE boundary at or after step0
+        int64_t rnext = ((a.step0 + a.Tstep - 1) / a.Tstep) * a.Tstep;
+        const int64_t rend = (int64_t)a.timepoints * a.Tstep;  // records only for r < timepoints
+        int kb = 1;
+        int64_t bnext = -1;
+        if (SIG & 2) {
+            while (kb <= nbound && J.bounds[kb] <= a.step0) kb++;
+            bnext = kb <= nbound ? J.bounds[kb] : -1;
+        }
+        U4 dblk = {0, 0, 0, 0};
+        double *part = J.part + lb * (int64_t)a.timepoints * nval;
+
+        for (int64_t i = a.step0; i < a.step1; i++) {
+            const bool fresh = ((i & 1) == 0) || (i == a.step0);
+            walker_step<R>(w, L, wid, i, fresh, dblk);
+            if (SIG & 2) {
+                const V3<double> d = displacement<R>(w, L.res);
+                ps.x = ps.x + d.x;
+                ps.y = ps.y + d.y;
+                ps.z = ps.z + d.z;
+                if (i + 1 == bnext) {
+                    double *sp = J.snap + (int64_t)(kb * 3) * ws + slot;
+                    sp[0] = ps.x;
+                    sp[ws] = ps.y;
+                    sp[2 * ws] = ps.z;
+                    kb++;
+                    bnext = kb <= nbound ? J.bounds[kb] : -1;
+                }
+            }
+            if (i == rnext && i < rend) {
+                // records, main.cu:333-378
+                const int64_t r = i / a.Tstep;
+                rnext += a.Tstep;
+                const double s0 = t2_weight<R>(w, L);
+                const V3<double> d = displacement<R>(w, L.res);
+                const double x = d.x, y = d.y, z = d.z;
+                double *dst = part + r * nval;
+                put(s0, active, dst + 0, lane);
+                for (int cc = 0; cc < L.nclass; cc++) put(cc == w.c ? 1.0 : 0.0, active, dst + 1 + cc, lane);
+                double *m = dst + 1 + L.nclass;
+                put(s0 * x * x, active, m + 0, lane);
+                put(s0 * x * y, active, m + 1, lane);
+                put(s0 * x * z, active, m + 2, lane);
+                put(s0 * y * y, active, m + 3, lane);
+                put(s0 * y * z, active, m + 4, lane);
+                put(s0 * z * z, active, m + 5, lane);
+                put(s0 * x * x * x * x, active, m + 6, lane);
+                put(s0 * x * x * x * y, active, m + 7, lane);
+                put(s0 * x * x * x * z, active, m + 8, lane);
+                put(s0 * x * x * y * y, active, m + 9, lane);
+                put(s0 * x * x * y * z, active, m + 10, lane);
+                put(s0 * x * x * z * z, active, m + 11, lane);
+                put(s0 * x * y * y * y, active, m + 12, lane);
+                put(s0 * x * y * y * z, active, m + 13, lane);
+                put(s0 * x * y * z * z, active, m + 14, lane);
+                put(s0 * x * z * z * z, active, m + 15, lane);
+                put(s0 * y * y * y * y, active, m + 16, lane);
+                put(s0 * y * y * y * z, active, m + 17, lane);
+                put(s0 * y * y * z * z, active, m + 18, lane);
+                put(s0 * y * z * z * z, active, m + 19, lane);
+                put(s0 * z * z * z * z, active, m + 20, lane);
+                if (SIG & 1) {
+                    const double *qs = J.qscale + r * nbvec;
+                    for (int jb = 0; jb < nbvec; jb++) {
+                        const double *bv = J.bvec + 3 * jb;
+                        const double qx = qs[jb] * (x * bv[0] + y * bv[1] + z * bv[2]);
+                        put(s0 * cos_any(qx), active, m + 21 + jb, lane);
+                    }
+                }
+            }
+        }
+        if ((SIG & 2) && a.step1 == a.TN) {
+            // final PGSE signal, main_PGSE.cu:348-357
+            const double s0 = t2_weight<R>(w, L);
+            double *psig = J.psig + lb * (int64_t)J.ngrad;
+            for (int t = 0; t < J.ntriple; t++) {
+                const int32_t *tr = J.triple + 4 * t;
+                const int ka = tr[0], kk = tr[1], ke = tr[2], g0 = tr[3], g1 = tr[7];
+                double df[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double s1 = ka == 0 ? 0.0 : J.snap[(int64_t)(ka * 3 + k) * ws + slot];
+                    const double se = ke == 0 ? 0.0 : J.snap[(int64_t)(ke * 3 + k) * ws + slot];
+                    const double sb = kk == 0 ? 0.0 : J.snap[(int64_t)(kk * 3 + k) * ws + slot];
+                    const double s2 = se - sb;
+                    df[k] = s1 - s2;
+                }
+                for (int jg = g0; jg < g1; jg++) {
+                    const double *gr = J.grad + 6 * jg;
+                    const double dot = df[0] * gr[3] + df[1] * gr[4] + df[2] * gr[5];
+                    const double ph = (gr[2] * dot) * a.dt;
+                    const double tsum = wave_sum(active ? s0 * cos_any(ph) : 0.0);
+                    if (lane == 0) psig[J.gperm[jg]] = tsum;
+                }
+            }
+        }
+        if (a.step1 == a.TN) {
+            // diagnostics: exact integer sums
+            const uint32_t cs[5] = {w.ncross, w.nrefl, w.nmirror, w.ntrunc, w.ndraw};
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const uint32_t t = wave_sum_u32(active ? cs[k] : 0u);
+                if (lane == 0) atomicAdd(J.counters + k, (unsigned long long)t);
+            }
+            if (a.fin.g && active) {
+                a.fin.g[slot] = w.g.x;
+                a.fin.g[ws + slot] = w.g.y;
+                a.fin.g[2 * ws + slot] = w.g.z;
+                a.fin.o[slot] = (double)w.o.x;
+                a.fin.o[ws + slot] = (double)w.o.y;
+                a.fin.o[2 * ws + slot] = (double)w.o.z;
+                a.fin.gi[slot] = w.gi.x;
+                a.fin.gi[ws + slot] = w.gi.y;
+                a.fin.gi[2 * ws + slot] = w.gi.z;
+                a.fin.oi[slot] = (double)w.oi.x;
+                a.fin.oi[ws + slot] = (double)w.oi.y;
+                a.fin.oi[2 * ws + slot] = (double)w.oi.z;
+                a.fin.c[slot] = w.c;
+                a.fin.cnt[slot] = w.ncross;
+                a.fin.cnt[ws + slot] = w.nrefl;
+                a.fin.cnt[2 * ws + slot] = w.nmirror;
+                a.fin.cnt[3 * ws + slot] = w.ntrunc;
+                a.fin.cnt[4 * ws + slot] = w.ndraw;
+            }
+        } else {
+            save_state<R>(w, a.st, slot);
+            if (SIG & 2) {
+                a.st.ps[slot] = ps.x;
+                a.st.ps[ws + slot] = ps.y;
+                a.st.ps[2 * ws + slot] = ps.z;
+            }
+        }
+    }
+}
+
+// Replica sums: rep[rho][q] = sum over batches b = rho, rho+R, ... of
+// part[b][q], in increasing b.  One thread per (job, rho, q).
+__global__ void reduce_replicas_kernel(const ReduceDesc *rd, int32_t njobs, int32_t R)
+{
+    const int jid = blockIdx.y;
+    if (jid >= njobs) return;
+    const ReduceDesc &D = rd[jid];
+    const int64_t nq = D.nq;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nq * R) return;
+    const int rho = (int)(t / nq);
+    const int64_t q = t - (int64_t)rho * nq;
+    double s = 0.0;
+    for (int64_t b = rho; b < D.nb; b += R) s = s + D.part[b * nq + q];
+    D.rep[(int64_t)rho * nq + q] = s;
+}
+
+// Totals: tot[q] = rep[0][q] + rep[1][q] + ... in order (starting from 0).
+__global__ void reduce_total_kernel(const ReduceDesc *rd, int32_t njobs, int32_t R)
+{
+    const int jid = blockIdx.y;
+    if (jid >= njobs) return;
+    const ReduceDesc &D = rd[jid];
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= D.nq) return;
+    double s = 0.0;
+    for (int r = 0; r < R; r++) s = s + D.rep[(int64_t)r * D.nq + q];
+    D.tot[q] = s;
+}
+
+// Zero fill (used instead of hipMemsetAsync so launches stay graph-safe).
+__global__ void fill_zero_kernel(unsigned long long *p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t s = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += s) p[i] = 0ull;
+}
+
+// ------------------------------------------------------------------ launch
+template <typename R, int SIG>
+static hipError_t launch_walk_t(const LaunchArgs &a, int grid, int block, hipStream_t s)
+{
+    hipLaunchKernelGGL((walk_kernel<R, SIG>), dim3(grid), dim3(block), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_walk(const LaunchArgs &a, int prec_f64, int sig, int grid, int block, hipStream_t s)
+{
+    if (prec_f64) {
+        switch (sig) {
+        case 0: return launch_walk_t<double, 0>(a, grid, block, s);
+        case 1: return launch_walk_t<double, 1>(a, grid, block, s);
+        case 2: return launch_walk_t<double, 2>(a, grid, block, s);
+        default: return launch_walk_t<double, 3>(a, grid, block, s);
+        }
+    }
+    switch (sig) {
+    case 0: return launch_walk_t<float, 0>(a, grid, block, s);
+    case 1: return launch_walk_t<float, 1>(a, grid, block, s);
+    case 2: return launch_walk_t<float, 2>(a, grid, block, s);
+    default: return launch_walk_t<float, 3>(a, grid, block, s);
+    }
+}
+
+hipError_t walk_occupancy(int prec_f64, int sig, int block, int *blocks_per_cu)
+{
+    const void *fn;
+    if (prec_f64)
+        fn = sig == 0 ? (const void *)walk_kernel<double, 0>
+                      : sig == 1 ? (const void *)walk_kernel<double, 1>
+                                 : sig == 2 ? (const void *)walk_kernel<double, 2> : (const void *)walk_kernel<double, 3>;
+    else
+        fn = sig == 0 ? (const void *)walk_kernel<float, 0>
+                      : sig == 1 ? (const void *)walk_kernel<float, 1>
+                                 : sig == 2 ? (const void *)walk_kernel<float, 2> : (const void *)walk_kernel<float, 3>;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, block, 0);
+}
+
+hipError_t launch_reduce(const ReduceDesc *rd_dev, int32_t njobs, int64_t max_nq, int32_t R, hipStream_t s)
+{
+    const int bs = 256;
+    dim3 g1((unsigned)((max_nq * R + bs - 1) / bs), (unsigned)njobs);
+    hipLaunchKernelGGL(reduce_replicas_kernel, g1, dim3(bs), 0, s, rd_dev, njobs, R);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    dim3 g2((unsigned)((max_nq + bs - 1) / bs), (unsigned)njobs);
+    hipLaunchKernelGGL(reduce_total_kernel, g2, dim3(bs), 0, s, rd_dev, njobs, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(void *p, int64_t bytes, hipStream_t s)
+{
+    int64_t n = bytes / 8;
+    if (n <= 0) return hipSuccess;
+    int grid = (int)((n + 255) / 256);
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid), dim3(256), 0, s, (unsigned long long *)p, n);
+    return hipGetLastError();
+}
+
+}  // namespace rms

@@ -1,0 +1,102 @@
+// rms_types.h -- descriptors shared by the host library and the HIP kernels.
+//
+// One "job" is one geometry with its walkers and acquisition tables; a launch
+// runs any number of jobs (the batched form).  Walkers are grouped in batches
+// of 64 (one wavefront); batch b of the launch belongs to job batch_job[b].
+#pragma once
+#include <stdint.h>
+
+namespace rms {
+
+constexpr int NC = 4;          // classes per geometry (reference Nc_max = 3)
+constexpr int NITE = 4;        // face iterations per step (main.cu:36)
+constexpr int WAVE = 64;       // walkers per batch
+constexpr int MAX_BOUNDS = 64; // distinct PGSE lobe boundaries
+constexpr uint32_t INIT_TRIALS = 1u << 26;
+
+struct JobDesc {
+    // geometry
+    const void *labels;        // uint8 or uint16, index (x*ny + y)*nz + z
+    const uint8_t *lclass;     // label -> class map or nullptr (identity)
+    int32_t nx, ny, nz;
+    int32_t label_bytes;
+    int32_t nclass;
+    int32_t nbvec;
+    int32_t ngrad;
+    int32_t ntriple;           // distinct PGSE lobe triples
+    int64_t nwalk;
+    int64_t batch0;            // first batch of this job in the launch
+    uint32_t key0, key1;       // Philox key = job seed
+    uint32_t pcat;             // 2 bits per class pair: 0 reflect, 1 cross, 2 draw
+    int32_t nbound;            // PGSE boundaries 1..nbound
+    double res;
+    float step_f[NC], inv_step_f[NC], P_f[NC * NC];
+    double step_d[NC], inv_step_d[NC], P_d[NC * NC];
+    double T2s[NC];            // T2 in steps (main.cu:433)
+    // tangent (adjoint) data: d step_c / d D_c and d P / d theta
+    double dstep_dD[NC];
+    double dlogP[NC * NC * 8]; // per pair: d log P / d theta (theta: D0..D3, kappa, pad)
+    double dlog1mP[NC * NC * 8];
+    // narrow-pulse b-table (main.cu:374-378)
+    const double *bvec;        // [nbvec*3]
+    const double *qscale;      // [timepoints*nbvec] sqrt(bval/TD)
+    // PGSE (main_PGSE.cu:340-357), window-sum form
+    const double *grad;        // [ngrad*6] sorted by lobe triple
+    const int32_t *gperm;      // [ngrad] original index of sorted gradient
+    const int32_t *triple;     // [ntriple*4]: ka, kb, ke, first gradient; sentinel at end
+    const int64_t *bounds;     // [nbound+1], bounds[0] = 0
+    double *snap;              // [(nbound+1)*3][wstride] prefix sums
+    // outputs
+    double *part;              // [nbatch_job][timepoints][nval]
+    double *psig;              // [nbatch_job][ngrad]
+    unsigned long long *counters; // [5]
+    // tangent outputs / inputs
+    const double *cot_rec;     // [timepoints][nval] cotangent of the record sums
+    const double *cot_sig;     // [ngrad]
+    double *pgrad;             // [nbatch_job][NPARAM]
+    int64_t wbase;             // first walker slot in the state arrays
+};
+
+// Walker state saved between chunked launches (structure of arrays).
+struct StateBuf {
+    int32_t *g;       // [3][ws]
+    int32_t *gi;      // [3][ws]
+    int32_t *c;       // [ws]
+    float *of;        // [6][ws] offsets o, oi (f32 mode)
+    double *od;       // [6][ws] offsets o, oi (f64 mode)
+    float *tcf;       // [ws]
+    double *tcd;      // [ws]
+    double *tsum;     // [NC][ws]
+    double *ps;       // [3][ws] PGSE running sums
+    uint32_t *cnt;    // [5][ws]
+    uint32_t *rng;    // [4][ws] cached direction block (f32 mode)
+    int64_t ws;       // stride
+};
+
+struct FinalOut {
+    int32_t *g;       // [3][ws] or nullptr
+    double *o;
+    int32_t *gi;
+    double *oi;
+    int32_t *c;
+    uint32_t *cnt;    // [5][ws]
+};
+
+constexpr int NPARAM = 16;     // tangent parameters (D0[4], T2[4], kappa, ...)
+
+struct LaunchArgs {
+    const JobDesc *jobs;
+    const int32_t *batch_job;  // [nbatch]
+    int64_t nbatch;
+    int64_t TN;
+    int64_t step0, step1;      // this launch runs steps [step0, step1)
+    int64_t Tstep;
+    int32_t timepoints;
+    int32_t nval;              // per job: 1 + nclass + 21 + nbvec (stride uses job value)
+    double dt;
+    StateBuf st;
+    FinalOut fin;
+    int32_t xcd_split;         // 1: partition batches per XCD
+};
+
+}  // namespace rms

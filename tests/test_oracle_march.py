@@ -1,0 +1,162 @@
+"""Pins of the oracle's face march (main.cu:153-300) against closed-form ray
+geometry: straight segments in free lattices, triangle-wave folding at
+reflecting walls and lattice-edge mirrors, corner ties, and the step-length
+change when a walker crosses into a class with another diffusivity."""
+import math
+
+import numpy as np
+import pytest
+
+DT = 2e-4
+
+
+def _step(orc, D):
+    return math.sqrt(6.0 * DT * D) / 0.1
+
+
+def _unit(v):
+    v = np.asarray(v, float)
+    return v / np.linalg.norm(v)
+
+
+def _pos(r):
+    return r["g"] + r["o"]
+
+
+def _fold(x, lo, hi):
+    """Position of a point reflected between walls at lo and hi."""
+    L = hi - lo
+    y = (x - lo) % (2 * L)
+    return lo + (y if y <= L else 2 * L - y)
+
+
+def test_free_segment_matches_straight_line(orc):
+    lab = np.ones((9, 9, 9), np.uint8)
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0])
+    s = _step(orc, 2.0)
+    rng = np.random.default_rng(0)
+    for _ in range(400):
+        x0 = 3.0 + rng.random(3) * 3.0
+        v = _unit(rng.normal(size=3))
+        g0 = np.floor(x0).astype(int)
+        r = orc.march_f64(g, DT, g0, x0 - g0, v.copy(), 1)
+        end = x0 + s * v
+        np.testing.assert_allclose(_pos(r), end, rtol=0, atol=1e-12)
+        np.testing.assert_array_equal(r["g"], np.floor(end).astype(int))
+        crossings = int(np.sum(np.abs(np.floor(end) - np.floor(x0))))
+        assert r["cnt"][0] == crossings and r["cnt"][1] == 0 and r["cnt"][3] == 0
+
+
+def test_reflecting_walls_fold_the_path(orc):
+    """Label-0 slabs at z=0 and z=5: the walker reflects between z=1 and z=5."""
+    lab = np.ones((6, 6, 6), np.uint8)
+    lab[:, :, 0] = 0
+    lab[:, :, 5] = 0
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0])
+    s = _step(orc, 2.0)
+    rng = np.random.default_rng(1)
+    nrefl = 0
+    for _ in range(400):
+        x0 = np.array([2.5, 2.5, 1.0]) + rng.random(3) * np.array([1.0, 1.0, 4.0])
+        v = _unit(rng.normal(size=3))
+        g0 = np.floor(x0).astype(int)
+        r = orc.march_f64(g, DT, g0, x0 - g0, v.copy(), 1)
+        end = x0 + s * v
+        want = np.array([end[0], end[1], _fold(end[2], 1.0, 5.0)])
+        np.testing.assert_allclose(_pos(r), want, atol=1e-12)
+        hits = int(end[2] < 1.0) + int(end[2] > 5.0)
+        assert r["cnt"][1] == hits
+        assert np.sign(r["v"][2]) == (np.sign(v[2]) * (-1) ** hits)
+        nrefl += hits
+    assert nrefl > 20
+
+
+def test_lattice_edge_mirror_and_origin_unfolding(orc):
+    """Edges of a 2x1x1 all-walkable lattice are mirrors (main.cu:249-265) and
+    the origin is mirrored with the walker (main.cu:302-310): the recorded
+    displacement is the free displacement with its sign flipped per hit."""
+    lab = np.ones((2, 1, 1), np.uint8)
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0])
+    s = _step(orc, 2.0)
+    N = np.array([2.0, 1.0, 1.0])
+    rng = np.random.default_rng(2)
+    for _ in range(400):
+        x0 = rng.random(3) * N
+        v = _unit(rng.normal(size=3))
+        g0 = np.floor(x0).astype(int)
+        r = orc.march_f64(g, DT, g0, x0 - g0, v.copy(), 1)
+        end = x0 + s * v
+        want = np.array([_fold(end[k], 0.0, N[k]) for k in range(3)])
+        np.testing.assert_allclose(_pos(r), want, atol=1e-12)
+        hits = np.array([int(end[k] < 0) + int(end[k] > N[k]) for k in range(3)])
+        assert r["cnt"][2] == hits.sum()
+        disp = (r["g"] - r["gi"]) + (r["o"] - r["oi"])
+        np.testing.assert_allclose(disp, (-1.0) ** hits * (s * v), atol=1e-12)
+
+
+def test_corner_tie_crosses_all_faces(orc):
+    lab = np.ones((4, 4, 4), np.uint8)
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0])
+    s = _step(orc, 2.0)
+    v = _unit([1.0, 1.0, 1.0])
+    # start so that the diagonal meets the corner (2,2,2) within the step
+    x0 = np.array([2.0, 2.0, 2.0]) - 0.2 * v
+    g0 = np.floor(x0).astype(int)
+    r = orc.march_f64(g, DT, g0, x0 - g0, v.copy(), 1)
+    np.testing.assert_array_equal(r["g"], [2, 2, 2])
+    np.testing.assert_allclose(_pos(r), x0 + s * v, atol=1e-12)
+    assert r["cnt"][0] == 3 and r["cnt"][3] == 0
+
+
+def test_class_change_rescales_the_remaining_step(orc):
+    """Crossing into a class with D2 continues with the remaining fraction of
+    the step at step[c2] (main.cu:231-236 with ai=aj, :274-276)."""
+    lab = np.ones((6, 4, 4), np.uint8)
+    lab[3:, :, :] = 2
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0, 0.5], kappa=None)
+    s1, s2 = _step(orc, 2.0), _step(orc, 0.5)
+    P = orc.derive(g, DT)["P"]
+    assert P[1, 2] == 0.5 and P[2, 1] == 1.0
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        v = _unit([1.0, 0.3 * rng.normal(), 0.3 * rng.normal()])
+        x0 = np.array([3.0 - 0.3 * rng.random(), 1.5, 1.5])
+        g0 = np.floor(x0).astype(int)
+        # u=0.1 < P(1->2)=0.5: the exchange draw accepts the crossing
+        r = orc.march_f64(g, DT, g0, x0 - g0, v.copy(), 1, u_exch=(0.1, 0.1, 0.1, 0.1))
+        d1 = (3.0 - x0[0]) / v[0]
+        if d1 <= s1:
+            want = x0 + d1 * v + s2 * (1.0 - d1 / s1) * v
+            assert r["cls"] == 2 and r["draws"] == 1
+        else:
+            want = x0 + s1 * v
+            assert r["cls"] == 1
+        np.testing.assert_allclose(_pos(r), want, atol=1e-12)
+        # u=0.9 > 0.5: rejected, elastic reflection off the face x=3
+        r = orc.march_f64(g, DT, g0, x0 - g0, v.copy(), 1, u_exch=(0.9, 0.9, 0.9, 0.9))
+        end = x0 + s1 * v
+        want = np.array([3.0 - (end[0] - 3.0) if end[0] > 3.0 else end[0], end[1], end[2]])
+        np.testing.assert_allclose(_pos(r), want, atol=1e-12)
+        assert r["cls"] == 1
+
+
+@pytest.mark.parametrize("D", [0.5, 2.0, 3.0])
+def test_step_length_is_conserved(orc, D):
+    """Path length of a step equals step[c] (no truncation for step < 1)."""
+    lab = np.ones((5, 5, 5), np.uint8)
+    lab[2, 2, 2] = 0
+    g = orc.Geometry(lab, 0.1, [0.0, D])
+    s = _step(orc, D)
+    rng = np.random.default_rng(4)
+    for _ in range(300):
+        x0 = 1.0 + rng.random(3) * 3.0
+        g0 = np.floor(x0).astype(int)
+        if lab[tuple(g0)] == 0:
+            continue
+        v = _unit(rng.normal(size=3))
+        r = orc.march_f64(g, DT, g0, x0 - g0, v.copy(), 1)
+        assert r["cnt"][3] == 0
+        # unfolded path: |v| stays 1 and the walker never sits in the null voxel
+        assert abs(np.linalg.norm(r["v"]) - 1.0) < 1e-14
+        assert lab[tuple(r["g"])] != 0
+        assert np.linalg.norm(_pos(r) - x0) <= s + 1e-12

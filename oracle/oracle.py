@@ -87,6 +87,15 @@ class _Out(C.Structure):
     ]
 
 
+class _Adj(C.Structure):
+    _fields_ = [
+        ("cot_rec", C.c_void_p),
+        ("cot_sig", C.c_void_p),
+        ("grad", C.c_void_p),
+        ("grad_rep", C.c_void_p),
+    ]
+
+
 _lib = None
 
 
@@ -127,6 +136,9 @@ def lib():
         L.or_march_f64.restype = C.c_int
         L.or_run.argtypes = [C.POINTER(_Job), C.c_int32, C.POINTER(_Params), C.POINTER(_Out)]
         L.or_run.restype = C.c_int
+        L.or_run_adjoint.argtypes = [C.POINTER(_Job), C.c_int32, C.POINTER(_Params), C.POINTER(_Adj),
+                                     C.POINTER(_Out)]
+        L.or_run_adjoint.restype = C.c_int
         L.or_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -274,7 +286,7 @@ def nval(nclass, nbvec):
 
 
 def run(jobs, dt, TN, timepoints, prec="f32", replicas=32, pgse_bruteforce=False, nthreads=0,
-        final_state=False):
+        final_state=False, cot_rec=None, cot_sig=None):
     """Run a list of jobs; each job is a dict(geom, nwalk, seed, bval, bvec, grad).
 
     Returns per job a dict with the record totals split into the
@@ -334,8 +346,25 @@ def run(jobs, dt, TN, timepoints, prec="f32", replicas=32, pgse_bruteforce=False
         res.append(r)
     p = _Params(dt=float(dt), TN=int(TN), timepoints=int(timepoints), prec=PREC[prec], replicas=max(1, replicas),
                 pgse_bruteforce=int(bool(pgse_bruteforce)), nthreads=int(nthreads))
-    if not L.or_run(cj, n, C.byref(p), co):
-        raise RuntimeError("oracle: " + L.or_last_error().decode())
+    if cot_rec is None and cot_sig is None:
+        if not L.or_run(cj, n, C.byref(p), co):
+            raise RuntimeError("oracle: " + L.or_last_error().decode())
+    else:
+        ca = (_Adj * n)()
+        for k in range(n):
+            cr = None if cot_rec is None or cot_rec[k] is None else np.ascontiguousarray(cot_rec[k], np.float64)
+            cs = None if cot_sig is None or cot_sig[k] is None else np.ascontiguousarray(cot_sig[k], np.float64)
+            g = np.zeros(16)
+            gr = np.zeros((max(1, replicas), 16))
+            keep += [cr, cs]
+            ca[k].cot_rec = _ptr(cr)
+            ca[k].cot_sig = _ptr(cs)
+            ca[k].grad = _ptr(g)
+            ca[k].grad_rep = _ptr(gr)
+            res[k]["grad"] = g
+            res[k]["grad_rep"] = gr
+        if not L.or_run_adjoint(cj, n, C.byref(p), ca, co):
+            raise RuntimeError("oracle: " + L.or_last_error().decode())
     for k, r in enumerate(res):
         r["counters"] = np.array(co[k].counters[:], np.uint64)
         split_records(r, r["_nclass"], r["_nbvec"])

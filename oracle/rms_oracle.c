@@ -363,6 +363,108 @@ static inline void or_record_values(double s0, const double dx[3], int32_t c, in
 #undef PUT
 }
 
+/* Constants of the tangent (adjoint) pass, derived on the host from the
+ * same quantities as or_derive.  Parameter p < 3 is D of class p+1. */
+typedef struct {
+    double dstep[OR_NC_MAX]; /* d step_c / d D_c = step_c / (2 D_c) */
+    double dinv[OR_NC_MAX];  /* d (1/step_c) / d D_c */
+    double rT[OR_NC_MAX];    /* 1 / T2s_c: rate of the T2 exponent per step in c */
+    double invT2[OR_NC_MAX]; /* 1 / T2_c (ms) */
+    double dlogP[OR_NC_MAX * OR_NC_MAX][3];
+    double dlog1mP[OR_NC_MAX * OR_NC_MAX][3];
+    double dlogPk[OR_NC_MAX * OR_NC_MAX];
+    double dlog1mPk[OR_NC_MAX * OR_NC_MAX];
+} or_tderiv;
+
+/* d log P_ij and d log(1 - P_ij) with respect to D_1..D_3 and kappa for the
+ * probabilistic pairs (Pcat = 2): P = min(1, sqrt(Dj/Di)) * pi0/(1+pi0),
+ * pi0 = 4 kappa dt / (res min(s_i, s_j)). */
+static void or_derive_tangent(const or_geom *g, double dt, const or_derived *d, or_tderiv *t)
+{
+    memset(t, 0, sizeof(*t));
+    for (int c = 0; c < g->nclass; c++) {
+        t->dstep[c] = (g->D[c] > 0.0) ? d->step[c] / (2.0 * g->D[c]) : 0.0;
+        t->dinv[c] = (g->D[c] > 0.0) ? -d->inv_step[c] / (2.0 * g->D[c]) : 0.0;
+        t->rT[c] = 1.0 / d->T2s[c];
+        t->invT2[c] = 1.0 / g->T2[c];
+    }
+    for (int i = 1; i < g->nclass; i++) {
+        for (int j = 1; j < g->nclass; j++) {
+            int ij = i * OR_NC_MAX + j;
+            if (d->Pcat[ij] != 2) continue;
+            double dl[3] = {0.0, 0.0, 0.0};
+            double dk = 0.0;
+            double ratio = sqrt(g->D[j] / g->D[i]);
+            if (ratio < 1.0) {
+                dl[j - 1] = dl[j - 1] + 0.5 / g->D[j];
+                dl[i - 1] = dl[i - 1] - 0.5 / g->D[i];
+            }
+            double kap = g->kappa[ij];
+            if (kap >= 0.0) {
+                double smin = fmin(d->step[i], d->step[j]);
+                double pi0 = 4.0 * kap * dt / (g->res * smin);
+                int m = (d->step[i] <= d->step[j]) ? i : j;
+                dl[m - 1] = dl[m - 1] - 0.5 / (g->D[m] * (1.0 + pi0));
+                dk = (kap > 0.0) ? 1.0 / (kap * (1.0 + pi0)) : 0.0;
+            }
+            double P = d->P[ij];
+            double r = -P / (1.0 - P);
+            for (int q = 0; q < 3; q++) {
+                t->dlogP[ij][q] = dl[q];
+                t->dlog1mP[ij][q] = r * dl[q];
+            }
+            t->dlogPk[ij] = dk;
+            t->dlog1mPk[ij] = r * dk;
+        }
+    }
+}
+
+static const int OR_DIND[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+static const int OR_WIND[15][4] = {{0, 0, 0, 0}, {0, 0, 0, 1}, {0, 0, 0, 2}, {0, 0, 1, 1}, {0, 0, 1, 2},
+                                   {0, 0, 2, 2}, {0, 1, 1, 1}, {0, 1, 1, 2}, {0, 1, 2, 2}, {0, 2, 2, 2},
+                                   {1, 1, 1, 1}, {1, 1, 1, 2}, {1, 1, 2, 2}, {1, 2, 2, 2}, {2, 2, 2, 2}};
+
+/* F = sum_q cot_q m_q(d) over one record row (m_q = the record value without
+ * the T2 weight) and its gradient with respect to the displacement d. */
+static void or_adj_record(const double d[3], int c, const double *cot, int nclass, int nbvec, const double *bvec,
+                          const double *qs, double *F, double g[3])
+{
+    double f = cot[0];
+    f = f + cot[1 + c];
+    g[0] = 0.0;
+    g[1] = 0.0;
+    g[2] = 0.0;
+    const double *c2 = cot + 1 + nclass;
+    for (int q = 0; q < 6; q++) {
+        int a = OR_DIND[q][0], b = OR_DIND[q][1];
+        f = f + c2[q] * (d[a] * d[b]);
+        g[a] = g[a] + c2[q] * d[b];
+        g[b] = g[b] + c2[q] * d[a];
+    }
+    const double *c4 = c2 + 6;
+    for (int q = 0; q < 15; q++) {
+        int i0 = OR_WIND[q][0], i1 = OR_WIND[q][1], i2 = OR_WIND[q][2], i3 = OR_WIND[q][3];
+        f = f + c4[q] * (((d[i0] * d[i1]) * d[i2]) * d[i3]);
+        g[i0] = g[i0] + c4[q] * ((d[i1] * d[i2]) * d[i3]);
+        g[i1] = g[i1] + c4[q] * ((d[i0] * d[i2]) * d[i3]);
+        g[i2] = g[i2] + c4[q] * ((d[i0] * d[i1]) * d[i3]);
+        g[i3] = g[i3] + c4[q] * ((d[i0] * d[i1]) * d[i2]);
+    }
+    const double *cs = c4 + 15;
+    for (int j = 0; j < nbvec; j++) {
+        const double *bv = bvec + 3 * j;
+        double arg = qs[j] * (d[0] * bv[0] + d[1] * bv[1] + d[2] * bv[2]);
+        double sn, cn;
+        or_sincos_f64(arg, &sn, &cn);
+        f = f + cs[j] * cn;
+        double t = (cs[j] * sn) * qs[j];
+        g[0] = g[0] - t * bv[0];
+        g[1] = g[1] - t * bv[1];
+        g[2] = g[2] - t * bv[2];
+    }
+    *F = f;
+}
+
 typedef struct {
     int nval;
     int64_t Tstep;
@@ -372,6 +474,11 @@ typedef struct {
     int32_t *gk;      /* [ngrad][3] indices (ka, kb, ke) into bounds */
     double *snapbuf;  /* per thread */
     double *phasebuf; /* per thread */
+    int32_t *gorder;  /* gradients sorted by lobe triple (stable) */
+    const double *cot_rec; /* adjoint: [timepoints][nval] or NULL */
+    const double *cot_sig; /* adjoint: [ngrad] or NULL */
+    const or_tderiv *td;   /* adjoint constants or NULL */
+    double *snapJ;    /* per thread [OR_BATCH][nb+1][3][3] tangent prefix sums */
 } or_jobaux;
 
 /* PGSE phase from window sums: phase = (|g| * (g_hat . (S1 - S2))) * dt with
@@ -444,7 +551,7 @@ int or_march_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double 
     or_dreal_f64 dr;
     or_dreal_make_f64(d, &dr);
     or_fixed_draws fd = {u_exch};
-    int nd = or_marchw_f64(&w, geom, d, &dr, v, or_draw_fixed, &fd);
+    int nd = or_marchw_f64(&w, geom, d, &dr, v, or_draw_fixed, &fd, NULL, NULL, NULL);
     (void)nd;
     for (int k = 0; k < 3; k++) {
         g[k] = w.g[k];
@@ -475,7 +582,29 @@ static int validate_job(const or_job *J, const or_params *p, const or_derived *d
     return 1;
 }
 
+static int or_run_impl(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs, const or_adj *adj);
+
 int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs)
+{
+    return or_run_impl(jobs, njobs, p, outs, NULL);
+}
+
+int or_run_adjoint(const or_job *jobs, int32_t njobs, const or_params *p, const or_adj *adj, or_out *outs)
+{
+    return or_run_impl(jobs, njobs, p, outs, adj);
+}
+
+static const or_jobaux *g_sort_aux;
+static int cmp_triple(const void *a, const void *b)
+{
+    int ia = *(const int32_t *)a, ib = *(const int32_t *)b;
+    const int32_t *ka = g_sort_aux->gk + 3 * ia, *kb = g_sort_aux->gk + 3 * ib;
+    for (int q = 0; q < 3; q++)
+        if (ka[q] != kb[q]) return ka[q] < kb[q] ? -1 : 1;
+    return ia < ib ? -1 : (ia > ib);
+}
+
+static int or_run_impl(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs, const or_adj *adj)
 {
     g_err[0] = 0;
     int R = p->replicas < 1 ? 1 : p->replicas;
@@ -487,6 +616,13 @@ int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs)
         if (!validate_job(J, p, &d)) return 0;
         or_jobaux aux;
         memset(&aux, 0, sizeof(aux));
+        or_tderiv td;
+        if (adj) {
+            or_derive_tangent(J->geom, p->dt, &d, &td);
+            aux.td = &td;
+            aux.cot_rec = adj[jb].cot_rec;
+            aux.cot_sig = adj[jb].cot_sig;
+        }
         aux.nval = or_nval(J->geom->nclass, J->nbvec);
         aux.Tstep = p->TN / p->timepoints;
         int T = p->timepoints;
@@ -538,8 +674,13 @@ int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs)
                     aux.gk[3 * j + q] = k;
                 }
             free(abe);
+            aux.gorder = (int32_t *)malloc(sizeof(int32_t) * J->ngrad);
+            for (int j = 0; j < J->ngrad; j++) aux.gorder[j] = j;
+            g_sort_aux = &aux;
+            qsort(aux.gorder, J->ngrad, sizeof(int32_t), cmp_triple);
         }
         int64_t nbatch = (J->nwalk + OR_BATCH - 1) / OR_BATCH;
+        double *pgrad = adj ? (double *)calloc((size_t)nbatch * 16, sizeof(double)) : NULL;
         double *part = (double *)calloc((size_t)nbatch * T * aux.nval, sizeof(double));
         double *psig = J->ngrad > 0 ? (double *)calloc((size_t)nbatch * J->ngrad, sizeof(double)) : NULL;
         int ok = 1;
@@ -554,18 +695,21 @@ int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs)
             or_jobaux la = aux;
             la.snapbuf = (double *)malloc(sizeof(double) * OR_BATCH * (aux.nbound + 1) * 3 + 8);
             la.phasebuf = (double *)malloc(sizeof(double) * OR_BATCH * (J->ngrad + 1));
+            la.snapJ = (double *)calloc((size_t)OR_BATCH * (aux.nbound + 1) * 9 + 8, sizeof(double));
             double *lanebuf = (double *)malloc(sizeof(double) * OR_BATCH * lanew);
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 1)
 #endif
             for (int64_t b = 0; b < nbatch; b++) {
                 int r;
+                if (la.snapJ) memset(la.snapJ, 0, sizeof(double) * ((size_t)OR_BATCH * (aux.nbound + 1) * 9 + 8));
+                double *pg = pgrad ? pgrad + (size_t)b * 16 : NULL;
                 if (p->prec == OR_PREC_F32)
                     r = or_batch_f32(J, &d, p, &la, b, part + (size_t)b * T * aux.nval,
-                                     psig ? psig + (size_t)b * J->ngrad : NULL, lanebuf, O, cnt_acc);
+                                     psig ? psig + (size_t)b * J->ngrad : NULL, lanebuf, O, cnt_acc, pg);
                 else
                     r = or_batch_f64(J, &d, p, &la, b, part + (size_t)b * T * aux.nval,
-                                     psig ? psig + (size_t)b * J->ngrad : NULL, lanebuf, O, cnt_acc);
+                                     psig ? psig + (size_t)b * J->ngrad : NULL, lanebuf, O, cnt_acc, pg);
                 if (!r) {
 #ifdef _OPENMP
 #pragma omp atomic write
@@ -579,6 +723,7 @@ int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs)
             for (int q = 0; q < 5; q++) cnt_tot[q] += cnt_acc[q];
             free(la.snapbuf);
             free(la.phasebuf);
+            free(la.snapJ);
             free(lanebuf);
         }
         if (!ok) {
@@ -587,6 +732,8 @@ int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs)
             free(TD);
             free(aux.qscale);
             free(aux.gk);
+            free(aux.gorder);
+            free(pgrad);
             return fail("walker initialisation found no non-null voxel");
         }
         /* replica sums in batch order, then replicas in order */
@@ -619,6 +766,21 @@ int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs)
             free(srep);
         }
         for (int q = 0; q < 5; q++) O->counters[q] = cnt_tot[q];
+        if (adj) {
+            double *grep = (double *)calloc((size_t)R * 16, sizeof(double));
+            for (int64_t b = 0; b < nbatch; b++)
+                for (int q = 0; q < 16; q++)
+                    grep[(size_t)(b % R) * 16 + q] = grep[(size_t)(b % R) * 16 + q] + pgrad[(size_t)b * 16 + q];
+            for (int q = 0; q < 16; q++) {
+                double s = 0.0;
+                for (int r = 0; r < R; r++) s = s + grep[(size_t)r * 16 + q];
+                adj[jb].grad[q] = s;
+            }
+            if (adj[jb].grad_rep) memcpy(adj[jb].grad_rep, grep, sizeof(double) * R * 16);
+            free(grep);
+            free(pgrad);
+        }
+        free(aux.gorder);
         free(part);
         free(psig);
         free(TD);

@@ -116,6 +116,16 @@ int or_march_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double 
 /* ---- full simulation of a list of jobs ---- */
 int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs);
 
+/* Adjoint pass: derivatives of sum(cot_rec*rec) + sum(cot_sig*sig) with
+ * respect to D[c] (grad[c], c = 1..3), T2[c] (grad[4+c]) and kappa (grad[8]). */
+typedef struct {
+    const double *cot_rec;  /* [timepoints*nval] or NULL */
+    const double *cot_sig;  /* [ngrad] or NULL */
+    double *grad;           /* [16] out */
+    double *grad_rep;       /* [replicas*16] out or NULL */
+} or_adj;
+int or_run_adjoint(const or_job *jobs, int32_t njobs, const or_params *p, const or_adj *adj, or_out *outs);
+
 const char *or_last_error(void);
 
 #ifdef __cplusplus

@@ -65,6 +65,7 @@ struct Walker {
     int32_t c;       // class
     int32_t lin;     // linear voxel index of g
     R tcur;          // steps spent in class c since the last flush
+    R sc, isc;       // step of class c (voxels) and its reciprocal
     double ts0, ts1, ts2, ts3;  // steps per class
     uint32_t ncross, nrefl, nmirror, ntrunc, ndraw;
 };
@@ -219,16 +220,34 @@ __device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, in
 // One step i of the walk: direction (main.cu:140-151) and face march
 // (main.cu:153-300).  `dblk` is the cached direction block (f32 layout:
 // one Philox block serves steps 2m and 2m+1).
+//
+// With a step shorter than one voxel each axis meets at most one face per
+// step, so the three face distances t_k (main.cu:199-219) are computed once,
+// visited in increasing (t, axis) order -- the order of the reference's strict
+// `tt < tmin` scan -- and the walker advances by t_e - t_{e-1}.  A face is
+// reached when f * step >= t_e - t_{e-1} (main.cu:231); the remaining
+// fraction finishes the step (main.cu:237-243).  The three event slots are
+// unrolled and predicated, so a wavefront never loops on its slowest lane.
 template <typename R>
 __device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, uint64_t wid, int64_t i,
-                                            bool fresh_block, U4 &dblk)
+                                            bool fresh_block, uint32_t &b0, uint32_t &b1, uint32_t &b2,
+                                            uint32_t &b3)
 {
     R u1, u2;
     if constexpr (sizeof(R) == 4) {
-        if (fresh_block) dblk = rng_block(L.key0, L.key1, (uint32_t)(i >> 1), STREAM_DIR, wid);
-        const bool odd = (i & 1) != 0;
-        u1 = u01_f32(odd ? dblk.z : dblk.x);
-        u2 = u01_f32(odd ? dblk.w : dblk.y);
+        // words (b0, b1) serve this step; (b2, b3) are shifted in for step i+1
+        if (fresh_block) {
+            const U4 r = rng_block(L.key0, L.key1, (uint32_t)(i >> 1), STREAM_DIR, wid);
+            const bool odd = (i & 1) != 0;
+            b0 = odd ? r.z : r.x;
+            b1 = odd ? r.w : r.y;
+            b2 = r.z;
+            b3 = r.w;
+        }
+        u1 = u01_f32(b0);
+        u2 = u01_f32(b1);
+        b0 = b2;
+        b1 = b3;
     } else {
         U4 r = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_DIR, wid);
         u1 = u01_f64(r.x, r.y);
@@ -240,105 +259,133 @@ __device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, 
     sincos_turns(u2 * PiRef<R>::v, sp, cp);
     V3<R> v = {sth * cp, sth * sp, cth};
     V3<R> iv = {(R)1 / v.x, (R)1 / v.y, (R)1 / v.z};
+    // face distances in units of path, main.cu:199-219
+    V3<R> t;
+    for3([&](auto K) __attribute__((always_inline)) {
+        constexpr int k = decltype(K)::value;
+        const R vk = at<k>(v);
+        const R ok = at<k>(w.o);
+        const R tmp = vk > (R)0 ? ((R)1 - ok) : (vk < (R)0 ? -ok : (R)2);
+        R q = tmp * at<k>(iv);
+        q = (q > (R)0) ? q : (R)0;
+        at<k>(t) = (fabs(tmp) > fabs(vk)) ? (R)2 : q;
+    });
+    // stable order by (t, axis)
+    R t0 = t.x, t1 = t.y, t2 = t.z;
+    int h0 = 0, h1 = 1, h2 = 2;
+    {
+        bool sw = t1 < t0;
+        R tt = sw ? t1 : t0;
+        t1 = sw ? t0 : t1;
+        t0 = tt;
+        int hh = sw ? h1 : h0;
+        h1 = sw ? h0 : h1;
+        h0 = hh;
+        sw = t2 < t1;
+        tt = sw ? t2 : t1;
+        t2 = sw ? t1 : t2;
+        t1 = tt;
+        hh = sw ? h2 : h1;
+        h2 = sw ? h1 : h2;
+        h1 = hh;
+        sw = t1 < t0;
+        tt = sw ? t1 : t0;
+        t1 = sw ? t0 : t1;
+        t0 = tt;
+        hh = sw ? h1 : h0;
+        h1 = sw ? h0 : h1;
+        h0 = hh;
+    }
     R f = (R)1;
+    R trav = (R)0;
     int nd = 0;
     U4 eblk = {0, 0, 0, 0};
-    for (int j = 0; (j < NITE) && (f > (R)0); j++) {
-        // nearest face within one unit of path, main.cu:199-219
-        R tmin = (R)2;
-        int hit = -1;
-        for3([&](auto K) __attribute__((always_inline)) {
-            constexpr int k = decltype(K)::value;
-            const R vk = at<k>(v);
-            const R ok = at<k>(w.o);
-            const R tmp = vk > (R)0 ? ((R)1 - ok) : (vk < (R)0 ? -ok : (R)2);
-            R q = tmp * at<k>(iv);
-            q = (q > (R)0) ? q : (R)0;
-            const R tt = (fabs(tmp) > fabs(vk)) ? (R)2 : q;
-            if (tt < tmin) {
-                tmin = tt;
-                hit = k;
-            }
-        });
-        const R sc = sel4(w.c, L.st0, L.st1, L.st2, L.st3);
-        if (f * sc >= tmin) {
+    bool live = true;
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const R te = e == 0 ? t0 : (e == 1 ? t1 : t2);
+        const int hit = e == 0 ? h0 : (e == 1 ? h1 : h2);
+        const R dte = te - trav;
+        live = live && (f > (R)0) && (f * w.sc >= dte);
+        if (live) {
             // reach the face, main.cu:231-236
-            const R fr = tmin * sel4(w.c, L.is0, L.is1, L.is2, L.is3);
+            const R fr = dte * w.isc;
             f = f - fr;
             w.tcur = w.tcur + fr;
             for3([&](auto K) __attribute__((always_inline)) {
                 constexpr int k = decltype(K)::value;
-                const R moved = at<k>(w.o) + tmin * at<k>(v);
+                const R moved = at<k>(w.o) + dte * at<k>(v);
                 const R snapv = at<k>(v) > (R)0 ? (R)1 : (R)0;
                 at<k>(w.o) = (k == hit) ? snapv : moved;
             });
-        } else {
-            // finish inside the voxel, main.cu:237-243
-            const R ds = sc * f;
-            for3([&](auto K) __attribute__((always_inline)) {
-                constexpr int k = decltype(K)::value;
-                at<k>(w.o) = at<k>(w.o) + ds * at<k>(v);
-            });
-            w.tcur = w.tcur + f;
-            f = (R)0;
-            break;
-        }
-        const R vh = pick(v.x, v.y, v.z, hit);
-        const int32_t gh = pick(w.g.x, w.g.y, w.g.z, hit);
-        const int32_t Nh = pick(L.nx, L.ny, L.nz, hit);
-        const int32_t Sh = pick(L.sx, L.nz, 1, hit);
-        const bool up = vh > (R)0;
-        const int32_t nk = gh + (up ? 1 : -1);
-        if (nk < 0 || nk >= Nh) {
-            // lattice-edge mirror, main.cu:249-265; origin unfold main.cu:302-310
-            for3([&](auto K) __attribute__((always_inline)) {
-                constexpr int k = decltype(K)::value;
-                const bool h = (k == hit);
-                at<k>(v) = h ? -at<k>(v) : at<k>(v);
-                at<k>(iv) = h ? -at<k>(iv) : at<k>(iv);
-                const int32_t gm = (nk < 0) ? -at<k>(w.gi) : 2 * Nh - at<k>(w.gi);
-                at<k>(w.gi) = h ? gm : at<k>(w.gi);
-                at<k>(w.oi) = h ? -at<k>(w.oi) : at<k>(w.oi);
-            });
-            w.nmirror++;
-            continue;
-        }
-        const int32_t lin_n = w.lin + (up ? Sh : -Sh);
-        const int32_t cn = class_at(L, lin_n);  // label gather, main.cu:272
-        const int pidx = w.c * NC + cn;
-        const int cat = (int)((L.pcat >> (2 * pidx)) & 3u);
-        bool cross;
-        if (cat == 2) {
-            const R u = exch_uniform<R>(L, wid, i, nd, eblk);
-            nd++;
-            cross = u < L.P[pidx];
-        } else {
-            cross = (cat == 1);
-        }
-        if (cross) {
-            for3([&](auto K) __attribute__((always_inline)) {
-                constexpr int k = decltype(K)::value;
-                const bool h = (k == hit);
-                at<k>(w.g) = h ? nk : at<k>(w.g);
-                at<k>(w.o) = h ? (up ? (R)0 : (R)1) : at<k>(w.o);
-            });
-            w.lin = lin_n;
-            if (cn != w.c) {
-                flush(w);
-                w.c = cn;
+            trav = te;
+            const R vh = pick(v.x, v.y, v.z, hit);
+            const int32_t gh = pick(w.g.x, w.g.y, w.g.z, hit);
+            const int32_t Nh = pick(L.nx, L.ny, L.nz, hit);
+            const bool up = vh > (R)0;
+            const int32_t nk = gh + (up ? 1 : -1);
+            bool flip;
+            if (nk < 0 || nk >= Nh) {
+                // lattice-edge mirror, main.cu:249-265; origin unfold main.cu:302-310
+                for3([&](auto K) __attribute__((always_inline)) {
+                    constexpr int k = decltype(K)::value;
+                    const bool h = (k == hit);
+                    const int32_t gm = (nk < 0) ? -at<k>(w.gi) : 2 * Nh - at<k>(w.gi);
+                    at<k>(w.gi) = h ? gm : at<k>(w.gi);
+                    at<k>(w.oi) = h ? -at<k>(w.oi) : at<k>(w.oi);
+                });
+                w.nmirror++;
+                flip = true;
+            } else {
+                const int32_t Sh = pick(L.sx, L.nz, 1, hit);
+                const int32_t lin_n = w.lin + (up ? Sh : -Sh);
+                const int32_t cn = class_at(L, lin_n);  // label gather, main.cu:272
+                const int pidx = w.c * NC + cn;
+                const int cat = (int)((L.pcat >> (2 * pidx)) & 3u);
+                bool cross = (cat == 1);
+                if (cat == 2) {
+                    const R u = exch_uniform<R>(L, wid, i, nd, eblk);
+                    nd++;
+                    cross = u < L.P[pidx];
+                }
+                if (cross) {
+                    for3([&](auto K) __attribute__((always_inline)) {
+                        constexpr int k = decltype(K)::value;
+                        const bool h = (k == hit);
+                        at<k>(w.g) = h ? nk : at<k>(w.g);
+                        at<k>(w.o) = h ? (up ? (R)0 : (R)1) : at<k>(w.o);
+                    });
+                    w.lin = lin_n;
+                    if (cn != w.c) {
+                        flush(w);
+                        w.c = cn;
+                        w.sc = sel4(cn, L.st0, L.st1, L.st2, L.st3);
+                        w.isc = sel4(cn, L.is0, L.is1, L.is2, L.is3);
+                    }
+                    w.ncross++;
+                } else {
+                    w.nrefl++;
+                }
+                flip = !cross;
             }
-            w.ncross++;
-        } else {
-            for3([&](auto K) __attribute__((always_inline)) {
-                constexpr int k = decltype(K)::value;
-                const bool h = (k == hit);
-                at<k>(v) = h ? -at<k>(v) : at<k>(v);
-                at<k>(iv) = h ? -at<k>(iv) : at<k>(iv);
-            });
-            w.nrefl++;
+            if (flip) {
+                for3([&](auto K) __attribute__((always_inline)) {
+                    constexpr int k = decltype(K)::value;
+                    const bool h = (k == hit);
+                    at<k>(v) = h ? -at<k>(v) : at<k>(v);
+                });
+            }
         }
     }
-    if (f > (R)0) w.ntrunc++;
+    if (f > (R)0) {
+        // finish inside the voxel, main.cu:237-243
+        const R ds = w.sc * f;
+        for3([&](auto K) __attribute__((always_inline)) {
+            constexpr int k = decltype(K)::value;
+            at<k>(w.o) = at<k>(w.o) + ds * at<k>(v);
+        });
+        w.tcur = w.tcur + f;
+    }
     w.ndraw += (uint32_t)nd;
 }
 
@@ -483,9 +530,13 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
         V3<double> ps = {0.0, 0.0, 0.0};
         if (a.step0 == 0) {
             init_walker<R>(w, L, wid);
+            w.sc = sel4(w.c, L.st0, L.st1, L.st2, L.st3);
+            w.isc = sel4(w.c, L.is0, L.is1, L.is2, L.is3);
         } else {
             load_state<R>(w, a.st, slot);
             w.lin = w.g.x * L.sx + w.g.y * L.nz + w.g.z;
+            w.sc = sel4(w.c, L.st0, L.st1, L.st2, L.st3);
+            w.isc = sel4(w.c, L.is0, L.is1, L.is2, L.is3);
             if (SIG & 2) ps = {a.st.ps[slot], a.st.ps[ws + slot], a.st.ps[2 * ws + slot]};
         }
         // next record step and next PGSE boundary at or after step0
@@ -497,12 +548,13 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
             while (kb <= nbound && J.bounds[kb] <= a.step0) kb++;
             bnext = kb <= nbound ? J.bounds[kb] : -1;
         }
-        U4 dblk = {0, 0, 0, 0};
+        uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
         double *part = J.part + lb * (int64_t)a.timepoints * nval;
+        int64_t r = rnext / a.Tstep;  // record index of rnext
 
         for (int64_t i = a.step0; i < a.step1; i++) {
             const bool fresh = ((i & 1) == 0) || (i == a.step0);
-            walker_step<R>(w, L, wid, i, fresh, dblk);
+            walker_step<R>(w, L, wid, i, fresh, b0, b1, b2, b3);
             if (SIG & 2) {
                 const V3<double> d = displacement<R>(w, L.res);
                 ps.x = ps.x + d.x;
@@ -519,7 +571,6 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
             }
             if (i == rnext && i < rend) {
                 // records, main.cu:333-378
-                const int64_t r = i / a.Tstep;
                 rnext += a.Tstep;
                 const double s0 = t2_weight<R>(w, L);
                 const V3<double> d = displacement<R>(w, L.res);
@@ -557,6 +608,7 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
                         put(s0 * cos_any(qx), active, m + 21 + jb, lane);
                     }
                 }
+                r++;
             }
         }
         if ((SIG & 2) && a.step1 == a.TN) {

@@ -1,0 +1,86 @@
+"""torch.autograd over the HIP engine.
+
+`simulate` runs one job through `Plan.execute` and returns the record sums
+(the sig0 / NPar_count / dx2 / dx4 / sigRe columns of hpc_code/lib/main.cu:
+341-378, one row per record) and the PGSE signal (main_PGSE.cu:348-357) as
+torch tensors that are differentiable with respect to the per-class
+diffusivity D, the per-class T2 and a common membrane permeability kappa.
+The backward pass is `Plan.adjoint` (rms_plan_adjoint): the same walk (same
+counter-based random stream) with forward-mode tangents, contracted with the
+incoming cotangents on the device.
+
+Estimators used by the adjoint pass:
+  D      pathwise derivative of the displacement (faces fixed, step length
+         d s_c / d D_c = s_c / (2 D_c)) plus the score of the exchange draws
+         whose probability depends on D (P = min(1, sqrt(Dj/Di)) ...);
+  T2     analytic: d/dT2_c exp(-sum t_c/T2_c) = (t_c / T2_c^2) exp(...);
+  kappa  score function of the Bernoulli exchange draws,
+         d log P / d kappa = 1 / (kappa (1 + pi0)).
+Class 0 (the null compartment) has no gradient.  The inputs are host tensors
+(float64); the simulation itself always runs on the engine's device.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .engine import Geometry
+
+
+class _Simulate(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, D, T2, kappa, spec):
+        nc = D.numel()
+        kap = None if kappa is None else float(kappa.detach().reshape(()).item())
+        geom = Geometry(spec["labels"], spec["res"], D.detach().cpu().double().numpy().tolist(),
+                        T2=T2.detach().cpu().double().numpy().tolist(), kappa=kap,
+                        label_class=spec.get("label_class"))
+        job = dict(geom=geom, nwalk=spec["nwalk"], seed=spec.get("seed", 1), bval=spec.get("bval"),
+                   bvec=spec.get("bvec"), grad=spec.get("grad"))
+        plan = spec["engine"].plan([job], spec["dt"], spec["TN"], spec.get("timepoints", 1000),
+                                   precision=spec.get("precision", "f32"), replicas=spec.get("replicas", 32))
+        try:
+            out = plan.execute(final_state=False, want_replicas=False)[0]
+        except Exception:
+            plan.close()
+            raise
+        ctx.plan = plan
+        ctx.nc = nc
+        ctx.has_kappa = kappa is not None
+        ctx.ngrad = 0 if out["sig"] is None else len(out["sig"])
+        rec = torch.from_numpy(np.ascontiguousarray(out["rec"]))
+        sig = torch.from_numpy(out["sig"]) if ctx.ngrad else torch.zeros(0, dtype=torch.float64)
+        return rec, sig
+
+    @staticmethod
+    def backward(ctx, g_rec, g_sig):
+        plan = ctx.plan
+        try:
+            cr = None if g_rec is None else g_rec.detach().cpu().double().contiguous().numpy()
+            cs = None if (g_sig is None or ctx.ngrad == 0) else g_sig.detach().cpu().double().contiguous().numpy()
+            g = plan.adjoint([cr], [cs])[0]["grad"]
+        finally:
+            plan.close()
+            ctx.plan = None
+        nc = ctx.nc
+        gD = torch.from_numpy(g[:nc].copy())
+        gT = torch.from_numpy(g[4:4 + nc].copy())
+        gk = torch.tensor(g[8], dtype=torch.float64) if ctx.has_kappa else None
+        return gD, gT, gk, None
+
+
+def simulate(engine, labels, res, D, T2, nwalk, dt, TN, timepoints=1000, kappa=None, seed=1, bval=None, bvec=None,
+             grad=None, precision="f32", replicas=32, label_class=None):
+    """Differentiable simulation of one geometry.
+
+    D, T2: float64 tensors of length nclass (T2 = inf for no relaxation);
+    kappa: None (the reference rule P = min(1, sqrt(Dj/Di))) or a scalar
+    tensor applied to every pair of non-null classes.  Returns (rec, sig):
+    rec [timepoints, nval] record sums, sig [ngrad] PGSE signal sums."""
+    D = torch.as_tensor(D, dtype=torch.float64)
+    T2 = torch.as_tensor(T2, dtype=torch.float64)
+    if kappa is not None:
+        kappa = torch.as_tensor(kappa, dtype=torch.float64)
+    spec = dict(engine=engine, labels=labels, res=res, nwalk=nwalk, dt=dt, TN=TN, timepoints=timepoints, seed=seed,
+                bval=bval, bvec=bvec, grad=grad, precision=precision, replicas=replicas, label_class=label_class)
+    return _Simulate.apply(D, T2, kappa, spec)

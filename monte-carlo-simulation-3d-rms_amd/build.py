@@ -44,7 +44,7 @@ def build_lib(verbose=False):
     os.makedirs(LIB_DIR, exist_ok=True)
     tmp = LIB + ".tmp"
     cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp,
-           os.path.join(CSRC, "rms_kernels.hip"), os.path.join(CSRC, "rms_adjoint.hip"),
+           os.path.join(CSRC, "rms_kernels.hip"),
            os.path.join(CSRC, "rms_api.cpp")]
     out = _run(cmd)
     os.replace(tmp, LIB)

@@ -77,6 +77,15 @@ struct rms_plan {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    int maxbound = 0;
+    // adjoint pass buffers, allocated by the first rms_plan_adjoint call
+    bool adj_ready = false;
+    int adj_np = 1, adj_grid = 0;
+    JobDesc *d_jobs_adj = nullptr;
+    ReduceDesc *d_red_adj = nullptr;
+    std::vector<ReduceDesc> red_adj;
+    std::vector<double *> cot_rec, cot_sig;
+    double *d_tsnap = nullptr;
 };
 
 static const char *kVersion = "rms_amd 0.1 (gfx950)";
@@ -199,6 +208,58 @@ extern "C" rms_status rms_derive(const rms_geometry *geom, double dt, double *st
     if (T2s) std::memcpy(T2s, t2, sizeof(t2));
     if (P) std::memcpy(P, pp, sizeof(pp));
     return RMS_OK;
+}
+
+// Constants of the tangent (adjoint) pass, from the same quantities as
+// derive(): parameter p < 3 is D of class p+1, p = 3 is kappa.  For the
+// probabilistic pairs P = min(1, sqrt(Dj/Di)) * pi0/(1+pi0) with
+// pi0 = 4 kappa dt / (res min(s_i, s_j)); d log P and d log(1-P) = -P/(1-P) d log P.
+static void derive_tangent(const rms_geometry &g, double dt, const double step[NC], const double inv_step[NC],
+                           const double T2s[NC], const double P[NC * NC], const int Pcat[NC * NC], JobDesc &D)
+{
+    for (int c = 0; c < NC; c++) {
+        D.dstep_d[c] = D.dinv_d[c] = D.rT[c] = D.invT2[c] = 0.0;
+        D.dstep_f[c] = D.dinv_f[c] = 0.0f;
+    }
+    for (int i = 0; i < NC * NC * 4; i++) D.dlogP[i] = D.dlog1mP[i] = 0.0;
+    for (int c = 0; c < g.nclass; c++) {
+        const double ds = (g.D[c] > 0.0) ? step[c] / (2.0 * g.D[c]) : 0.0;
+        const double di = (g.D[c] > 0.0) ? -inv_step[c] / (2.0 * g.D[c]) : 0.0;
+        D.dstep_d[c] = ds;
+        D.dinv_d[c] = di;
+        D.dstep_f[c] = (float)ds;
+        D.dinv_f[c] = (float)di;
+        D.rT[c] = 1.0 / T2s[c];
+        D.invT2[c] = 1.0 / g.T2[c];
+    }
+    for (int i = 1; i < g.nclass; i++) {
+        for (int j = 1; j < g.nclass; j++) {
+            const int ij = i * NC + j;
+            if (Pcat[ij] != 2) continue;
+            double dl[3] = {0.0, 0.0, 0.0};
+            double dk = 0.0;
+            const double ratio = std::sqrt(g.D[j] / g.D[i]);
+            if (ratio < 1.0) {
+                dl[j - 1] = dl[j - 1] + 0.5 / g.D[j];
+                dl[i - 1] = dl[i - 1] - 0.5 / g.D[i];
+            }
+            const double kap = g.kappa[ij];
+            if (kap >= 0.0) {
+                const double smin = std::fmin(step[i], step[j]);
+                const double pi0 = 4.0 * kap * dt / (g.res * smin);
+                const int m = (step[i] <= step[j]) ? i : j;
+                dl[m - 1] = dl[m - 1] - 0.5 / (g.D[m] * (1.0 + pi0));
+                dk = (kap > 0.0) ? 1.0 / (kap * (1.0 + pi0)) : 0.0;
+            }
+            const double r = -P[ij] / (1.0 - P[ij]);
+            for (int q = 0; q < 3; q++) {
+                D.dlogP[ij * 4 + q] = dl[q];
+                D.dlog1mP[ij * 4 + q] = r * dl[q];
+            }
+            D.dlogP[ij * 4 + 3] = dk;
+            D.dlog1mP[ij * 4 + 3] = r * dk;
+        }
+    }
 }
 
 extern "C" rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id)
@@ -421,6 +482,7 @@ extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t
             D.P_f[i] = (float)Pp[i];
             D.P_d[i] = Pp[i];
         }
+        derive_tangent(G.meta, P.dt, step, inv_step, T2s, Pp, Pcat, D);
         JobHost &H = p->jobs[j];
         H.geom_id = J.geom_id;
         H.nwalk = J.nwalk;
@@ -554,6 +616,7 @@ extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t
             p->max_nq = std::max(p->max_nq, (int64_t)H.ngrad);
         }
     }
+    p->maxbound = maxbound;
     if (sig & 2) {
         TRY(palloc(p, &p->d_snap, sizeof(double) * (size_t)(maxbound + 1) * 3 * p->ws));
         for (int j = 0; j < njobs; j++) p->hjobs[j].snap = p->d_snap;
@@ -766,10 +829,113 @@ extern "C" rms_status rms_run(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, 
     return s;
 }
 
-extern "C" rms_status rms_plan_adjoint(rms_plan *plan, const rms_job_adj *adj, rms_timing *timing)
+// Buffers of the adjoint pass: cotangents, per-batch gradient partials with
+// their ordered reductions, and the PGSE tangent prefix-sum snapshots.
+static rms_status adjoint_setup(rms_plan *p)
 {
-    (void)adj;
-    (void)timing;
-    if (!plan) return RMS_ERR_INVALID_ARG;
-    return fail(plan->ctx, RMS_ERR_UNSUPPORTED, "adjoint pass not built yet");
+    rms_ctx *ctx = p->ctx;
+    const int nj = (int)p->jobs.size();
+    const int R = p->prm.replicas;
+    const int64_t T = p->prm.timepoints;
+    int np = 1;
+    for (int j = 0; j < nj; j++) np = std::max(np, (int)p->hjobs[j].nclass - 1);
+    p->adj_np = std::min(np, 3);
+    p->cot_rec.assign(nj, nullptr);
+    p->cot_sig.assign(nj, nullptr);
+    rms_status st;
+    for (int j = 0; j < nj; j++) {
+        const JobHost &H = p->jobs[j];
+        double *pg, *rep, *tot;
+        if ((st = palloc(p, &p->cot_rec[j], sizeof(double) * T * H.nval)) != RMS_OK) return st;
+        if (H.ngrad > 0 && (st = palloc(p, &p->cot_sig[j], sizeof(double) * H.ngrad)) != RMS_OK) return st;
+        if ((st = palloc(p, &pg, sizeof(double) * H.nbatch * NPARAM)) != RMS_OK) return st;
+        if ((st = palloc(p, &rep, sizeof(double) * R * NPARAM)) != RMS_OK) return st;
+        if ((st = palloc(p, &tot, sizeof(double) * NPARAM)) != RMS_OK) return st;
+        p->red_adj.push_back(ReduceDesc{pg, rep, tot, H.nbatch, NPARAM});
+    }
+    if (p->sig & 2) {
+        const size_t n = (size_t)(p->maxbound + 1) * p->adj_np * 3 * p->ws;
+        if ((st = palloc(p, &p->d_tsnap, sizeof(double) * n)) != RMS_OK) return st;
+    }
+    if ((st = palloc(p, &p->d_jobs_adj, sizeof(JobDesc) * nj)) != RMS_OK) return st;
+    if ((st = pupload(p, &p->d_red_adj, p->red_adj.data(), p->red_adj.size())) != RMS_OK) return st;
+    int bpc = 0;
+    const bool f64 = p->prm.precision == RMS_PREC_F64;
+    if (adjoint_occupancy(f64, p->sig, p->adj_np, p->block, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+    const int wpb = p->block / WAVE;
+    const int64_t need = (p->nbatch + wpb - 1) / wpb;
+    int64_t grid = p->prm.grid_blocks > 0 ? p->prm.grid_blocks : std::min(need, (int64_t)bpc * ctx->ncu);
+    if (grid < 1) grid = 1;
+    if (p->xcd_split) grid = std::max<int64_t>(8, (grid / 8) * 8);
+    p->adj_grid = (int)grid;
+    p->adj_ready = true;
+    return RMS_OK;
+}
+
+extern "C" rms_status rms_plan_adjoint(rms_plan *p, const rms_job_adj *adj, rms_timing *timing)
+{
+    if (!p || !adj) return RMS_ERR_INVALID_ARG;
+    rms_ctx *ctx = p->ctx;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int nj = (int)p->jobs.size();
+    for (int j = 0; j < nj; j++)
+        if (!adj[j].grad) return fail(ctx, RMS_ERR_INVALID_ARG, "adjoint: grad output is required for every job");
+    if (!p->adj_ready) {
+        rms_status st = adjoint_setup(p);
+        if (st != RMS_OK) return st;
+    }
+    hipStream_t s = ctx->stream;
+    const int64_t T = p->prm.timepoints;
+    std::vector<JobDesc> hj = p->hjobs;
+    for (int j = 0; j < nj; j++) {
+        const JobHost &H = p->jobs[j];
+        JobDesc &D = hj[j];
+        D.cot_rec = nullptr;
+        D.cot_sig = nullptr;
+        if (adj[j].cot_rec) {
+            HIPCHK(ctx, hipMemcpyAsync(p->cot_rec[j], adj[j].cot_rec, sizeof(double) * T * H.nval,
+                                       hipMemcpyHostToDevice, s));
+            D.cot_rec = p->cot_rec[j];
+        }
+        if (adj[j].cot_sig && H.ngrad > 0) {
+            HIPCHK(ctx, hipMemcpyAsync(p->cot_sig[j], adj[j].cot_sig, sizeof(double) * H.ngrad,
+                                       hipMemcpyHostToDevice, s));
+            D.cot_sig = p->cot_sig[j];
+        }
+        D.pgrad = const_cast<double *>(p->red_adj[j].part);
+        D.tsnap = p->d_tsnap;
+        HIPCHK(ctx, launch_fill_zero(D.pgrad, sizeof(double) * H.nbatch * NPARAM, s));
+    }
+    HIPCHK(ctx, hipMemcpyAsync(p->d_jobs_adj, hj.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipEventRecord(p->ev[0], s));
+    LaunchArgs a = make_args(p, 0, p->prm.TN);
+    a.jobs = p->d_jobs_adj;
+    a.st.g = nullptr;  // single launch: no walker state is saved
+    a.fin = FinalOut{};
+    HIPCHK(ctx, launch_adjoint(a, p->prm.precision == RMS_PREC_F64, p->sig, p->adj_np, p->adj_grid, p->block, s));
+    HIPCHK(ctx, hipEventRecord(p->ev[1], s));
+    HIPCHK(ctx, launch_reduce(p->d_red_adj, (int32_t)p->red_adj.size(), NPARAM, p->prm.replicas, s));
+    HIPCHK(ctx, hipEventRecord(p->ev[2], s));
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    if (timing) {
+        std::memset(timing, 0, sizeof(*timing));
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, p->ev[0], p->ev[2]);
+        timing->total_ms = ms;
+        hipEventElapsedTime(&ms, p->ev[0], p->ev[1]);
+        timing->walk_ms = ms;
+        hipEventElapsedTime(&ms, p->ev[1], p->ev[2]);
+        timing->reduce_ms = ms;
+        timing->grid = p->adj_grid;
+        timing->block = p->block;
+        timing->launches = 1;
+    }
+    const int R = p->prm.replicas;
+    for (int j = 0; j < nj; j++) {
+        const ReduceDesc &rd = p->red_adj[j];
+        HIPCHK(ctx, hipMemcpy(adj[j].grad, rd.tot, sizeof(double) * NPARAM, hipMemcpyDeviceToHost));
+        if (adj[j].grad_rep)
+            HIPCHK(ctx, hipMemcpy(adj[j].grad_rep, rd.rep, sizeof(double) * NPARAM * R, hipMemcpyDeviceToHost));
+    }
+    return RMS_OK;
 }

@@ -23,7 +23,7 @@ hipError_t launch_fill_zero(void *p, int64_t bytes, hipStream_t s);
 
 // Tangent (adjoint) pass: same walk with forward-mode tangents, contracted
 // with the record/signal cotangents per walker.
-hipError_t launch_adjoint(const LaunchArgs &a, int prec_f64, int sig, int grid, int block, hipStream_t s);
-hipError_t adjoint_occupancy(int prec_f64, int sig, int block, int *blocks_per_cu);
+hipError_t launch_adjoint(const LaunchArgs &a, int prec_f64, int sig, int np, int grid, int block, hipStream_t s);
+hipError_t adjoint_occupancy(int prec_f64, int sig, int np, int block, int *blocks_per_cu);
 
 }  // namespace rms

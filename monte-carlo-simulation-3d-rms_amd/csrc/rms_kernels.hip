@@ -70,6 +70,18 @@ struct Walker {
     uint32_t ncross, nrefl, nmirror, ntrunc, ndraw;
 };
 
+// Tangent state of the adjoint pass (NP pathwise parameters: D of classes
+// 1..NP): position tangents, tangent of the T2 exponent E = sum t_c / T2s_c,
+// score sums of the exchange draws, PGSE prefix sums of res * Jo.
+template <typename R, int NP>
+struct Tan {
+    V3<R> Jo[NP > 0 ? NP : 1];
+    double dE[NP > 0 ? NP : 1];
+    double S[NP > 0 ? NP : 1];
+    double Sk;
+    V3<double> PJ[NP > 0 ? NP : 1];
+};
+
 // Per-job constants held in registers for the whole batch.
 template <typename R>
 struct JobLocal {
@@ -83,6 +95,10 @@ struct JobLocal {
     double T0, T1, T2, T3;   // T2 in steps
     double res;
     const R *P;              // permeation probabilities [NC*NC]
+    // adjoint constants
+    R ds1, ds2, ds3;         // d step_c / d D_c, classes 1..3
+    R di1, di2, di3;         // d (1/step_c) / d D_c
+    double rT0, rT1, rT2, rT3;  // 1 / T2s_c
 };
 
 template <typename R>
@@ -132,6 +148,25 @@ __device__ __forceinline__ JobLocal<R> job_local(const JobDesc &J)
     L.T2 = J.T2s[2];
     L.T3 = J.T2s[3];
     L.res = J.res;
+    if constexpr (sizeof(R) == 4) {
+        L.ds1 = J.dstep_f[1];
+        L.ds2 = J.dstep_f[2];
+        L.ds3 = J.dstep_f[3];
+        L.di1 = J.dinv_f[1];
+        L.di2 = J.dinv_f[2];
+        L.di3 = J.dinv_f[3];
+    } else {
+        L.ds1 = J.dstep_d[1];
+        L.ds2 = J.dstep_d[2];
+        L.ds3 = J.dstep_d[3];
+        L.di1 = J.dinv_d[1];
+        L.di2 = J.dinv_d[2];
+        L.di3 = J.dinv_d[3];
+    }
+    L.rT0 = J.rT[0];
+    L.rT1 = J.rT[1];
+    L.rT2 = J.rT[2];
+    L.rT3 = J.rT[3];
     return L;
 }
 
@@ -228,10 +263,10 @@ __device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, in
 // reached when f * step >= t_e - t_{e-1} (main.cu:231); the remaining
 // fraction finishes the step (main.cu:237-243).  The three event slots are
 // unrolled and predicated, so a wavefront never loops on its slowest lane.
-template <typename R>
-__device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, uint64_t wid, int64_t i,
-                                            bool fresh_block, uint32_t &b0, uint32_t &b1, uint32_t &b2,
-                                            uint32_t &b3)
+template <typename R, int NP>
+__device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t wid,
+                                            int64_t i, bool fresh_block, uint32_t &b0, uint32_t &b1, uint32_t &b2,
+                                            uint32_t &b3, Tan<R, NP> &T)
 {
     R u1, u2;
     if constexpr (sizeof(R) == 4) {
@@ -301,6 +336,9 @@ __device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, 
     int nd = 0;
     U4 eblk = {0, 0, 0, 0};
     bool live = true;
+    R df[NP > 0 ? NP : 1];
+#pragma unroll
+    for (int p = 0; p < (NP > 0 ? NP : 1); p++) df[p] = (R)0;
 #pragma unroll
     for (int e = 0; e < 3; e++) {
         const R te = e == 0 ? t0 : (e == 1 ? t1 : t2);
@@ -310,6 +348,25 @@ __device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, 
         if (live) {
             // reach the face, main.cu:231-236
             const R fr = dte * w.isc;
+            if constexpr (NP > 0) {
+                // tangents: the face is fixed, d(dte) = -d o_hit / v_hit
+                const R ivh = pick(iv.x, iv.y, iv.z, hit);
+                const double rTc = sel4(w.c, L.rT0, L.rT1, L.rT2, L.rT3);
+#pragma unroll
+                for (int p = 0; p < NP; p++) {
+                    const R Jh = pick(T.Jo[p].x, T.Jo[p].y, T.Jo[p].z, hit);
+                    const R dtm = -(Jh)*ivh;
+                    const R dip = p == 0 ? L.di1 : (p == 1 ? L.di2 : L.di3);
+                    const R dis = (w.c == p + 1) ? dip : (R)0;
+                    const R dfr = dtm * w.isc + dte * dis;
+                    df[p] = df[p] - dfr;
+                    T.dE[p] = T.dE[p] + rTc * (double)dfr;
+                    for3([&](auto K) __attribute__((always_inline)) {
+                        constexpr int k = decltype(K)::value;
+                        at<k>(T.Jo[p]) = (k == hit) ? (R)0 : at<k>(T.Jo[p]) + dtm * at<k>(v);
+                    });
+                }
+            }
             f = f - fr;
             w.tcur = w.tcur + fr;
             for3([&](auto K) __attribute__((always_inline)) {
@@ -347,6 +404,13 @@ __device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, 
                     const R u = exch_uniform<R>(L, wid, i, nd, eblk);
                     nd++;
                     cross = u < L.P[pidx];
+                    if constexpr (NP > 0) {
+                        // score of the Bernoulli(P) draw
+                        const double *sc4 = cross ? (J.dlogP + 4 * pidx) : (J.dlog1mP + 4 * pidx);
+#pragma unroll
+                        for (int p = 0; p < NP; p++) T.S[p] = T.S[p] + sc4[p];
+                        T.Sk = T.Sk + sc4[3];
+                    }
                 }
                 if (cross) {
                     for3([&](auto K) __attribute__((always_inline)) {
@@ -380,6 +444,20 @@ __device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, 
     if (f > (R)0) {
         // finish inside the voxel, main.cu:237-243
         const R ds = w.sc * f;
+        if constexpr (NP > 0) {
+            const double rTc = sel4(w.c, L.rT0, L.rT1, L.rT2, L.rT3);
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                const R dsp = p == 0 ? L.ds1 : (p == 1 ? L.ds2 : L.ds3);
+                const R dsc = (w.c == p + 1) ? dsp : (R)0;
+                const R dds = dsc * f + w.sc * df[p];
+                for3([&](auto K) __attribute__((always_inline)) {
+                    constexpr int k = decltype(K)::value;
+                    at<k>(T.Jo[p]) = at<k>(T.Jo[p]) + dds * at<k>(v);
+                });
+                T.dE[p] = T.dE[p] + rTc * (double)df[p];
+            }
+        }
         for3([&](auto K) __attribute__((always_inline)) {
             constexpr int k = decltype(K)::value;
             at<k>(w.o) = at<k>(w.o) + ds * at<k>(v);
@@ -486,9 +564,62 @@ __device__ __forceinline__ void save_state(const Walker<R> &w, const StateBuf &s
     st.cnt[4 * ws + s] = w.ndraw;
 }
 
+// Contraction of one record row with its cotangent (adjoint pass):
+// F = sum_q cot_q m_q(d) (m_q = record value without the T2 weight) and
+// dF/dd.  Same operation order as the host restatement.
+__device__ __forceinline__ void adj_record(const double *cot, int c, int nclass, int nbvec, const double *bvec,
+                                           const double *qs, double x, double y, double z, double &F, double &g0,
+                                           double &g1, double &g2)
+{
+    const double d[3] = {x, y, z};
+    double g[3] = {0.0, 0.0, 0.0};
+    double f = cot[0];
+    f = f + cot[1 + c];
+    const double *c2 = cot + 1 + nclass;
+    constexpr int DI[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        const int ia = DI[q][0], ib = DI[q][1];
+        f = f + c2[q] * (d[ia] * d[ib]);
+        g[ia] = g[ia] + c2[q] * d[ib];
+        g[ib] = g[ib] + c2[q] * d[ia];
+    }
+    const double *c4 = c2 + 6;
+    constexpr int WI[15][4] = {{0, 0, 0, 0}, {0, 0, 0, 1}, {0, 0, 0, 2}, {0, 0, 1, 1}, {0, 0, 1, 2},
+                               {0, 0, 2, 2}, {0, 1, 1, 1}, {0, 1, 1, 2}, {0, 1, 2, 2}, {0, 2, 2, 2},
+                               {1, 1, 1, 1}, {1, 1, 1, 2}, {1, 1, 2, 2}, {1, 2, 2, 2}, {2, 2, 2, 2}};
+#pragma unroll
+    for (int q = 0; q < 15; q++) {
+        const int i0 = WI[q][0], i1 = WI[q][1], i2 = WI[q][2], i3 = WI[q][3];
+        f = f + c4[q] * (((d[i0] * d[i1]) * d[i2]) * d[i3]);
+        g[i0] = g[i0] + c4[q] * ((d[i1] * d[i2]) * d[i3]);
+        g[i1] = g[i1] + c4[q] * ((d[i0] * d[i2]) * d[i3]);
+        g[i2] = g[i2] + c4[q] * ((d[i0] * d[i1]) * d[i3]);
+        g[i3] = g[i3] + c4[q] * ((d[i0] * d[i1]) * d[i2]);
+    }
+    const double *cs = c4 + 15;
+    for (int j = 0; j < nbvec; j++) {
+        const double *bv = bvec + 3 * j;
+        const double arg = qs[j] * (d[0] * bv[0] + d[1] * bv[1] + d[2] * bv[2]);
+        double sn, cn;
+        sincos_any(arg, sn, cn);
+        f = f + cs[j] * cn;
+        const double t = (cs[j] * sn) * qs[j];
+        g[0] = g[0] - t * bv[0];
+        g[1] = g[1] - t * bv[1];
+        g[2] = g[2] - t * bv[2];
+    }
+    F = f;
+    g0 = g[0];
+    g1 = g[1];
+    g2 = g[2];
+}
+
 // The persistent walk kernel.  SIG: bit 0 narrow-pulse b-table signal,
-// bit 1 PGSE signal.
-template <typename R, int SIG>
+// bit 1 PGSE signal.  NP > 0 instantiates the adjoint pass: the same walk
+// with tangents for D of classes 1..NP, contracted with the cotangents of the
+// record rows and PGSE signal into per-batch parameter gradients.
+template <typename R, int SIG, int NP>
 __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
 {
     const int lane = (int)(threadIdx.x & 63u);
@@ -527,6 +658,19 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
         w.ts0 = w.ts1 = w.ts2 = w.ts3 = 0.0;
         w.ncross = w.nrefl = w.nmirror = w.ntrunc = w.ndraw = 0;
         w.tcur = (R)0;
+        Tan<R, NP> T;
+        double GD[NP > 0 ? NP : 1], GT0 = 0.0, GT1 = 0.0, GT2 = 0.0, GT3 = 0.0, GK = 0.0;
+        if constexpr (NP > 0) {
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                T.Jo[p] = {(R)0, (R)0, (R)0};
+                T.dE[p] = 0.0;
+                T.S[p] = 0.0;
+                T.PJ[p] = {0.0, 0.0, 0.0};
+                GD[p] = 0.0;
+            }
+            T.Sk = 0.0;
+        }
         V3<double> ps = {0.0, 0.0, 0.0};
         if (a.step0 == 0) {
             init_walker<R>(w, L, wid);
@@ -554,17 +698,34 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
 
         for (int64_t i = a.step0; i < a.step1; i++) {
             const bool fresh = ((i & 1) == 0) || (i == a.step0);
-            walker_step<R>(w, L, wid, i, fresh, b0, b1, b2, b3);
+            walker_step<R, NP>(w, L, J, wid, i, fresh, b0, b1, b2, b3, T);
             if (SIG & 2) {
                 const V3<double> d = displacement<R>(w, L.res);
                 ps.x = ps.x + d.x;
                 ps.y = ps.y + d.y;
                 ps.z = ps.z + d.z;
+                if constexpr (NP > 0) {
+#pragma unroll
+                    for (int p = 0; p < NP; p++) {
+                        T.PJ[p].x = T.PJ[p].x + (double)T.Jo[p].x * L.res;
+                        T.PJ[p].y = T.PJ[p].y + (double)T.Jo[p].y * L.res;
+                        T.PJ[p].z = T.PJ[p].z + (double)T.Jo[p].z * L.res;
+                    }
+                }
                 if (i + 1 == bnext) {
                     double *sp = J.snap + (int64_t)(kb * 3) * ws + slot;
                     sp[0] = ps.x;
                     sp[ws] = ps.y;
                     sp[2 * ws] = ps.z;
+                    if constexpr (NP > 0) {
+#pragma unroll
+                        for (int p = 0; p < NP; p++) {
+                            double *tp = J.tsnap + (int64_t)((kb * NP + p) * 3) * ws + slot;
+                            tp[0] = T.PJ[p].x;
+                            tp[ws] = T.PJ[p].y;
+                            tp[2 * ws] = T.PJ[p].z;
+                        }
+                    }
                     kb++;
                     bnext = kb <= nbound ? J.bounds[kb] : -1;
                 }
@@ -575,38 +736,59 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
                 const double s0 = t2_weight<R>(w, L);
                 const V3<double> d = displacement<R>(w, L.res);
                 const double x = d.x, y = d.y, z = d.z;
-                double *dst = part + r * nval;
-                put(s0, active, dst + 0, lane);
-                for (int cc = 0; cc < L.nclass; cc++) put(cc == w.c ? 1.0 : 0.0, active, dst + 1 + cc, lane);
-                double *m = dst + 1 + L.nclass;
-                put(s0 * x * x, active, m + 0, lane);
-                put(s0 * x * y, active, m + 1, lane);
-                put(s0 * x * z, active, m + 2, lane);
-                put(s0 * y * y, active, m + 3, lane);
-                put(s0 * y * z, active, m + 4, lane);
-                put(s0 * z * z, active, m + 5, lane);
-                put(s0 * x * x * x * x, active, m + 6, lane);
-                put(s0 * x * x * x * y, active, m + 7, lane);
-                put(s0 * x * x * x * z, active, m + 8, lane);
-                put(s0 * x * x * y * y, active, m + 9, lane);
-                put(s0 * x * x * y * z, active, m + 10, lane);
-                put(s0 * x * x * z * z, active, m + 11, lane);
-                put(s0 * x * y * y * y, active, m + 12, lane);
-                put(s0 * x * y * y * z, active, m + 13, lane);
-                put(s0 * x * y * z * z, active, m + 14, lane);
-                put(s0 * x * z * z * z, active, m + 15, lane);
-                put(s0 * y * y * y * y, active, m + 16, lane);
-                put(s0 * y * y * y * z, active, m + 17, lane);
-                put(s0 * y * y * z * z, active, m + 18, lane);
-                put(s0 * y * z * z * z, active, m + 19, lane);
-                put(s0 * z * z * z * z, active, m + 20, lane);
-                if (SIG & 1) {
-                    const double *qs = J.qscale + r * nbvec;
-                    for (int jb = 0; jb < nbvec; jb++) {
-                        const double *bv = J.bvec + 3 * jb;
-                        const double qx = qs[jb] * (x * bv[0] + y * bv[1] + z * bv[2]);
-                        put(s0 * cos_any(qx), active, m + 21 + jb, lane);
+                if constexpr (NP == 0) {
+                    double *dst = part + r * nval;
+                    put(s0, active, dst + 0, lane);
+                    for (int cc = 0; cc < L.nclass; cc++) put(cc == w.c ? 1.0 : 0.0, active, dst + 1 + cc, lane);
+                    double *m = dst + 1 + L.nclass;
+                    put(s0 * x * x, active, m + 0, lane);
+                    put(s0 * x * y, active, m + 1, lane);
+                    put(s0 * x * z, active, m + 2, lane);
+                    put(s0 * y * y, active, m + 3, lane);
+                    put(s0 * y * z, active, m + 4, lane);
+                    put(s0 * z * z, active, m + 5, lane);
+                    put(s0 * x * x * x * x, active, m + 6, lane);
+                    put(s0 * x * x * x * y, active, m + 7, lane);
+                    put(s0 * x * x * x * z, active, m + 8, lane);
+                    put(s0 * x * x * y * y, active, m + 9, lane);
+                    put(s0 * x * x * y * z, active, m + 10, lane);
+                    put(s0 * x * x * z * z, active, m + 11, lane);
+                    put(s0 * x * y * y * y, active, m + 12, lane);
+                    put(s0 * x * y * y * z, active, m + 13, lane);
+                    put(s0 * x * y * z * z, active, m + 14, lane);
+                    put(s0 * x * z * z * z, active, m + 15, lane);
+                    put(s0 * y * y * y * y, active, m + 16, lane);
+                    put(s0 * y * y * y * z, active, m + 17, lane);
+                    put(s0 * y * y * z * z, active, m + 18, lane);
+                    put(s0 * y * z * z * z, active, m + 19, lane);
+                    put(s0 * z * z * z * z, active, m + 20, lane);
+                    if (SIG & 1) {
+                        const double *qs = J.qscale + r * nbvec;
+                        for (int jb = 0; jb < nbvec; jb++) {
+                            const double *bv = J.bvec + 3 * jb;
+                            const double qx = qs[jb] * (x * bv[0] + y * bv[1] + z * bv[2]);
+                            put(s0 * cos_any(qx), active, m + 21 + jb, lane);
+                        }
                     }
+                } else if (J.cot_rec) {
+                    // adjoint: contract this record row with its cotangent
+                    double F, g0, g1, g2;
+                    adj_record(J.cot_rec + r * nval, w.c, L.nclass, nbvec, J.bvec,
+                               (SIG & 1) ? J.qscale + r * nbvec : nullptr, x, y, z, F, g0, g1, g2);
+                    const double sF = s0 * F;
+                    const double s0r = s0 * L.res;
+#pragma unroll
+                    for (int p = 0; p < NP; p++) {
+                        if (p < L.nclass - 1) {
+                            const double gJ = g0 * (double)T.Jo[p].x + g1 * (double)T.Jo[p].y + g2 * (double)T.Jo[p].z;
+                            GD[p] = GD[p] + (sF * (T.S[p] - T.dE[p]) + s0r * gJ);
+                        }
+                    }
+                    GT0 = GT0 + (sF * (w.ts0 / L.T0)) * J.invT2[0];
+                    if (L.nclass > 1) GT1 = GT1 + (sF * (w.ts1 / L.T1)) * J.invT2[1];
+                    if (L.nclass > 2) GT2 = GT2 + (sF * (w.ts2 / L.T2)) * J.invT2[2];
+                    if (L.nclass > 3) GT3 = GT3 + (sF * (w.ts3 / L.T3)) * J.invT2[3];
+                    GK = GK + sF * T.Sk;
                 }
                 r++;
             }
@@ -615,6 +797,9 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
             // final PGSE signal, main_PGSE.cu:348-357
             const double s0 = t2_weight<R>(w, L);
             double *psig = J.psig + lb * (int64_t)J.ngrad;
+            double Fs = 0.0, H[NP > 0 ? NP : 1];
+#pragma unroll
+            for (int p = 0; p < (NP > 0 ? NP : 1); p++) H[p] = 0.0;
             for (int t = 0; t < J.ntriple; t++) {
                 const int32_t *tr = J.triple + 4 * t;
                 const int ka = tr[0], kk = tr[1], ke = tr[2], g0 = tr[3], g1 = tr[7];
@@ -627,16 +812,69 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
                     const double s2 = se - sb;
                     df[k] = s1 - s2;
                 }
+                V3<double> dJ[NP > 0 ? NP : 1];
+                if constexpr (NP > 0) {
+#pragma unroll
+                    for (int p = 0; p < NP; p++) {
+                        double q3[3];
+#pragma unroll
+                        for (int k = 0; k < 3; k++) {
+                            const double s1 = ka == 0 ? 0.0 : J.tsnap[(int64_t)((ka * NP + p) * 3 + k) * ws + slot];
+                            const double se = ke == 0 ? 0.0 : J.tsnap[(int64_t)((ke * NP + p) * 3 + k) * ws + slot];
+                            const double sb = kk == 0 ? 0.0 : J.tsnap[(int64_t)((kk * NP + p) * 3 + k) * ws + slot];
+                            const double s2 = se - sb;
+                            q3[k] = s1 - s2;
+                        }
+                        dJ[p] = {q3[0], q3[1], q3[2]};
+                    }
+                }
                 for (int jg = g0; jg < g1; jg++) {
                     const double *gr = J.grad + 6 * jg;
                     const double dot = df[0] * gr[3] + df[1] * gr[4] + df[2] * gr[5];
                     const double ph = (gr[2] * dot) * a.dt;
-                    const double tsum = wave_sum(active ? s0 * cos_any(ph) : 0.0);
-                    if (lane == 0) psig[J.gperm[jg]] = tsum;
+                    if constexpr (NP == 0) {
+                        const double tsum = wave_sum(active ? s0 * cos_any(ph) : 0.0);
+                        if (lane == 0) psig[J.gperm[jg]] = tsum;
+                    } else if (J.cot_sig) {
+                        double sn, cn;
+                        sincos_any(ph, sn, cn);
+                        const double cs = J.cot_sig[J.gperm[jg]];
+                        Fs = Fs + cs * cn;
+                        const double wgt = cs * sn;
+#pragma unroll
+                        for (int p = 0; p < NP; p++) {
+                            const double dotJ = dJ[p].x * gr[3] + dJ[p].y * gr[4] + dJ[p].z * gr[5];
+                            const double dph = (gr[2] * dotJ) * a.dt;
+                            H[p] = H[p] + wgt * dph;
+                        }
+                    }
+                }
+            }
+            if constexpr (NP > 0) {
+                if (J.cot_sig) {
+                    const double sFs = s0 * Fs;
+#pragma unroll
+                    for (int p = 0; p < NP; p++)
+                        if (p < L.nclass - 1) GD[p] = GD[p] + (sFs * (T.S[p] - T.dE[p]) - s0 * H[p]);
+                    GT0 = GT0 + (sFs * (w.ts0 / L.T0)) * J.invT2[0];
+                    if (L.nclass > 1) GT1 = GT1 + (sFs * (w.ts1 / L.T1)) * J.invT2[1];
+                    if (L.nclass > 2) GT2 = GT2 + (sFs * (w.ts2 / L.T2)) * J.invT2[2];
+                    if (L.nclass > 3) GT3 = GT3 + (sFs * (w.ts3 / L.T3)) * J.invT2[3];
+                    GK = GK + sFs * T.Sk;
                 }
             }
         }
-        if (a.step1 == a.TN) {
+        if constexpr (NP > 0) {
+            // per-batch gradient partials [16]: D[1..3], T2[0..3] at 4.., kappa at 8
+            double *pg = J.pgrad + lb * NPARAM;
+#pragma unroll
+            for (int p = 0; p < NP; p++) put(GD[p], active, pg + 1 + p, lane);
+            put(GT0, active, pg + 4, lane);
+            put(GT1, active, pg + 5, lane);
+            put(GT2, active, pg + 6, lane);
+            put(GT3, active, pg + 7, lane);
+            put(GK, active, pg + 8, lane);
+        } else if (a.step1 == a.TN) {
             // diagnostics: exact integer sums
             const uint32_t cs[5] = {w.ncross, w.nrefl, w.nmirror, w.ntrunc, w.ndraw};
 #pragma unroll
@@ -714,43 +952,60 @@ __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)
 }
 
 // ------------------------------------------------------------------ launch
-template <typename R, int SIG>
-static hipError_t launch_walk_t(const LaunchArgs &a, int grid, int block, hipStream_t s)
+template <int NP>
+static const void *walk_fn(int prec_f64, int sig)
 {
-    hipLaunchKernelGGL((walk_kernel<R, SIG>), dim3(grid), dim3(block), 0, s, a);
-    return hipGetLastError();
+    if (prec_f64) {
+        switch (sig) {
+        case 0: return (const void *)walk_kernel<double, 0, NP>;
+        case 1: return (const void *)walk_kernel<double, 1, NP>;
+        case 2: return (const void *)walk_kernel<double, 2, NP>;
+        default: return (const void *)walk_kernel<double, 3, NP>;
+        }
+    }
+    switch (sig) {
+    case 0: return (const void *)walk_kernel<float, 0, NP>;
+    case 1: return (const void *)walk_kernel<float, 1, NP>;
+    case 2: return (const void *)walk_kernel<float, 2, NP>;
+    default: return (const void *)walk_kernel<float, 3, NP>;
+    }
+}
+
+static const void *walk_fn_np(int prec_f64, int sig, int np)
+{
+    switch (np) {
+    case 0: return walk_fn<0>(prec_f64, sig);
+    case 1: return walk_fn<1>(prec_f64, sig);
+    case 2: return walk_fn<2>(prec_f64, sig);
+    default: return walk_fn<3>(prec_f64, sig);
+    }
+}
+
+static hipError_t launch_fn(const void *fn, const LaunchArgs &a, int grid, int block, hipStream_t s)
+{
+    void *args[] = {(void *)&a};
+    return hipLaunchKernel(fn, dim3(grid), dim3(block), args, 0, s);
 }
 
 hipError_t launch_walk(const LaunchArgs &a, int prec_f64, int sig, int grid, int block, hipStream_t s)
 {
-    if (prec_f64) {
-        switch (sig) {
-        case 0: return launch_walk_t<double, 0>(a, grid, block, s);
-        case 1: return launch_walk_t<double, 1>(a, grid, block, s);
-        case 2: return launch_walk_t<double, 2>(a, grid, block, s);
-        default: return launch_walk_t<double, 3>(a, grid, block, s);
-        }
-    }
-    switch (sig) {
-    case 0: return launch_walk_t<float, 0>(a, grid, block, s);
-    case 1: return launch_walk_t<float, 1>(a, grid, block, s);
-    case 2: return launch_walk_t<float, 2>(a, grid, block, s);
-    default: return launch_walk_t<float, 3>(a, grid, block, s);
-    }
+    return launch_fn(walk_fn_np(prec_f64, sig, 0), a, grid, block, s);
 }
 
 hipError_t walk_occupancy(int prec_f64, int sig, int block, int *blocks_per_cu)
 {
-    const void *fn;
-    if (prec_f64)
-        fn = sig == 0 ? (const void *)walk_kernel<double, 0>
-                      : sig == 1 ? (const void *)walk_kernel<double, 1>
-                                 : sig == 2 ? (const void *)walk_kernel<double, 2> : (const void *)walk_kernel<double, 3>;
-    else
-        fn = sig == 0 ? (const void *)walk_kernel<float, 0>
-                      : sig == 1 ? (const void *)walk_kernel<float, 1>
-                                 : sig == 2 ? (const void *)walk_kernel<float, 2> : (const void *)walk_kernel<float, 3>;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, fn, block, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_fn_np(prec_f64, sig, 0), block, 0);
+}
+
+hipError_t launch_adjoint(const LaunchArgs &a, int prec_f64, int sig, int np, int grid, int block, hipStream_t s)
+{
+    return launch_fn(walk_fn_np(prec_f64, sig, np < 1 ? 1 : np), a, grid, block, s);
+}
+
+hipError_t adjoint_occupancy(int prec_f64, int sig, int np, int block, int *blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, walk_fn_np(prec_f64, sig, np < 1 ? 1 : np),
+                                                        block, 0);
 }
 
 hipError_t launch_reduce(const ReduceDesc *rd_dev, int32_t njobs, int64_t max_nq, int32_t R, hipStream_t s)

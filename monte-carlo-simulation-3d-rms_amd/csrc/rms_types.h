@@ -33,10 +33,14 @@ struct JobDesc {
     float step_f[NC], inv_step_f[NC], P_f[NC * NC];
     double step_d[NC], inv_step_d[NC], P_d[NC * NC];
     double T2s[NC];            // T2 in steps (main.cu:433)
-    // tangent (adjoint) data: d step_c / d D_c and d P / d theta
-    double dstep_dD[NC];
-    double dlogP[NC * NC * 8]; // per pair: d log P / d theta (theta: D0..D3, kappa, pad)
-    double dlog1mP[NC * NC * 8];
+    // tangent (adjoint) data; parameter p < 3 is D of class p+1, p = 3 is kappa
+    float dstep_f[NC], dinv_f[NC];    // d step_c / d D_c, d (1/step_c) / d D_c
+    double dstep_d[NC], dinv_d[NC];
+    double rT[NC];                    // 1 / T2s_c
+    double invT2[NC];                 // 1 / T2_c (ms)
+    double dlogP[NC * NC * 4];        // d log P_ij / d theta
+    double dlog1mP[NC * NC * 4];      // d log (1 - P_ij) / d theta
+    double *tsnap;                    // [(nbound+1)*NP*3][wstride] tangent prefix sums
     // narrow-pulse b-table (main.cu:374-378)
     const double *bvec;        // [nbvec*3]
     const double *qscale;      // [timepoints*nbvec] sqrt(bval/TD)

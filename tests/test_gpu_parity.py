@@ -109,3 +109,94 @@ def test_chunked_and_graph_equal_single_launch(rms, orc, engine, prec):
     gr, _ = _both(rms, orc, engine, jobs, prec, final_state=False, dt=2e-4, TN=301, timepoints=30,
                   chunk_steps=100, use_graph=True)
     _assert_same(gr, ref, final_state=False)
+
+
+def _adj_jobs(rms, orc, jobs):
+    gj, oj = [], []
+    for jb in jobs:
+        g = jb["geom"]
+        mk = lambda M: M.Geometry(g["labels"], g["res"], g["D"], T2=g.get("T2"), kappa=g.get("kappa"),  # noqa: E731
+                                  label_class=g.get("label_class"))
+        rest = {k: v for k, v in jb.items() if k != "geom"}
+        gj.append(dict(geom=mk(rms), **rest))
+        oj.append(dict(geom=mk(orc), **rest))
+    return gj, oj
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_adjoint_bitexact(rms, orc, engine, prec):
+    """Tangent pass (pathwise D, score-function D/kappa, analytic T2) against
+    the oracle's adjoint on the same seeds: gradients and their replica
+    partials are bit-identical."""
+    print(f"case adjoint {prec}", flush=True)
+    lab = random_blobs((10, 9, 11), 3, seed=3)
+    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0, 0.6], T2=[50.0, 80.0, 20.0], kappa=0.08)
+    bval = np.array([0.5, 2.0])
+    bvec = fib_dirs(2)
+    grad = pgse_table(fib_dirs(3), [0.5, 1.5], [0.03], 0.01)
+    jobs = [dict(geom=geom, nwalk=700, seed=77, bval=bval, bvec=bvec, grad=grad)]
+    dt, TN, T = 2e-4, 300, 30
+    nval = rms.nval(3, 2)
+    rng = np.random.default_rng(0)
+    cot_rec = [rng.standard_normal((T, nval))]
+    cot_sig = [rng.standard_normal(grad.shape[0])]
+    gj, oj = _adj_jobs(rms, orc, jobs)
+    plan = engine.plan(gj, dt, TN, T, precision=prec, replicas=8)
+    try:
+        fwd = plan.execute(final_state=False)
+        out = plan.adjoint(cot_rec, cot_sig, want_replicas=True)
+        out2 = plan.adjoint(cot_rec, None, want_replicas=True)
+    finally:
+        plan.close()
+    ref = orc.run(oj, dt, TN, T, prec=prec, replicas=8, cot_rec=cot_rec, cot_sig=cot_sig)
+    ref2 = orc.run(oj, dt, TN, T, prec=prec, replicas=8, cot_rec=cot_rec, cot_sig=None)
+    np.testing.assert_array_equal(fwd[0]["rec"], ref[0]["rec"])
+    np.testing.assert_array_equal(out[0]["grad"], ref[0]["grad"])
+    np.testing.assert_array_equal(out[0]["grad_rep"], ref[0]["grad_rep"])
+    np.testing.assert_array_equal(out2[0]["grad"], ref2[0]["grad"])
+    g = out[0]["grad"]
+    assert np.all(np.isfinite(g)) and g[1] != 0.0 and g[2] != 0.0 and g[8] != 0.0
+
+
+def test_adjoint_batched_jobs_bitexact(rms, orc, engine):
+    print("case adjoint batched jobs", flush=True)
+    g1 = dict(labels=planes(6, 7, 10), res=0.1, D=[0.0, 2.0], T2=[70.0, 70.0])
+    g2 = dict(labels=random_blobs((9, 8, 7), 3, seed=8, dtype=np.uint16), res=0.1, D=[0.0, 1.5, 2.0],
+              T2=[60.0, 60.0, 80.0], kappa=0.05)
+    jobs = [dict(geom=g1, nwalk=130, seed=1), dict(geom=g2, nwalk=257, seed=2)]
+    dt, TN, T = 2e-4, 240, 24
+    rng = np.random.default_rng(1)
+    cot = [rng.standard_normal((T, rms.nval(2, 0))), rng.standard_normal((T, rms.nval(3, 0)))]
+    gj, oj = _adj_jobs(rms, orc, jobs)
+    plan = engine.plan(gj, dt, TN, T, precision="f32", replicas=4)
+    try:
+        out = plan.adjoint(cot, None, want_replicas=True)
+    finally:
+        plan.close()
+    ref = orc.run(oj, dt, TN, T, prec="f32", replicas=4, cot_rec=cot)
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a["grad"], b["grad"])
+        np.testing.assert_array_equal(a["grad_rep"], b["grad_rep"])
+
+
+def test_graph_replays_match_eager(rms, orc, engine):
+    """A captured plan replayed several times, with eager runs of another plan
+    in between, gives the eager result on every replay."""
+    print("case graph replays", flush=True)
+    lab = random_blobs((8, 8, 8), 3, seed=6)
+    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0, 1.0], T2=[80.0, 70.0, 40.0])
+    grad = pgse_table(fib_dirs(2), [1.0], [0.03], 0.01)
+    jobs = [dict(geom=geom, nwalk=300, seed=8, grad=grad)]
+    gj, _ = _adj_jobs(rms, orc, jobs)
+    eager = engine.run(gj, 2e-4, 200, 20, precision="f32", replicas=4)
+    gp = engine.plan(gj, 2e-4, 200, 20, precision="f32", replicas=4, chunk_steps=64, use_graph=True)
+    try:
+        for k in range(4):
+            r = gp.execute(final_state=False)
+            np.testing.assert_array_equal(r[0]["rec"], eager[0]["rec"], err_msg=f"replay {k}")
+            np.testing.assert_array_equal(r[0]["sig"], eager[0]["sig"], err_msg=f"replay {k}")
+            np.testing.assert_array_equal(r[0]["counters"], eager[0]["counters"], err_msg=f"replay {k}")
+            other = engine.run(gj, 2e-4, 100, 10, precision="f64", replicas=2)
+            assert np.isfinite(other[0]["rec"]).all()
+    finally:
+        gp.close()

@@ -120,7 +120,11 @@ typedef struct {
 } rms_timing;
 
 /* Adjoint outputs: derivatives of sum(cot_rec * rec) + sum(cot_sig * sig)
- * with respect to D[c] (index c), T2[c] (4 + c) and kappa (8). */
+ * with respect to D[c] (index c), T2[c] (4 + c) and kappa (8).
+ * Estimators: T2 analytic (weights only); kappa score function of the
+ * exchange draws; D pathwise (faces fixed, step ~ sqrt(D)).  The pathwise D
+ * derivative is exact only while no walker can change class: for a geometry
+ * with exchange between non-null classes the D entries are NaN. */
 typedef struct {
     const double *cot_rec;       /* [timepoints * nval] or NULL */
     const double *cot_sig;       /* [ngrad] or NULL */

@@ -46,6 +46,7 @@ struct DevBuf {
 };
 
 struct JobHost {
+    bool exchange;      // walkers can cross between different non-null classes
     int32_t geom_id;
     int64_t nwalk, nbatch, batch0, wbase;
     int32_t nbvec, ngrad, nval, nbound, ntriple;
@@ -484,6 +485,10 @@ extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t
         }
         derive_tangent(G.meta, P.dt, step, inv_step, T2s, Pp, Pcat, D);
         JobHost &H = p->jobs[j];
+        H.exchange = false;
+        for (int a = 1; a < G.meta.nclass; a++)
+            for (int b = 1; b < G.meta.nclass; b++)
+                if (a != b && Pcat[a * NC + b] != 0) H.exchange = true;
         H.geom_id = J.geom_id;
         H.nwalk = J.nwalk;
         H.nbatch = (J.nwalk + WAVE - 1) / WAVE;
@@ -936,6 +941,15 @@ extern "C" rms_status rms_plan_adjoint(rms_plan *p, const rms_job_adj *adj, rms_
         HIPCHK(ctx, hipMemcpy(adj[j].grad, rd.tot, sizeof(double) * NPARAM, hipMemcpyDeviceToHost));
         if (adj[j].grad_rep)
             HIPCHK(ctx, hipMemcpy(adj[j].grad_rep, rd.rep, sizeof(double) * NPARAM * R, hipMemcpyDeviceToHost));
+        if (p->jobs[j].exchange) {
+            // a crossing between non-null classes changes the class (and the
+            // step length) discontinuously in D: no pathwise D derivative
+            for (int c = 1; c < NC; c++) {
+                adj[j].grad[c] = NAN;
+                if (adj[j].grad_rep)
+                    for (int r = 0; r < R; r++) adj[j].grad_rep[(size_t)r * NPARAM + c] = NAN;
+            }
+        }
     }
     return RMS_OK;
 }

@@ -565,16 +565,16 @@ __device__ __forceinline__ void save_state(const Walker<R> &w, const StateBuf &s
 }
 
 // Contraction of one record row with its cotangent (adjoint pass):
-// F = sum_q cot_q m_q(d) (m_q = record value without the T2 weight) and
-// dF/dd.  Same operation order as the host restatement.
-__device__ __forceinline__ void adj_record(const double *cot, int c, int nclass, int nbvec, const double *bvec,
+// F = sum_q cot_q m_q(d) over the T2-weighted columns (m_q = record value
+// without the weight: sig0, dx2, dx4, sigRe) and dF/dd.  The unweighted
+// NPar_count column is handled by the caller.
+__device__ __forceinline__ void adj_record(const double *cot, int nclass, int nbvec, const double *bvec,
                                            const double *qs, double x, double y, double z, double &F, double &g0,
                                            double &g1, double &g2)
 {
     const double d[3] = {x, y, z};
     double g[3] = {0.0, 0.0, 0.0};
     double f = cot[0];
-    f = f + cot[1 + c];
     const double *c2 = cot + 1 + nclass;
     constexpr int DI[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
 #pragma unroll
@@ -773,22 +773,24 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
                 } else if (J.cot_rec) {
                     // adjoint: contract this record row with its cotangent
                     double F, g0, g1, g2;
-                    adj_record(J.cot_rec + r * nval, w.c, L.nclass, nbvec, J.bvec,
+                    adj_record(J.cot_rec + r * nval, L.nclass, nbvec, J.bvec,
                                (SIG & 1) ? J.qscale + r * nbvec : nullptr, x, y, z, F, g0, g1, g2);
                     const double sF = s0 * F;
                     const double s0r = s0 * L.res;
+                    // NPar_count is unweighted: only the score terms
+                    const double fc = J.cot_rec[r * nval + 1 + w.c];
 #pragma unroll
                     for (int p = 0; p < NP; p++) {
                         if (p < L.nclass - 1) {
                             const double gJ = g0 * (double)T.Jo[p].x + g1 * (double)T.Jo[p].y + g2 * (double)T.Jo[p].z;
-                            GD[p] = GD[p] + (sF * (T.S[p] - T.dE[p]) + s0r * gJ);
+                            GD[p] = GD[p] + ((sF * (T.S[p] - T.dE[p]) + s0r * gJ) + fc * T.S[p]);
                         }
                     }
                     GT0 = GT0 + (sF * (w.ts0 / L.T0)) * J.invT2[0];
                     if (L.nclass > 1) GT1 = GT1 + (sF * (w.ts1 / L.T1)) * J.invT2[1];
                     if (L.nclass > 2) GT2 = GT2 + (sF * (w.ts2 / L.T2)) * J.invT2[2];
                     if (L.nclass > 3) GT3 = GT3 + (sF * (w.ts3 / L.T3)) * J.invT2[3];
-                    GK = GK + sF * T.Sk;
+                    GK = GK + (sF + fc) * T.Sk;
                 }
                 r++;
             }

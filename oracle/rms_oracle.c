@@ -379,6 +379,15 @@ typedef struct {
 /* d log P_ij and d log(1 - P_ij) with respect to D_1..D_3 and kappa for the
  * probabilistic pairs (Pcat = 2): P = min(1, sqrt(Dj/Di)) * pi0/(1+pi0),
  * pi0 = 4 kappa dt / (res min(s_i, s_j)). */
+/* 1 when walkers can move between two different non-null classes. */
+static int or_has_exchange(const or_derived *d)
+{
+    for (int i = 1; i < d->nclass; i++)
+        for (int j = 1; j < d->nclass; j++)
+            if (i != j && d->Pcat[i * OR_NC_MAX + j] != 0) return 1;
+    return 0;
+}
+
 static void or_derive_tangent(const or_geom *g, double dt, const or_derived *d, or_tderiv *t)
 {
     memset(t, 0, sizeof(*t));
@@ -424,13 +433,15 @@ static const int OR_WIND[15][4] = {{0, 0, 0, 0}, {0, 0, 0, 1}, {0, 0, 0, 2}, {0,
                                    {0, 0, 2, 2}, {0, 1, 1, 1}, {0, 1, 1, 2}, {0, 1, 2, 2}, {0, 2, 2, 2},
                                    {1, 1, 1, 1}, {1, 1, 1, 2}, {1, 1, 2, 2}, {1, 2, 2, 2}, {2, 2, 2, 2}};
 
-/* F = sum_q cot_q m_q(d) over one record row (m_q = the record value without
- * the T2 weight) and its gradient with respect to the displacement d. */
+/* F = sum_q cot_q m_q(d) over the T2-weighted columns of one record row
+ * (m_q = the record value without the T2 weight: sig0, dx2, dx4, sigRe) and
+ * its gradient with respect to the displacement d.  The NPar_count column is
+ * not weighted; the caller adds its cotangent cot[1 + c] separately. */
 static void or_adj_record(const double d[3], int c, const double *cot, int nclass, int nbvec, const double *bvec,
                           const double *qs, double *F, double g[3])
 {
     double f = cot[0];
-    f = f + cot[1 + c];
+    (void)c;
     g[0] = 0.0;
     g[1] = 0.0;
     g[2] = 0.0;
@@ -777,6 +788,16 @@ static int or_run_impl(const or_job *jobs, int32_t njobs, const or_params *p, or
                 adj[jb].grad[q] = s;
             }
             if (adj[jb].grad_rep) memcpy(adj[jb].grad_rep, grep, sizeof(double) * R * 16);
+            /* With exchange between non-null compartments a crossing changes
+             * the class (and the step length) discontinuously in D: the
+             * pathwise D derivative misses that jump, so it is reported as NaN. */
+            if (or_has_exchange(&d)) {
+                for (int c = 1; c < OR_NC_MAX; c++) {
+                    adj[jb].grad[c] = NAN;
+                    if (adj[jb].grad_rep)
+                        for (int r = 0; r < R; r++) adj[jb].grad_rep[(size_t)r * 16 + c] = NAN;
+                }
+            }
             free(grep);
             free(pgrad);
         }

@@ -117,7 +117,9 @@ int or_march_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double 
 int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs);
 
 /* Adjoint pass: derivatives of sum(cot_rec*rec) + sum(cot_sig*sig) with
- * respect to D[c] (grad[c], c = 1..3), T2[c] (grad[4+c]) and kappa (grad[8]). */
+ * respect to D[c] (grad[c], c = 1..3), T2[c] (grad[4+c]) and kappa (grad[8]).
+ * D entries are NaN when walkers can cross between non-null classes (the
+ * pathwise derivative misses the jump of a class change). */
 typedef struct {
     const double *cot_rec;  /* [timepoints*nval] or NULL */
     const double *cot_sig;  /* [ngrad] or NULL */

@@ -123,14 +123,15 @@ def _adj_jobs(rms, orc, jobs):
     return gj, oj
 
 
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_adjoint_bitexact(rms, orc, engine, prec):
-    """Tangent pass (pathwise D, score-function D/kappa, analytic T2) against
+@pytest.mark.parametrize("prec,kappa", [("f32", 0.08), ("f64", 0.08), ("f32", 0.0), ("f64", 0.0)])
+def test_adjoint_bitexact(rms, orc, engine, prec, kappa):
+    """Tangent pass (pathwise D, score-function kappa, analytic T2) against
     the oracle's adjoint on the same seeds: gradients and their replica
-    partials are bit-identical."""
-    print(f"case adjoint {prec}", flush=True)
+    partials are bit-identical.  kappa = 0: impermeable compartments, the D
+    entries are defined; kappa > 0: exchange, D entries NaN on both sides."""
+    print(f"case adjoint {prec} kappa={kappa}", flush=True)
     lab = random_blobs((10, 9, 11), 3, seed=3)
-    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0, 0.6], T2=[50.0, 80.0, 20.0], kappa=0.08)
+    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0, 0.6], T2=[50.0, 80.0, 20.0], kappa=kappa)
     bval = np.array([0.5, 2.0])
     bvec = fib_dirs(2)
     grad = pgse_table(fib_dirs(3), [0.5, 1.5], [0.03], 0.01)
@@ -155,7 +156,11 @@ def test_adjoint_bitexact(rms, orc, engine, prec):
     np.testing.assert_array_equal(out[0]["grad_rep"], ref[0]["grad_rep"])
     np.testing.assert_array_equal(out2[0]["grad"], ref2[0]["grad"])
     g = out[0]["grad"]
-    assert np.all(np.isfinite(g)) and g[1] != 0.0 and g[2] != 0.0 and g[8] != 0.0
+    assert np.all(np.isfinite(g[4:9])) and g[5] != 0.0 and g[6] != 0.0
+    if kappa > 0:
+        assert np.all(np.isnan(g[1:4])) and g[8] != 0.0
+    else:
+        assert np.all(np.isfinite(g[1:4])) and g[1] != 0.0 and g[2] != 0.0 and g[8] == 0.0
 
 
 def test_adjoint_batched_jobs_bitexact(rms, orc, engine):
@@ -200,3 +205,32 @@ def test_graph_replays_match_eager(rms, orc, engine):
             assert np.isfinite(other[0]["rec"]).all()
     finally:
         gp.close()
+
+
+def test_autograd_matches_adjoint(rms, orc, engine):
+    """torch.autograd through autograd.simulate: the backward pass is the
+    adjoint pass, bit-identical to the oracle's gradient."""
+    import importlib
+    import torch
+    ag = importlib.import_module(rms.__name__ + ".autograd")
+    print("case autograd", flush=True)
+    lab = random_blobs((9, 9, 9), 3, seed=12)
+    grad = pgse_table(fib_dirs(2), [1.0], [0.03], 0.01)
+    dt, TN, T = 2e-4, 200, 10
+    D = torch.tensor([0.0, 2.0, 0.9], dtype=torch.float64, requires_grad=True)
+    T2 = torch.tensor([40.0, 70.0, 30.0], dtype=torch.float64, requires_grad=True)
+    kap = torch.tensor(0.0, dtype=torch.float64, requires_grad=True)
+    rec, sig = ag.simulate(engine, lab, 0.1, D, T2, nwalk=300, dt=dt, TN=TN, timepoints=T, kappa=kap, seed=4,
+                           grad=grad, precision="f32", replicas=4)
+    rng = np.random.default_rng(2)
+    wr = torch.from_numpy(rng.standard_normal(tuple(rec.shape)))
+    ws = torch.from_numpy(rng.standard_normal(tuple(sig.shape)))
+    loss = (wr * rec).sum() + (ws * sig).sum()
+    loss.backward()
+    og = orc.Geometry(lab, 0.1, [0.0, 2.0, 0.9], T2=[40.0, 70.0, 30.0], kappa=0.0)
+    ref = orc.run([dict(geom=og, nwalk=300, seed=4, grad=grad)], dt, TN, T, prec="f32", replicas=4,
+                  cot_rec=[wr.numpy()], cot_sig=[ws.numpy()])[0]
+    np.testing.assert_array_equal(rec.detach().numpy(), ref["rec"])
+    np.testing.assert_array_equal(D.grad.numpy(), ref["grad"][:3])
+    np.testing.assert_array_equal(T2.grad.numpy(), ref["grad"][4:7])
+    assert float(kap.grad) == ref["grad"][8]

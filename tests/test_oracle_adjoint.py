@@ -1,0 +1,187 @@
+"""The oracle's adjoint pass against finite differences of its own forward
+pass (common random numbers: the same seed gives the same direction and
+exchange draws, so F(theta + h) - F(theta - h) isolates the parameter).
+
+Pins, per estimator:
+  T2      analytic weight derivative; paths do not depend on T2, so the
+          central difference matches to rounding.
+  D       pathwise derivative (faces fixed, step length ~ sqrt(D)) on a
+          homogeneous lattice, between impermeable planes (reflections are
+          continuous in D) and in impermeable compartments, records and PGSE
+          signal; free diffusion dD(t)/dD0 = 1 and dK(t)/dD0 = 0.  With
+          exchange the class change at a crossing is a jump the pathwise
+          derivative misses: the D entries are NaN there.
+  kappa   score function of the exchange draws (the forward map is
+          piecewise constant in P, so the check is statistical: batch-means
+          error bars on both sides, large-step central differences).
+The GPU engine reproduces these gradients bit for bit (test_gpu_parity).
+"""
+import numpy as np
+import pytest
+
+from helpers import fib_dirs, pgse_table, planes, random_blobs
+
+DT = 2e-4
+
+
+def _F(res, cot, cots):
+    v = float(np.sum(cot * res["rec"]))
+    if cots is not None:
+        v += float(np.sum(cots * res["sig"]))
+    return v
+
+
+def _job(orc, lab, D, T2, kappa=None, nwalk=512, seed=7, **kw):
+    g = orc.Geometry(lab, 0.1, D, T2=T2, kappa=kappa)
+    return [dict(geom=g, nwalk=nwalk, seed=seed, **kw)]
+
+
+def _scaled_cot(rng, T, nclass, nbvec):
+    # comparable magnitudes for sig0, counts, dx2 (um^2) and dx4 (um^4)
+    sc = np.r_[1.0, np.ones(nclass), np.full(6, 1e2), np.full(15, 1e4), np.ones(nbvec)]
+    return rng.standard_normal((T, sc.size)) * sc
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_t2_gradient_matches_central_difference(orc, prec):
+    lab = random_blobs((9, 9, 9), 3, seed=2)
+    D, T2 = [0.0, 2.0, 1.0], [np.inf, 60.0, 25.0]
+    bval, bvec = np.array([1.0, 2.5]), fib_dirs(2)
+    grad = pgse_table(fib_dirs(2), [1.0], [0.03], 0.01)
+    TN, T = 200, 10
+    rng = np.random.default_rng(3)
+    cot = _scaled_cot(rng, T, 3, 2)
+    cots = rng.standard_normal(grad.shape[0])
+    kw = dict(bval=bval, bvec=bvec, grad=grad)
+    ra = orc.run(_job(orc, lab, D, T2, 0.2, **kw), DT, TN, T, prec=prec, cot_rec=[cot], cot_sig=[cots])[0]
+    g = ra["grad"]
+    assert g[4] == 0.0  # class 0 has T2 = inf
+    for c in (1, 2):
+        h = 1e-4 * T2[c]
+        tp, tm = list(T2), list(T2)
+        tp[c] += h
+        tm[c] -= h
+        fp = _F(orc.run(_job(orc, lab, D, tp, 0.2, **kw), DT, TN, T, prec=prec)[0], cot, cots)
+        fm = _F(orc.run(_job(orc, lab, D, tm, 0.2, **kw), DT, TN, T, prec=prec)[0], cot, cots)
+        fd = (fp - fm) / (2 * h)
+        assert g[4 + c] == pytest.approx(fd, rel=1e-6), (c, g[4 + c], fd)
+        assert abs(fd) > 0
+
+
+@pytest.mark.parametrize("geom", ["homogeneous", "planes"])
+def test_d_pathwise_gradient_matches_central_difference(orc, geom):
+    if geom == "homogeneous":
+        lab = np.ones((12, 12, 12), np.uint8)
+    else:
+        lab = planes(6, 6, 9)  # walls at z = 0 and z = 8: many reflections
+    D, T2 = [0.0, 1.7], [np.inf, 70.0]
+    bval, bvec = np.array([2.0, 4.0]), fib_dirs(2)
+    grad = pgse_table(fib_dirs(3), [1.0, 3.0], [0.03], 0.012)
+    TN, T = 240, 12
+    rng = np.random.default_rng(5)
+    cot = _scaled_cot(rng, T, 2, 2)
+    cots = rng.standard_normal(grad.shape[0])
+    kw = dict(bval=bval, bvec=bvec, grad=grad)
+    ra = orc.run(_job(orc, lab, D, T2, **kw), DT, TN, T, prec="f64", cot_rec=[cot], cot_sig=[cots])[0]
+    g = ra["grad"][1]
+    h = 1e-6 * D[1]
+    fp = _F(orc.run(_job(orc, lab, [0.0, D[1] + h], T2, **kw), DT, TN, T, prec="f64")[0], cot, cots)
+    fm = _F(orc.run(_job(orc, lab, [0.0, D[1] - h], T2, **kw), DT, TN, T, prec="f64")[0], cot, cots)
+    fd = (fp - fm) / (2 * h)
+    assert g == pytest.approx(fd, rel=2e-4), (g, fd)
+    if geom == "planes":
+        r = orc.run(_job(orc, lab, D, T2, **kw), DT, TN, T, prec="f64")[0]
+        assert r["counters"][1] > 1000  # the reflections were exercised
+
+
+def test_d_gradient_impermeable_compartments(orc):
+    """Two non-null classes separated by impermeable membranes (kappa = 0):
+    no walker changes class, so the pathwise D derivative is exact for both."""
+    lab = random_blobs((9, 9, 9), 3, seed=4)
+    D, T2 = [0.0, 2.0, 0.8], [np.inf, 70.0, 35.0]
+    TN, T = 200, 10
+    rng = np.random.default_rng(9)
+    cot = _scaled_cot(rng, T, 3, 0)
+    ra = orc.run(_job(orc, lab, D, T2, 0.0), DT, TN, T, prec="f64", cot_rec=[cot])[0]
+    for c in (1, 2):
+        h = 1e-6 * D[c]
+        dp, dm = list(D), list(D)
+        dp[c] += h
+        dm[c] -= h
+        fd = (_F(orc.run(_job(orc, lab, dp, T2, 0.0), DT, TN, T, prec="f64")[0], cot, None)
+              - _F(orc.run(_job(orc, lab, dm, T2, 0.0), DT, TN, T, prec="f64")[0], cot, None)) / (2 * h)
+        assert ra["grad"][c] == pytest.approx(fd, rel=2e-4), (c, ra["grad"][c], fd)
+
+
+def test_free_diffusion_dD_dD0_is_one_and_dK_dD0_is_zero(orc):
+    """Free diffusion: the apparent D(t) = <dz^2>/(2t) has dD/dD0 = 1 and the
+    kurtosis K(t) = <dz^4>/<dz^2>^2 - 3 has dK/dD0 = 0 (scale invariance of
+    a path whose step length is sqrt(6 D0 dt)).  The lattice edges are
+    mirrors, which keep the scaling."""
+    n = 48
+    lab = np.ones((n, n, n), np.uint8)
+    D0 = 1.5
+    TN, T, N = 200, 4, 4096
+    TD = orc.diff_times(DT, TN, T)
+    res = orc.run(_job(orc, lab, [0.0, D0], [np.inf, np.inf], nwalk=N), DT, TN, T, prec="f64")[0]
+    rec = res["rec"]
+    iz2, iz4 = 1 + 2 + 5, 1 + 2 + 6 + 14  # dx2 zz, dx4 zzzz columns
+    # D_est = sum_t dz2_t / (N 2 TD_t): cotangent on the dz2 column
+    cot = np.zeros_like(rec)
+    cot[:, iz2] = 1.0 / (N * 2.0 * TD)
+    gD = orc.run(_job(orc, lab, [0.0, D0], [np.inf, np.inf], nwalk=N), DT, TN, T, prec="f64",
+                 cot_rec=[cot])[0]["grad"][1]
+    Dest = rec[:, iz2] / (N * 2.0 * TD)
+    # each path is the unit path scaled by sqrt(D0): dD_est/dD0 = D_est/D0 exactly
+    assert gD == pytest.approx(Dest.sum() / D0, rel=1e-9)
+    assert gD / T == pytest.approx(1.0, abs=0.05)  # and D_est estimates D0
+    # K_t = (dz4/N) / (dz2/N)^2 - 3, summed over t: chain rule for the cotangent
+    m2, m4 = rec[:, iz2] / N, rec[:, iz4] / N
+    cotK = np.zeros_like(rec)
+    cotK[:, iz4] = 1.0 / (N * m2 ** 2)
+    cotK[:, iz2] = -2.0 * m4 / (N * m2 ** 3)
+    gK = orc.run(_job(orc, lab, [0.0, D0], [np.inf, np.inf], nwalk=N), DT, TN, T, prec="f64",
+                 cot_rec=[cotK])[0]["grad"][1]
+    assert abs(gK) < 1e-9 * T / D0
+
+
+def test_d_gradient_is_nan_with_exchange(orc):
+    lab = random_blobs((8, 8, 8), 3, seed=4)
+    cot = np.ones((5, orc.nval(3, 0)))
+    for kappa in (0.3, None):  # permeable membranes, or the sqrt(Dj/Di) rule
+        r = orc.run(_job(orc, lab, [0.0, 2.0, 1.0], [np.inf, 50.0, 30.0], kappa), DT, 50, 5, prec="f64",
+                    replicas=4, cot_rec=[cot])[0]
+        assert np.all(np.isnan(r["grad"][1:4])) and np.all(np.isnan(r["grad_rep"][:, 1:4]))
+        assert np.all(np.isfinite(r["grad"][4:9]))
+
+
+def _batch_stats(x):
+    R = x.shape[0]
+    return x.sum(), x.std(ddof=1) * np.sqrt(R)
+
+
+def test_kappa_score_gradient_matches_finite_difference(orc):
+    """Membrane exchange (P between 0 and 1): the score-function gradient and a
+    central difference of the expectation agree within their error bars."""
+    w, nper = 5, 3
+    z = np.arange(2 * w * nper)
+    cls = np.where(((z + w // 2) // w) % 2 == 0, 1, 2).astype(np.uint8)
+    lab = np.ascontiguousarray(np.broadcast_to(cls, (4, 4, z.size)))
+    D, T2, kap = [0.0, 2.0, 1.2], [np.inf, 80.0, 40.0], 0.5
+    TN, T, R, N = 800, 8, 32, 1 << 13
+    cot = np.zeros((T, orc.nval(3, 0)))
+    cot[:, 2] = 1.0          # walkers in class 1
+    cot[:, 1 + 3 + 5] = 50.0  # sum of dz^2 (um^2)
+    ra = orc.run(_job(orc, lab, D, T2, kap, nwalk=N, seed=11), DT, TN, T, prec="f64", replicas=R,
+                 cot_rec=[cot])[0]
+    adj, adj_se = _batch_stats(ra["grad_rep"][:, 8])
+
+    def L(d, k):
+        r = orc.run(_job(orc, lab, d, T2, k, nwalk=N, seed=11), DT, TN, T, prec="f64", replicas=R)[0]
+        return (cot[None] * r["rec_rep"]).reshape(R, -1).sum(1)
+
+    h = 0.2 * kap
+    fdr = (L(D, kap + h) - L(D, kap - h)) / (2 * h)
+    fd, fd_se = _batch_stats(fdr)
+    assert abs(adj) > 4 * adj_se  # a real signal, not noise
+    assert abs(adj - fd) < 4 * np.hypot(adj_se, fd_se) + 0.03 * abs(fd), (adj, adj_se, fd, fd_se)

@@ -52,6 +52,34 @@ def akc_mom(sig0, dx2, dx4, TD, n):
     return K, D
 
 
+def akc_cotangent(rec, nclass, TD, n, wD=None, wK=None):
+    """Cotangent of L = sum_t wD[t] D_t(n) + wK[t] K_t(n) with respect to the
+    record sums rec [timepoints, nval] (chain rule through akc_mom and the
+    sig0 normalisation), for the adjoint pass.  D_t = a2/(2 TD sig0),
+    K_t = a4 sig0 / a2^2 - 3 with a2 = dx2 . (D_CNT n2), a4 = dx4 . (W_CNT n4)."""
+    rec = np.asarray(rec, float)
+    T = rec.shape[0]
+    TD = np.asarray(TD, float)
+    wD = np.ones(T) if wD is None else np.asarray(wD, float)
+    wK = np.ones(T) if wK is None else np.asarray(wK, float)
+    c2 = D_CNT * ndir2(n)[0]
+    c4 = W_CNT * ndir4(n)[0]
+    i2 = 1 + nclass
+    i4 = i2 + 6
+    sig0 = rec[:, 0]
+    a2 = rec[:, i2:i2 + 6] @ c2
+    a4 = rec[:, i4:i4 + 15] @ c4
+    cot = np.zeros_like(rec)
+    # D_t
+    cot[:, i2:i2 + 6] += (wD / (2.0 * TD * sig0))[:, None] * c2
+    cot[:, 0] += -wD * a2 / (2.0 * TD * sig0 ** 2)
+    # K_t
+    cot[:, i4:i4 + 15] += (wK * sig0 / a2 ** 2)[:, None] * c4
+    cot[:, i2:i2 + 6] += (-2.0 * wK * a4 * sig0 / a2 ** 3)[:, None] * c2
+    cot[:, 0] += wK * a4 / a2 ** 2
+    return cot
+
+
 def narrow_pulse_signal(sig0, sigRe):
     """sig = |sigRe| / sig0 (readRMS.m:119-120)."""
     return np.abs(np.asarray(sigRe)) / np.asarray(sig0, float)[:, None]

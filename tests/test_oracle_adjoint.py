@@ -185,3 +185,34 @@ def test_kappa_score_gradient_matches_finite_difference(orc):
     fd, fd_se = _batch_stats(fdr)
     assert abs(adj) > 4 * adj_se  # a real signal, not noise
     assert abs(adj - fd) < 4 * np.hypot(adj_se, fd_se) + 0.03 * abs(fd), (adj, adj_se, fd, fd_se)
+
+
+def test_akc_cotangent_matches_numeric_derivative(rms):
+    """analysis.akc_cotangent (the chain rule from D(t), K(t) to the record
+    sums) against central differences of analysis.akc_mom."""
+    import importlib
+    A = importlib.import_module(rms.__name__ + ".analysis")
+    rng = np.random.default_rng(4)
+    T, nclass = 5, 2
+    nv = rms.nval(nclass, 0)
+    rec = rng.uniform(0.5, 2.0, (T, nv))
+    rec[:, 0] = rng.uniform(900, 1000, T)
+    TD = np.linspace(0.1, 0.5, T)
+    n = np.array([0.3, -0.5, 0.8])
+    n /= np.linalg.norm(n)
+    wD, wK = rng.standard_normal(T), rng.standard_normal(T)
+
+    def L(r):
+        s = A.split(r, nclass)
+        K, D = A.akc_mom(s["sig0"], s["dx2"], s["dx4"], TD, n)
+        return float(np.sum(wD * D[:, 0]) + np.sum(wK * K[:, 0]))
+
+    cot = A.akc_cotangent(rec, nclass, TD, n, wD, wK)
+    num = np.zeros_like(rec)
+    for idx in np.ndindex(rec.shape):
+        h = 1e-6 * abs(rec[idx])
+        rp, rm = rec.copy(), rec.copy()
+        rp[idx] += h
+        rm[idx] -= h
+        num[idx] = (L(rp) - L(rm)) / (2 * h)
+    np.testing.assert_allclose(cot, num, rtol=1e-6, atol=1e-9 * np.abs(num).max())

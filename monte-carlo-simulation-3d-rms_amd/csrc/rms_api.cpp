@@ -24,8 +24,8 @@ using namespace rms;
 
 struct GeomDev {
     rms_geometry meta;     // labels / label_class pointers cleared
-    void *d_labels = nullptr;
-    uint8_t *d_lclass = nullptr;
+    uint32_t *d_cells = nullptr;  // cell words (straight-move and null-face bits)
+    uint8_t *d_cls = nullptr;     // class per voxel
     int64_t nvox = 0;
     bool has_walkable = false;
 };
@@ -143,8 +143,8 @@ extern "C" rms_status rms_destroy(rms_ctx *ctx)
     if (!ctx) return RMS_ERR_INVALID_ARG;
     hipSetDevice(ctx->device);
     for (auto &g : ctx->geoms) {
-        if (g.d_labels) hipFree(g.d_labels);
-        if (g.d_lclass) hipFree(g.d_lclass);
+        if (g.d_cells) hipFree(g.d_cells);
+        if (g.d_cls) hipFree(g.d_cls);
     }
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -301,12 +301,29 @@ extern "C" rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, i
     gd.meta.label_class = nullptr;
     gd.nvox = nvox;
     gd.has_walkable = walk;
-    size_t bytes = (size_t)nvox * g.label_bytes;
-    HIPCHK(ctx, hipMalloc(&gd.d_labels, bytes));
-    HIPCHK(ctx, hipMemcpy(gd.d_labels, g.labels, bytes, hipMemcpyHostToDevice));
-    if (g.label_class) {
-        HIPCHK(ctx, hipMalloc((void **)&gd.d_lclass, (size_t)g.nlabel));
-        HIPCHK(ctx, hipMemcpy(gd.d_lclass, g.label_class, (size_t)g.nlabel, hipMemcpyHostToDevice));
+    // the labels and class map travel once; the kernels read only what is
+    // built from them: a class per voxel (uint8) and a cell word (uint32)
+    const size_t bytes = (size_t)nvox * g.label_bytes;
+    void *d_lab = nullptr;
+    uint8_t *d_map = nullptr;
+    hipError_t e = hipMalloc(&gd.d_cells, sizeof(uint32_t) * (size_t)nvox);
+    if (e == hipSuccess) e = hipMalloc(&gd.d_cls, (size_t)nvox);
+    if (e == hipSuccess) e = hipMalloc(&d_lab, bytes);
+    if (e == hipSuccess) e = hipMemcpy(d_lab, g.labels, bytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess && g.label_class) {
+        e = hipMalloc((void **)&d_map, (size_t)g.nlabel);
+        if (e == hipSuccess) e = hipMemcpy(d_map, g.label_class, (size_t)g.nlabel, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess)
+        e = launch_build_cells(d_lab, g.label_bytes, d_map, g.nx, g.ny, g.nz, gd.d_cells, gd.d_cls, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (d_lab) hipFree(d_lab);
+    if (d_map) hipFree(d_map);
+    if (e != hipSuccess) {
+        if (gd.d_cells) hipFree(gd.d_cells);
+        if (gd.d_cls) hipFree(gd.d_cls);
+        ctx->err = std::string("geometry upload: ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? RMS_ERR_OUT_OF_MEMORY : RMS_ERR_HIP;
     }
     ctx->geoms.push_back(gd);
     *geom_id = (int32_t)ctx->geoms.size() - 1;
@@ -456,12 +473,14 @@ extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t
                 BAD("step size sqrt(6*D*dt) must be below one voxel (main.cu:210-214 assumes it)");
             if (!(G.meta.T2[c] > 0.0)) BAD("T2 must be positive");
         }
-        D.labels = G.d_labels;
-        D.lclass = G.d_lclass;
+        D.cells = G.d_cells;
+        D.cls = G.d_cls;
+        // measured on the single-axon workload: lockstep (1) is best with FP32
+        // offsets, letting 3 lanes gather before a march is best with FP64
+        D.slow_min = P.slow_lanes > 0 ? P.slow_lanes : (P.precision == RMS_PREC_F64 ? 3 : 1);
         D.nx = G.meta.nx;
         D.ny = G.meta.ny;
         D.nz = G.meta.nz;
-        D.label_bytes = G.meta.label_bytes;
         D.nclass = G.meta.nclass;
         D.nbvec = J.nbvec;
         D.ngrad = J.ngrad;

@@ -20,6 +20,9 @@ hipError_t launch_walk(const LaunchArgs &a, int prec_f64, int sig, int grid, int
 hipError_t walk_occupancy(int prec_f64, int sig, int block, int *blocks_per_cu);
 hipError_t launch_reduce(const ReduceDesc *rd_dev, int32_t nred, int64_t max_nq, int32_t R, hipStream_t s);
 hipError_t launch_fill_zero(void *p, int64_t bytes, hipStream_t s);
+// Cell words and classes of a label lattice (see rms_types.h).
+hipError_t launch_build_cells(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
+                              int32_t ny, int32_t nz, uint32_t *cells, uint8_t *cls, hipStream_t s);
 
 // Tangent (adjoint) pass: same walk with forward-mode tangents, contracted
 // with the record/signal cotangents per walker.

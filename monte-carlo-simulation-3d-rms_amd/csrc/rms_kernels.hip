@@ -3,11 +3,22 @@
 // One lane = one walker (hpc_code/lib/main.cu:64-70), but the grid is
 // persistent: each wavefront walks batches of 64 walkers taken from a batch
 // list that can span many geometries (jobs).  Walker state lives in VGPRs for
-// the whole chunk of steps; the lattice is read as uint8/uint16 labels through
-// L2 / Infinity Cache.  Per-record moments are summed across the wavefront
-// with a fixed butterfly and written as per-batch partials; a second kernel
-// sums the partials in a fixed order, so results are bit-reproducible and
-// identical to the host restatement.
+// the whole chunk of steps; the lattice is read as one 32-bit cell word per
+// voxel (class + which straight moves stay inside same-class voxels) through
+// L2 / Infinity Cache.
+//
+// A step whose straight move only crosses same-class faces is a few adds (the
+// fast path); the others need the face march.  Lanes do not step in lockstep
+// inside a record window: every pass, each lane whose next step is fast takes
+// it, and lanes that need the march wait until enough of them (slow_min) are
+// waiting, then march together.  The window closes when every lane has done
+// the record step.  Each walker's trajectory is the same as in a lockstep
+// walk; only the schedule differs.
+//
+// Per-record moments are summed across the wavefront with a fixed butterfly
+// and written as per-batch partials; a second kernel sums the partials in a
+// fixed order, so results are bit-reproducible and identical to the host
+// restatement.
 //
 // Reference anchors:
 //   propagate (narrow pulse)  main.cu:63-384
@@ -85,10 +96,10 @@ struct Tan {
 // Per-job constants held in registers for the whole batch.
 template <typename R>
 struct JobLocal {
-    const void *labels;
-    const uint8_t *lclass;
+    const uint32_t *cells;
+    const uint8_t *cls;
     int32_t nx, ny, nz, sx;  // sx = ny*nz
-    int32_t label_bytes, nclass;
+    int32_t slow_min, nclass;
     uint32_t key0, key1, pcat;
     R st0, st1, st2, st3;    // step per class (voxels)
     R is0, is1, is2, is3;    // 1/step
@@ -111,13 +122,13 @@ template <typename R>
 __device__ __forceinline__ JobLocal<R> job_local(const JobDesc &J)
 {
     JobLocal<R> L;
-    L.labels = J.labels;
-    L.lclass = J.lclass;
+    L.cells = J.cells;
+    L.cls = J.cls;
     L.nx = J.nx;
     L.ny = J.ny;
     L.nz = J.nz;
     L.sx = J.ny * J.nz;
-    L.label_bytes = J.label_bytes;
+    L.slow_min = J.slow_min;
     L.nclass = J.nclass;
     L.key0 = J.key0;
     L.key1 = J.key1;
@@ -184,9 +195,7 @@ struct PiRef<double> {
 template <typename R>
 __device__ __forceinline__ int32_t class_at(const JobLocal<R> &L, int32_t lin)
 {
-    uint32_t lab = (L.label_bytes == 1) ? (uint32_t)((const uint8_t *)L.labels)[lin]
-                                        : (uint32_t)((const uint16_t *)L.labels)[lin];
-    return L.lclass ? (int32_t)L.lclass[lab] : (int32_t)lab;
+    return (int32_t)L.cls[lin];
 }
 
 // Move the time spent in the current class into its FP64 total.  Every class
@@ -252,25 +261,15 @@ __device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, in
     }
 }
 
-// One step i of the walk: direction (main.cu:140-151) and face march
-// (main.cu:153-300).  `dblk` is the cached direction block (f32 layout:
-// one Philox block serves steps 2m and 2m+1).
-//
-// With a step shorter than one voxel each axis meets at most one face per
-// step, so the three face distances t_k (main.cu:199-219) are computed once,
-// visited in increasing (t, axis) order -- the order of the reference's strict
-// `tt < tmin` scan -- and the walker advances by t_e - t_{e-1}.  A face is
-// reached when f * step >= t_e - t_{e-1} (main.cu:231); the remaining
-// fraction finishes the step (main.cu:237-243).  The three event slots are
-// unrolled and predicated, so a wavefront never loops on its slowest lane.
-template <typename R, int NP>
-__device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t wid,
-                                            int64_t i, bool fresh_block, uint32_t &b0, uint32_t &b1, uint32_t &b2,
-                                            uint32_t &b3, Tan<R, NP> &T)
+// Direction of step i, main.cu:140-151.  f32 layout: one Philox block serves
+// steps 2m and 2m+1; (b0, b1) are this step's words and (b2, b3) are shifted
+// in for step i+1.
+template <typename R>
+__device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, int64_t i, bool fresh_block,
+                                          uint32_t &b0, uint32_t &b1, uint32_t &b2, uint32_t &b3)
 {
     R u1, u2;
     if constexpr (sizeof(R) == 4) {
-        // words (b0, b1) serve this step; (b2, b3) are shifted in for step i+1
         if (fresh_block) {
             const U4 r = rng_block(L.key0, L.key1, (uint32_t)(i >> 1), STREAM_DIR, wid);
             const bool odd = (i & 1) != 0;
@@ -292,7 +291,95 @@ __device__ __forceinline__ void walker_step(Walker<R> &w, const JobLocal<R> &L, 
     const R sth = (R)2 * sqrt(u1 * ((R)1 - u1));
     R sp, cp;
     sincos_turns(u2 * PiRef<R>::v, sp, cp);
-    V3<R> v = {sth * cp, sth * sp, cth};
+    return V3<R>{sth * cp, sth * sp, cth};
+}
+
+// Straight-move fast path.  The straight step ends at e = o + s*v and crosses
+// the faces of the axes where e leaves [0,1] (c_k = +-1).  When the cell word
+// says every voxel the path can visit (g + any subset of the crossed axes)
+// has the walker's class and lies inside the lattice, every face on the path
+// is a same-class face, which the reference crosses without a draw and
+// without turning (main.cu:274-289): the step is the straight move.  A single
+// crossed face with a null voxel behind it reflects without a draw
+// (main.cu:276-277, P = 0 next to label 0, :450-451): the step is the straight
+// move folded back at that face.  Returns false when the march is needed.
+template <typename R, int NP>
+__device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, uint32_t cell, const V3<R> &v,
+                                          Tan<R, NP> &T)
+{
+    const R sc = w.sc;
+    const R ex = w.o.x + sc * v.x;
+    const R ey = w.o.y + sc * v.y;
+    const R ez = w.o.z + sc * v.z;
+    const int32_t cx = ex > (R)1 ? 1 : (ex < (R)0 ? -1 : 0);
+    const int32_t cy = ey > (R)1 ? 1 : (ey < (R)0 ? -1 : 0);
+    const int32_t cz = ez > (R)1 ? 1 : (ez < (R)0 ? -1 : 0);
+    const int32_t j = 13 + 9 * cx + 3 * cy + cz;
+    const bool pass = (j == 13) || (((cell >> (j - (j > 13 ? 1 : 0))) & 1u) != 0u);
+    if (pass) {
+        if constexpr (NP > 0) {
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                const R dsp = p == 0 ? L.ds1 : (p == 1 ? L.ds2 : L.ds3);
+                const R dsc = (w.c == p + 1) ? dsp : (R)0;
+                T.Jo[p].x = T.Jo[p].x + dsc * v.x;
+                T.Jo[p].y = T.Jo[p].y + dsc * v.y;
+                T.Jo[p].z = T.Jo[p].z + dsc * v.z;
+            }
+        }
+        w.g.x += cx;
+        w.g.y += cy;
+        w.g.z += cz;
+        w.o.x = ex - (R)cx;
+        w.o.y = ey - (R)cy;
+        w.o.z = ez - (R)cz;
+        w.lin += cx * L.sx + cy * L.nz + cz;
+        w.ncross += (uint32_t)(cx != 0) + (uint32_t)(cy != 0) + (uint32_t)(cz != 0);
+        w.tcur = w.tcur + (R)1;
+        return true;
+    }
+    // one crossed face with a null voxel behind it
+    const int32_t nk = (cx != 0) + (cy != 0) + (cz != 0);
+    const int32_t h = cx != 0 ? 0 : (cy != 0 ? 1 : 2);
+    const int32_t ch = cx + cy + cz;  // the one nonzero component when nk == 1
+    const int32_t f = 2 * h + (ch > 0 ? 1 : 0);
+    if (nk != 1 || ((cell >> (CELL_NULL_SHIFT + f)) & 1u) == 0u) return false;
+    if constexpr (NP > 0) {
+#pragma unroll
+        for (int p = 0; p < NP; p++) {
+            const R dsp = p == 0 ? L.ds1 : (p == 1 ? L.ds2 : L.ds3);
+            const R dsc = (w.c == p + 1) ? dsp : (R)0;
+            T.Jo[p].x = T.Jo[p].x + dsc * v.x;
+            T.Jo[p].y = T.Jo[p].y + dsc * v.y;
+            T.Jo[p].z = T.Jo[p].z + dsc * v.z;
+            T.Jo[p].x = h == 0 ? -T.Jo[p].x : T.Jo[p].x;
+            T.Jo[p].y = h == 1 ? -T.Jo[p].y : T.Jo[p].y;
+            T.Jo[p].z = h == 2 ? -T.Jo[p].z : T.Jo[p].z;
+        }
+    }
+    const R eh = h == 0 ? ex : (h == 1 ? ey : ez);
+    const R fold = ch > 0 ? (R)2 - eh : -eh;
+    w.o.x = h == 0 ? fold : ex;
+    w.o.y = h == 1 ? fold : ey;
+    w.o.z = h == 2 ? fold : ez;
+    w.nrefl += 1u;
+    w.tcur = w.tcur + (R)1;
+    return true;
+}
+
+// Face march of step i with direction v (main.cu:153-300).
+//
+// With a step shorter than one voxel each axis meets at most one face per
+// step, so the three face distances t_k (main.cu:199-219) are computed once,
+// visited in increasing (t, axis) order -- the order of the reference's strict
+// `tt < tmin` scan -- and the walker advances by t_e - t_{e-1}.  A face is
+// reached when f * step >= t_e - t_{e-1} (main.cu:231); the remaining
+// fraction finishes the step (main.cu:237-243).  The three event slots are
+// unrolled and predicated, so a wavefront never loops on its slowest lane.
+template <typename R, int NP>
+__device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t wid,
+                                      int64_t i, V3<R> v, Tan<R, NP> &T)
+{
     V3<R> iv = {(R)1 / v.x, (R)1 / v.y, (R)1 / v.z};
     // face distances in units of path, main.cu:199-219
     V3<R> t;
@@ -695,40 +782,86 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
         uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
         double *part = J.part + lb * (int64_t)a.timepoints * nval;
         int64_t r = rnext / a.Tstep;  // record index of rnext
+        uint32_t cell = L.cells[w.lin];
+        int64_t il = a.step0;         // this lane's next step
+        bool pend = false;            // direction drawn, face march outstanding
+        V3<R> pv = {(R)0, (R)0, (R)0};
+        int64_t done = a.step0;       // every lane has executed the steps < done
 
-        for (int64_t i = a.step0; i < a.step1; i++) {
-            const bool fresh = ((i & 1) == 0) || (i == a.step0);
-            walker_step<R, NP>(w, L, J, wid, i, fresh, b0, b1, b2, b3, T);
-            if (SIG & 2) {
-                const V3<double> d = displacement<R>(w, L.res);
-                ps.x = ps.x + d.x;
-                ps.y = ps.y + d.y;
-                ps.z = ps.z + d.z;
+        while (done < a.step1) {
+            // window: up to the next record step, the step before the next PGSE
+            // boundary, or the end of this launch (all wave-uniform)
+            int64_t tgt = a.step1 - 1;
+            if (rnext < rend && rnext < tgt) tgt = rnext;
+            if ((SIG & 2) && bnext > 0 && bnext - 1 < tgt) tgt = bnext - 1;
+            for (;;) {
+                const bool can = (il <= tgt) && !pend;
+                const uint64_t cmask = __ballot(can);
+                if (cmask == 0 && __ballot(pend) == 0) break;
+                bool adv = false;
+                if (cmask != 0) {
+                    if (can) {
+                        const bool fresh = ((il & 1) == 0) || (il == a.step0);
+                        const V3<R> v = draw_dir<R>(L, wid, il, fresh, b0, b1, b2, b3);
+                        if (fast_step<R, NP>(w, L, cell, v, T)) {
+                            adv = true;
+                        } else {
+                            pend = true;
+                            pv = v;
+                        }
+                    }
+                }
+                // march the waiting lanes once enough of them wait, or when no
+                // other lane can move
+                const uint64_t pmask = __ballot(pend);
+                if (pmask != 0) {
+                    const uint64_t movable = __ballot(!pend && (il + (adv ? 1 : 0)) <= tgt);
+                    if (__popcll(pmask) >= L.slow_min || movable == 0) {
+                        if (pend) {
+                            march<R, NP>(w, L, J, wid, il, pv, T);
+                            pend = false;
+                            adv = true;
+                        }
+                    }
+                }
+                if (adv) {
+                    il++;
+                    cell = L.cells[w.lin];
+                    if (SIG & 2) {
+                        const V3<double> d = displacement<R>(w, L.res);
+                        ps.x = ps.x + d.x;
+                        ps.y = ps.y + d.y;
+                        ps.z = ps.z + d.z;
+                        if constexpr (NP > 0) {
+#pragma unroll
+                            for (int p = 0; p < NP; p++) {
+                                T.PJ[p].x = T.PJ[p].x + (double)T.Jo[p].x * L.res;
+                                T.PJ[p].y = T.PJ[p].y + (double)T.Jo[p].y * L.res;
+                                T.PJ[p].z = T.PJ[p].z + (double)T.Jo[p].z * L.res;
+                            }
+                        }
+                    }
+                }
+            }
+            // every lane has executed step tgt
+            const int64_t i = tgt;
+            done = tgt + 1;
+            if ((SIG & 2) && i + 1 == bnext) {
+                double *sp = J.snap + (int64_t)(kb * 3) * ws + slot;
+                sp[0] = ps.x;
+                sp[ws] = ps.y;
+                sp[2 * ws] = ps.z;
                 if constexpr (NP > 0) {
 #pragma unroll
                     for (int p = 0; p < NP; p++) {
-                        T.PJ[p].x = T.PJ[p].x + (double)T.Jo[p].x * L.res;
-                        T.PJ[p].y = T.PJ[p].y + (double)T.Jo[p].y * L.res;
-                        T.PJ[p].z = T.PJ[p].z + (double)T.Jo[p].z * L.res;
+                        double *tp = J.tsnap + (int64_t)((kb * NP + p) * 3) * ws + slot;
+                        tp[0] = T.PJ[p].x;
+                        tp[ws] = T.PJ[p].y;
+                        tp[2 * ws] = T.PJ[p].z;
                     }
                 }
-                if (i + 1 == bnext) {
-                    double *sp = J.snap + (int64_t)(kb * 3) * ws + slot;
-                    sp[0] = ps.x;
-                    sp[ws] = ps.y;
-                    sp[2 * ws] = ps.z;
-                    if constexpr (NP > 0) {
-#pragma unroll
-                        for (int p = 0; p < NP; p++) {
-                            double *tp = J.tsnap + (int64_t)((kb * NP + p) * 3) * ws + slot;
-                            tp[0] = T.PJ[p].x;
-                            tp[ws] = T.PJ[p].y;
-                            tp[2 * ws] = T.PJ[p].z;
-                        }
-                    }
-                    kb++;
-                    bnext = kb <= nbound ? J.bounds[kb] : -1;
-                }
+                kb++;
+                bnext = kb <= nbound ? J.bounds[kb] : -1;
             }
             if (i == rnext && i < rend) {
                 // records, main.cu:333-378
@@ -953,6 +1086,54 @@ __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)
     for (; i < n; i += s) p[i] = 0ull;
 }
 
+// Cell words and classes of a label lattice: one thread per voxel reads its
+// 27-voxel neighbourhood.  Voxels outside the lattice never match (the
+// lattice edges are mirrors, handled by the march).
+__global__ void build_cells_kernel(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
+                                   int32_t ny, int32_t nz, uint32_t *cells, uint8_t *cls_out)
+{
+    const int64_t n = (int64_t)nx * ny * nz;
+    for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < n;
+         lin += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t z = (int32_t)(lin % nz);
+        const int64_t t = lin / nz;
+        const int32_t y = (int32_t)(t % ny);
+        const int32_t x = (int32_t)(t / ny);
+        auto cls = [&](int32_t xx, int32_t yy, int32_t zz) -> int32_t {
+            if (xx < 0 || xx >= nx || yy < 0 || yy >= ny || zz < 0 || zz >= nz) return -1;
+            const int64_t k = ((int64_t)xx * ny + yy) * nz + zz;
+            const uint32_t lab = label_bytes == 1 ? (uint32_t)((const uint8_t *)labels)[k]
+                                                  : (uint32_t)((const uint16_t *)labels)[k];
+            return lclass ? (int32_t)lclass[lab] : (int32_t)lab;
+        };
+        const int32_t c0 = cls(x, y, z);
+        uint32_t same = 0;
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dz = -1; dz <= 1; dz++)
+                    if (cls(x + dx, y + dy, z + dz) == c0) same |= 1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1));
+        uint32_t bits = 0;
+        for (int cx = -1; cx <= 1; cx++)
+            for (int cy = -1; cy <= 1; cy++)
+                for (int cz = -1; cz <= 1; cz++) {
+                    const int j = (cx + 1) * 9 + (cy + 1) * 3 + (cz + 1);
+                    if (j == 13) continue;
+                    bool ok = true;
+                    for (int a = 0; a < 8; a++) {
+                        const int qx = (a & 1) ? cx : 0, qy = (a & 2) ? cy : 0, qz = (a & 4) ? cz : 0;
+                        ok = ok && ((same >> ((qx + 1) * 9 + (qy + 1) * 3 + (qz + 1))) & 1u);
+                    }
+                    if (ok) bits |= 1u << (j - (j > 13 ? 1 : 0));
+                }
+        const int32_t face[6] = {cls(x - 1, y, z), cls(x + 1, y, z), cls(x, y - 1, z),
+                                 cls(x, y + 1, z), cls(x, y, z - 1), cls(x, y, z + 1)};
+        for (int f = 0; f < 6; f++)
+            if (face[f] == 0) bits |= 1u << (CELL_NULL_SHIFT + f);
+        cells[lin] = bits;
+        cls_out[lin] = (uint8_t)c0;
+    }
+}
+
 // ------------------------------------------------------------------ launch
 template <int NP>
 static const void *walk_fn(int prec_f64, int sig)
@@ -1019,6 +1200,18 @@ hipError_t launch_reduce(const ReduceDesc *rd_dev, int32_t njobs, int64_t max_nq
     if (e != hipSuccess) return e;
     dim3 g2((unsigned)((max_nq + bs - 1) / bs), (unsigned)njobs);
     hipLaunchKernelGGL(reduce_total_kernel, g2, dim3(bs), 0, s, rd_dev, njobs, R);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_cells(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
+                              int32_t ny, int32_t nz, uint32_t *cells, uint8_t *cls, hipStream_t s)
+{
+    const int64_t n = (int64_t)nx * ny * nz;
+    int64_t grid = (n + 255) / 256;
+    if (grid > 65536) grid = 65536;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(build_cells_kernel, dim3((unsigned)grid), dim3(256), 0, s, labels, label_bytes, lclass, nx, ny,
+                       nz, cells, cls);
     return hipGetLastError();
 }
 

@@ -14,12 +14,21 @@ constexpr int WAVE = 64;       // walkers per batch
 constexpr int MAX_BOUNDS = 64; // distinct PGSE lobe boundaries
 constexpr uint32_t INIT_TRIALS = 1u << 26;
 
+// Cell word of a voxel (one uint32 per voxel, index (x*ny + y)*nz + z).
+// For a crossing pattern (cx, cy, cz) in {-1,0,1}^3 other than (0,0,0), with
+// j = 13 + 9*cx + 3*cy + cz, bit j - (j > 13) is set when every voxel
+// g + sum over any subset of {cx e_x, cy e_y, cz e_z} lies inside the lattice
+// and has the class of g -- the voxels a straight step crossing those faces
+// can visit.  Bits 26 + f, f = 2*axis + (side > 0), are set when the face
+// neighbour on that side is a null voxel (class 0) inside the lattice.
+constexpr int CELL_NULL_SHIFT = 26;
+
 struct JobDesc {
     // geometry
-    const void *labels;        // uint8 or uint16, index (x*ny + y)*nz + z
-    const uint8_t *lclass;     // label -> class map or nullptr (identity)
+    const uint32_t *cells;     // cell words, see above
+    const uint8_t *cls;        // class of each voxel
     int32_t nx, ny, nz;
-    int32_t label_bytes;
+    int32_t slow_min;          // lanes that must wait before a face-march pass runs
     int32_t nclass;
     int32_t nbvec;
     int32_t ngrad;

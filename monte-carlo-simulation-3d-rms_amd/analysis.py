@@ -1,5 +1,8 @@
 """Cumulant analysis of the record sums (the semantics of analysis/readRMS.m).
 
+dki_shell / akc_shell / dki_lls restate readRMS.m:51-112 (signal-based
+tensor fits); akc_lls needs the DKI class the reference does not ship.
+
 readRMS.m:13-18 define the multiplicities of the 6 second-order and 15
 fourth-order moment components; readRMS.m:115-117 normalise by sig0;
 akc_mom (readRMS.m:168-175) gives D(t) = x2/(2t) and K(t) = x4/x2^2 - 3 along
@@ -78,6 +81,70 @@ def akc_cotangent(rec, nclass, TD, n, wD=None, wK=None):
     cot[:, i2:i2 + 6] += (-2.0 * wK * a4 * sig0 / a2 ** 3)[:, None] * c2
     cot[:, 0] += wK * a4 / a2 ** 2
     return cot
+
+
+def dki_shell(sig, bvec, bval, TD):
+    """Per-direction cumulant fit of the signal, then the moment tensors
+    (readRMS.m:51-78): for each unique direction, least squares of log S on
+    [-b, b^2/6, b^3/6, ...] (one column per distinct b-value) gives D and
+    K = X2/D^2; <x^2> = 2 D t and <x^4> = (K+3) <x^2>^2 along the direction;
+    the 6 + 15 tensor entries are the least-squares solutions over the
+    directions.  sig [timepoints, nb], bvec [nb, 3], bval [nb]; returns
+    [timepoints, 21] (dx2 entries first, then dx4), the layout akc_shell reads."""
+    sig = np.atleast_2d(np.asarray(sig, float))
+    bvec = np.asarray(bvec, float).reshape(-1, 3)
+    bval = np.asarray(bval, float).ravel()
+    TD = np.asarray(TD, float).ravel()
+    bvecu, inv = np.unique(bvec, axis=0, return_inverse=True)
+    inv = np.asarray(inv).ravel()
+    n2 = ndir2(bvecu)
+    n4 = ndir4(bvecu)
+    nbval = np.unique(bval).size
+    eps = np.finfo(float).eps
+    out = np.zeros((sig.shape[0], 21))
+    for i in range(sig.shape[0]):
+        Di = np.zeros(bvecu.shape[0])
+        Ki = np.zeros(bvecu.shape[0])
+        for j in range(bvecu.shape[0]):
+            sel = inv == j
+            bj = bval[sel]
+            A = np.column_stack([-bj] + [bj ** p / 6.0 for p in range(2, nbval + 1)])
+            X = np.linalg.lstsq(A, np.log(sig[i, sel] + eps), rcond=None)[0]
+            Di[j] = X[0]
+            Ki[j] = X[1] / X[0] ** 2
+        dx2g = Di * 2.0 * TD[i]
+        dx4g = (Ki + 3.0) * dx2g ** 2
+        out[i, :6] = np.linalg.lstsq(n2, dx2g, rcond=None)[0]
+        out[i, 6:] = np.linalg.lstsq(n4, dx4g, rcond=None)[0]
+    return out
+
+
+def akc_shell(dt, TD, n):
+    """[K, D] along directions n from dki_shell's tensors (readRMS.m:105-112)."""
+    dt = np.atleast_2d(np.asarray(dt, float))
+    x2 = dt[:, :6] @ ndir2(n).T
+    x4 = dt[:, 6:21] @ ndir4(n).T
+    D = x2 / 2.0 / np.asarray(TD, float)[:, None]
+    K = x4 / x2 ** 2 - 3.0
+    return K, D
+
+
+def dki_lls(sig, bvec, bval):
+    """Linear least-squares DKI fit of log S (readRMS.m:80-94): the b-matrix
+    of the 6 diffusion entries (multiplicities D_CNT) and the 15 kurtosis
+    entries (b^2/6, multiplicities W_CNT) on unit-normalised directions.
+    sig [timepoints, nb]; returns [21, timepoints] like the reference."""
+    sig = np.atleast_2d(np.asarray(sig, float))
+    g = np.asarray(bvec, float).reshape(-1, 3).copy()
+    b = np.asarray(bval, float).ravel()
+    nrm = np.linalg.norm(g, axis=1)
+    nrm[nrm == 0] = 1.0
+    g /= nrm[:, None]
+    B2 = -(b[:, None] * np.prod(g[:, D_IND], axis=2)) * D_CNT
+    B4 = (b[:, None] ** 2 / 6.0 * np.prod(g[:, W_IND], axis=2)) * W_CNT
+    Bm = np.hstack([B2, B4])
+    eps = np.finfo(float).eps
+    return np.linalg.lstsq(Bm, np.log(sig + eps).T, rcond=None)[0]
 
 
 def narrow_pulse_signal(sig0, sigRe):

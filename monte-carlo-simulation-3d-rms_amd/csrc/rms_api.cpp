@@ -24,7 +24,7 @@ using namespace rms;
 
 struct GeomDev {
     rms_geometry meta;     // labels / label_class pointers cleared
-    uint32_t *d_cells = nullptr;  // cell words (straight-move and null-face bits)
+    uint64_t *d_cells = nullptr;  // cell words (straight-move outcomes per crossing pattern)
     uint8_t *d_cls = nullptr;     // class per voxel
     int64_t nvox = 0;
     bool has_walkable = false;
@@ -302,11 +302,11 @@ extern "C" rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, i
     gd.nvox = nvox;
     gd.has_walkable = walk;
     // the labels and class map travel once; the kernels read only what is
-    // built from them: a class per voxel (uint8) and a cell word (uint32)
+    // built from them: a class per voxel (uint8) and a cell word (uint64)
     const size_t bytes = (size_t)nvox * g.label_bytes;
     void *d_lab = nullptr;
     uint8_t *d_map = nullptr;
-    hipError_t e = hipMalloc(&gd.d_cells, sizeof(uint32_t) * (size_t)nvox);
+    hipError_t e = hipMalloc(&gd.d_cells, sizeof(uint64_t) * (size_t)nvox);
     if (e == hipSuccess) e = hipMalloc(&gd.d_cls, (size_t)nvox);
     if (e == hipSuccess) e = hipMalloc(&d_lab, bytes);
     if (e == hipSuccess) e = hipMemcpy(d_lab, g.labels, bytes, hipMemcpyHostToDevice);

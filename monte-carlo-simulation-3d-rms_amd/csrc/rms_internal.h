@@ -22,7 +22,7 @@ hipError_t launch_reduce(const ReduceDesc *rd_dev, int32_t nred, int64_t max_nq,
 hipError_t launch_fill_zero(void *p, int64_t bytes, hipStream_t s);
 // Cell words and classes of a label lattice (see rms_types.h).
 hipError_t launch_build_cells(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
-                              int32_t ny, int32_t nz, uint32_t *cells, uint8_t *cls, hipStream_t s);
+                              int32_t ny, int32_t nz, uint64_t *cells, uint8_t *cls, hipStream_t s);
 
 // Tangent (adjoint) pass: same walk with forward-mode tangents, contracted
 // with the record/signal cotangents per walker.

@@ -96,7 +96,7 @@ struct Tan {
 // Per-job constants held in registers for the whole batch.
 template <typename R>
 struct JobLocal {
-    const uint32_t *cells;
+    const uint64_t *cells;
     const uint8_t *cls;
     int32_t nx, ny, nz, sx;  // sx = ny*nz
     int32_t slow_min, nclass;
@@ -295,16 +295,15 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
 }
 
 // Straight-move fast path.  The straight step ends at e = o + s*v and crosses
-// the faces of the axes where e leaves [0,1] (c_k = +-1).  When the cell word
-// says every voxel the path can visit (g + any subset of the crossed axes)
-// has the walker's class and lies inside the lattice, every face on the path
-// is a same-class face, which the reference crosses without a draw and
-// without turning (main.cu:274-289): the step is the straight move.  A single
-// crossed face with a null voxel behind it reflects without a draw
-// (main.cu:276-277, P = 0 next to label 0, :450-451): the step is the straight
-// move folded back at that face.  Returns false when the march is needed.
+// the faces of the axes where e leaves [0,1] (c_k = +-1).  The cell word says,
+// for this pattern of crossed faces, whether the outcome is independent of
+// the order the faces are met in and which of them reflect: same-class faces
+// are crossed without a draw or a turn, faces next to a null voxel reflect
+// without a draw (main.cu:274-289; P = 0 next to label 0, :450-451).  The step
+// is then the straight move folded back at the reflecting faces.  Returns
+// false when the march is needed.
 template <typename R, int NP>
-__device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, uint32_t cell, const V3<R> &v,
+__device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, uint64_t cell, const V3<R> &v,
                                           Tan<R, NP> &T)
 {
     const R sc = w.sc;
@@ -314,36 +313,33 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, ui
     const int32_t cx = ex > (R)1 ? 1 : (ex < (R)0 ? -1 : 0);
     const int32_t cy = ey > (R)1 ? 1 : (ey < (R)0 ? -1 : 0);
     const int32_t cz = ez > (R)1 ? 1 : (ez < (R)0 ? -1 : 0);
-    const int32_t j = 13 + 9 * cx + 3 * cy + cz;
-    const bool pass = (j == 13) || (((cell >> (j - (j > 13 ? 1 : 0))) & 1u) != 0u);
-    if (pass) {
-        if constexpr (NP > 0) {
-#pragma unroll
-            for (int p = 0; p < NP; p++) {
-                const R dsp = p == 0 ? L.ds1 : (p == 1 ? L.ds2 : L.ds3);
-                const R dsc = (w.c == p + 1) ? dsp : (R)0;
-                T.Jo[p].x = T.Jo[p].x + dsc * v.x;
-                T.Jo[p].y = T.Jo[p].y + dsc * v.y;
-                T.Jo[p].z = T.Jo[p].z + dsc * v.z;
-            }
-        }
-        w.g.x += cx;
-        w.g.y += cy;
-        w.g.z += cz;
-        w.o.x = ex - (R)cx;
-        w.o.y = ey - (R)cy;
-        w.o.z = ez - (R)cz;
-        w.lin += cx * L.sx + cy * L.nz + cz;
-        w.ncross += (uint32_t)(cx != 0) + (uint32_t)(cy != 0) + (uint32_t)(cz != 0);
-        w.tcur = w.tcur + (R)1;
-        return true;
+    const int32_t ax = cx != 0, ay = cy != 0, az = cz != 0;
+    const int32_t nk = ax + ay + az;
+    const int32_t px = cx > 0, py = cy > 0, pz = cz > 0;
+    uint32_t rmask = 0;  // reflected axes (bit k)
+    bool ok = true;
+    if (nk == 1) {
+        const int32_t h = ax ? 0 : (ay ? 1 : 2);
+        const int32_t f = 2 * h + (px | py | pz);
+        const uint32_t code = (uint32_t)(cell >> (2 * f)) & 3u;
+        ok = code != 0u;
+        rmask = code == 2u ? (1u << h) : 0u;
+    } else if (nk == 2) {
+        const int32_t m = !ax ? 0 : (!ay ? 1 : 2);  // the axis not crossed
+        const int32_t a = m == 0 ? 1 : 0;           // first crossed axis
+        const int32_t b = m == 2 ? 1 : 2;           // second crossed axis
+        const int32_t sa = a == 0 ? px : py;
+        const int32_t sb = b == 1 ? py : pz;
+        const int32_t idx = 4 * (2 - m) + 2 * sa + sb;
+        const uint32_t code = (uint32_t)(cell >> (CELL_PAIR_SHIFT + 3 * idx)) & 7u;
+        ok = code != 0u;
+        rmask = (code == 2u || code == 4u ? (1u << a) : 0u) | (code == 3u || code == 4u ? (1u << b) : 0u);
+    } else if (nk == 3) {
+        const int32_t idx = 4 * px + 2 * py + pz;
+        ok = ((cell >> (CELL_TRIPLE_SHIFT + idx)) & 1ull) != 0ull;
     }
-    // one crossed face with a null voxel behind it
-    const int32_t nk = (cx != 0) + (cy != 0) + (cz != 0);
-    const int32_t h = cx != 0 ? 0 : (cy != 0 ? 1 : 2);
-    const int32_t ch = cx + cy + cz;  // the one nonzero component when nk == 1
-    const int32_t f = 2 * h + (ch > 0 ? 1 : 0);
-    if (nk != 1 || ((cell >> (CELL_NULL_SHIFT + f)) & 1u) == 0u) return false;
+    if (!ok) return false;
+    const bool rx = (rmask & 1u) != 0u, ry = (rmask & 2u) != 0u, rz = (rmask & 4u) != 0u;
     if constexpr (NP > 0) {
 #pragma unroll
         for (int p = 0; p < NP; p++) {
@@ -352,17 +348,21 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, ui
             T.Jo[p].x = T.Jo[p].x + dsc * v.x;
             T.Jo[p].y = T.Jo[p].y + dsc * v.y;
             T.Jo[p].z = T.Jo[p].z + dsc * v.z;
-            T.Jo[p].x = h == 0 ? -T.Jo[p].x : T.Jo[p].x;
-            T.Jo[p].y = h == 1 ? -T.Jo[p].y : T.Jo[p].y;
-            T.Jo[p].z = h == 2 ? -T.Jo[p].z : T.Jo[p].z;
+            T.Jo[p].x = rx ? -T.Jo[p].x : T.Jo[p].x;
+            T.Jo[p].y = ry ? -T.Jo[p].y : T.Jo[p].y;
+            T.Jo[p].z = rz ? -T.Jo[p].z : T.Jo[p].z;
         }
     }
-    const R eh = h == 0 ? ex : (h == 1 ? ey : ez);
-    const R fold = ch > 0 ? (R)2 - eh : -eh;
-    w.o.x = h == 0 ? fold : ex;
-    w.o.y = h == 1 ? fold : ey;
-    w.o.z = h == 2 ? fold : ez;
-    w.nrefl += 1u;
+    const int32_t gx = rx ? 0 : cx, gy = ry ? 0 : cy, gz = rz ? 0 : cz;
+    w.o.x = rx ? (cx > 0 ? (R)2 - ex : -ex) : ex - (R)cx;
+    w.o.y = ry ? (cy > 0 ? (R)2 - ey : -ey) : ey - (R)cy;
+    w.o.z = rz ? (cz > 0 ? (R)2 - ez : -ez) : ez - (R)cz;
+    w.g.x += gx;
+    w.g.y += gy;
+    w.g.z += gz;
+    w.lin += gx * L.sx + gy * L.nz + gz;
+    w.ncross += (uint32_t)(gx != 0) + (uint32_t)(gy != 0) + (uint32_t)(gz != 0);
+    w.nrefl += (uint32_t)rx + (uint32_t)ry + (uint32_t)rz;
     w.tcur = w.tcur + (R)1;
     return true;
 }
@@ -782,7 +782,7 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
         uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
         double *part = J.part + lb * (int64_t)a.timepoints * nval;
         int64_t r = rnext / a.Tstep;  // record index of rnext
-        uint32_t cell = L.cells[w.lin];
+        uint64_t cell = L.cells[w.lin];
         int64_t il = a.step0;         // this lane's next step
         bool pend = false;            // direction drawn, face march outstanding
         V3<R> pv = {(R)0, (R)0, (R)0};
@@ -1087,10 +1087,10 @@ __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)
 }
 
 // Cell words and classes of a label lattice: one thread per voxel reads its
-// 27-voxel neighbourhood.  Voxels outside the lattice never match (the
-// lattice edges are mirrors, handled by the march).
+// 27-voxel neighbourhood and decides, for every pattern of crossed faces,
+// the order-independent straight-move outcome (see rms_types.h).
 __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
-                                   int32_t ny, int32_t nz, uint32_t *cells, uint8_t *cls_out)
+                                   int32_t ny, int32_t nz, uint64_t *cells, uint8_t *cls_out)
 {
     const int64_t n = (int64_t)nx * ny * nz;
     for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < n;
@@ -1099,37 +1099,79 @@ __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, cons
         const int64_t t = lin / nz;
         const int32_t y = (int32_t)(t % ny);
         const int32_t x = (int32_t)(t / ny);
-        auto cls = [&](int32_t xx, int32_t yy, int32_t zz) -> int32_t {
-            if (xx < 0 || xx >= nx || yy < 0 || yy >= ny || zz < 0 || zz >= nz) return -1;
-            const int64_t k = ((int64_t)xx * ny + yy) * nz + zz;
-            const uint32_t lab = label_bytes == 1 ? (uint32_t)((const uint8_t *)labels)[k]
-                                                  : (uint32_t)((const uint16_t *)labels)[k];
-            return lclass ? (int32_t)lclass[lab] : (int32_t)lab;
-        };
-        const int32_t c0 = cls(x, y, z);
-        uint32_t same = 0;
+        // classes of the 3x3x3 neighbourhood, -1 outside the lattice
+        int32_t nb[27];
         for (int dx = -1; dx <= 1; dx++)
             for (int dy = -1; dy <= 1; dy++)
-                for (int dz = -1; dz <= 1; dz++)
-                    if (cls(x + dx, y + dy, z + dz) == c0) same |= 1u << ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1));
-        uint32_t bits = 0;
-        for (int cx = -1; cx <= 1; cx++)
-            for (int cy = -1; cy <= 1; cy++)
-                for (int cz = -1; cz <= 1; cz++) {
-                    const int j = (cx + 1) * 9 + (cy + 1) * 3 + (cz + 1);
-                    if (j == 13) continue;
-                    bool ok = true;
-                    for (int a = 0; a < 8; a++) {
-                        const int qx = (a & 1) ? cx : 0, qy = (a & 2) ? cy : 0, qz = (a & 4) ? cz : 0;
-                        ok = ok && ((same >> ((qx + 1) * 9 + (qy + 1) * 3 + (qz + 1))) & 1u);
+                for (int dz = -1; dz <= 1; dz++) {
+                    const int32_t xx = x + dx, yy = y + dy, zz = z + dz;
+                    int32_t c = -1;
+                    if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+                        const int64_t k = ((int64_t)xx * ny + yy) * nz + zz;
+                        const uint32_t lab = label_bytes == 1 ? (uint32_t)((const uint8_t *)labels)[k]
+                                                              : (uint32_t)((const uint16_t *)labels)[k];
+                        c = lclass ? (int32_t)lclass[lab] : (int32_t)lab;
                     }
-                    if (ok) bits |= 1u << (j - (j > 13 ? 1 : 0));
+                    nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)] = c;
                 }
-        const int32_t face[6] = {cls(x - 1, y, z), cls(x + 1, y, z), cls(x, y - 1, z),
-                                 cls(x, y + 1, z), cls(x, y, z - 1), cls(x, y, z + 1)};
-        for (int f = 0; f < 6; f++)
-            if (face[f] == 0) bits |= 1u << (CELL_NULL_SHIFT + f);
-        cells[lin] = bits;
+        const int32_t c0 = nb[13];
+        auto at = [&](int qx, int qy, int qz) { return nb[(qx + 1) * 9 + (qy + 1) * 3 + (qz + 1)]; };
+        // outcome of crossing pattern c: reflect set R (bits), or -1 for the march
+        auto outcome = [&](const int c[3]) -> int {
+            const int nk = (c[0] != 0) + (c[1] != 0) + (c[2] != 0);
+            for (int R = 0; R < 8; R++) {
+                bool ok = !(nk == 3 && R != 0);
+                for (int k = 0; k < 3; k++)
+                    if (((R >> k) & 1) && c[k] == 0) ok = false;
+                for (int S = 0; S < 8 && ok; S++) {
+                    bool valid = true;
+                    for (int k = 0; k < 3; k++)
+                        if (((S >> k) & 1) && (c[k] == 0 || ((R >> k) & 1))) valid = false;
+                    if (!valid) continue;
+                    const int q[3] = {(S & 1) ? c[0] : 0, (S & 2) ? c[1] : 0, (S & 4) ? c[2] : 0};
+                    if (at(q[0], q[1], q[2]) != c0) ok = false;
+                    for (int a = 0; a < 3 && ok; a++) {
+                        if (!((R >> a) & 1)) continue;
+                        int qa[3] = {q[0], q[1], q[2]};
+                        qa[a] += c[a];
+                        if (at(qa[0], qa[1], qa[2]) != 0) ok = false;
+                    }
+                }
+                if (ok) return R;
+            }
+            return -1;
+        };
+        uint64_t word = 0;
+        for (int h = 0; h < 3; h++)
+            for (int sgn = 0; sgn < 2; sgn++) {
+                int c[3] = {0, 0, 0};
+                c[h] = sgn ? 1 : -1;
+                const int R = outcome(c);
+                const uint64_t code = R < 0 ? 0u : (R == 0 ? 1u : 2u);
+                word |= code << (2 * (2 * h + sgn));
+            }
+        for (int m = 0; m < 3; m++) {
+            const int a = m == 0 ? 1 : 0, b = m == 2 ? 1 : 2;
+            for (int sa = 0; sa < 2; sa++)
+                for (int sb = 0; sb < 2; sb++) {
+                    int c[3] = {0, 0, 0};
+                    c[a] = sa ? 1 : -1;
+                    c[b] = sb ? 1 : -1;
+                    const int R = outcome(c);
+                    uint64_t code = 0;
+                    if (R == 0) code = 1;
+                    else if (R == (1 << a)) code = 2;
+                    else if (R == (1 << b)) code = 3;
+                    else if (R == ((1 << a) | (1 << b))) code = 4;
+                    word |= code << (CELL_PAIR_SHIFT + 3 * (4 * (2 - m) + 2 * sa + sb));
+                }
+        }
+        for (int idx = 0; idx < 8; idx++) {
+            const int c[3] = {(idx & 4) ? 1 : -1, (idx & 2) ? 1 : -1, (idx & 1) ? 1 : -1};
+            if (outcome(c) == 0) word |= 1ull << (CELL_TRIPLE_SHIFT + idx);
+        }
+        (void)at;
+        cells[lin] = word;
         cls_out[lin] = (uint8_t)c0;
     }
 }
@@ -1204,7 +1246,7 @@ hipError_t launch_reduce(const ReduceDesc *rd_dev, int32_t njobs, int64_t max_nq
 }
 
 hipError_t launch_build_cells(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
-                              int32_t ny, int32_t nz, uint32_t *cells, uint8_t *cls, hipStream_t s)
+                              int32_t ny, int32_t nz, uint64_t *cells, uint8_t *cls, hipStream_t s)
 {
     const int64_t n = (int64_t)nx * ny * nz;
     int64_t grid = (n + 255) / 256;

@@ -14,18 +14,23 @@ constexpr int WAVE = 64;       // walkers per batch
 constexpr int MAX_BOUNDS = 64; // distinct PGSE lobe boundaries
 constexpr uint32_t INIT_TRIALS = 1u << 26;
 
-// Cell word of a voxel (one uint32 per voxel, index (x*ny + y)*nz + z).
-// For a crossing pattern (cx, cy, cz) in {-1,0,1}^3 other than (0,0,0), with
-// j = 13 + 9*cx + 3*cy + cz, bit j - (j > 13) is set when every voxel
-// g + sum over any subset of {cx e_x, cy e_y, cz e_z} lies inside the lattice
-// and has the class of g -- the voxels a straight step crossing those faces
-// can visit.  Bits 26 + f, f = 2*axis + (side > 0), are set when the face
-// neighbour on that side is a null voxel (class 0) inside the lattice.
-constexpr int CELL_NULL_SHIFT = 26;
+// Cell word of a voxel (one uint64 per voxel, index (x*ny + y)*nz + z): what
+// a straight step from this voxel does for each pattern of crossed faces
+// (cx, cy, cz) in {-1,0,1}^3, when that outcome does not depend on the order
+// the faces are met in.  A = crossed axes, split into X (crossed) and R
+// (reflected): every voxel g + any subset of the X crossings is inside the
+// lattice with the class of g, and from each of them the face across every R
+// axis has a null voxel inside the lattice behind it.
+//   |A| = 1, f = 2*axis + (c > 0):  bits 2f..2f+1      0 march, 1 cross, 2 reflect
+//   |A| = 2, axes a < b, missing axis m, idx = 4*(2-m) + 2*(ca > 0) + (cb > 0):
+//            bits 12+3*idx..+2   0 march, 1 cross both, 2 reflect a, 3 reflect b, 4 reflect both
+//   |A| = 3, idx = 4*(cx > 0) + 2*(cy > 0) + (cz > 0):  bit 48+idx  1 cross all
+constexpr int CELL_PAIR_SHIFT = 12;
+constexpr int CELL_TRIPLE_SHIFT = 48;
 
 struct JobDesc {
     // geometry
-    const uint32_t *cells;     // cell words, see above
+    const uint64_t *cells;     // cell words, see above
     const uint8_t *cls;        // class of each voxel
     int32_t nx, ny, nz;
     int32_t slow_min;          // lanes that must wait before a face-march pass runs

@@ -88,9 +88,6 @@ typedef struct {
     int32_t block_threads;       /* walkers per workgroup, 0 = auto (256) */
     int32_t grid_blocks;         /* workgroups, 0 = auto (resident limit) */
     int32_t xcd_split;           /* 1 = give each XCD a contiguous batch range */
-    int32_t slow_lanes;          /* walkers of a wavefront that wait for a face march
-                                    before it runs, 0 = auto (1 with FP32, 3 with FP64) */
-    int32_t reserved;
 } rms_run_params;
 
 /* Per-job outputs (caller-owned host buffers; NULL pointers are skipped).

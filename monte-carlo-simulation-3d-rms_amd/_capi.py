@@ -77,8 +77,6 @@ class RunParams(C.Structure):
         ("block_threads", C.c_int32),
         ("grid_blocks", C.c_int32),
         ("xcd_split", C.c_int32),
-        ("slow_lanes", C.c_int32),
-        ("reserved", C.c_int32),
     ]
 
 

@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <string>
 #include <vector>
@@ -111,8 +112,14 @@ int run_cli(int argc, char **argv, bool pgse)
     const size_t nvox = (size_t)sp.NPix1 * sp.NPix2 * sp.NPix3;
     std::vector<uint16_t> lab16;
     uint32_t maxlab = 0;
-    if (!rmsio::read_labels(path("fiber.txt"), nvox, lab16, maxlab, err)) return die(err);
-    if (maxlab >= (uint32_t)sp.Nc) return die("fiber.txt holds a label >= the number of compartments");
+    // fiber.bin (raw labels) takes precedence over fiber.txt when present
+    const bool have_bin = std::ifstream(path("fiber.bin")).good();
+    if (have_bin) {
+        if (!rmsio::read_labels_bin(path("fiber.bin"), nvox, lab16, maxlab, err)) return die(err);
+    } else if (!rmsio::read_labels(path("fiber.txt"), nvox, lab16, maxlab, err)) {
+        return die(err);
+    }
+    if (maxlab >= (uint32_t)sp.Nc) return die("the lattice holds a label >= the number of compartments");
     std::vector<uint8_t> lab8;
     rms_geometry g;
     std::memset(&g, 0, sizeof(g));

@@ -115,6 +115,41 @@ bool read_int(const std::string &path, int64_t &v, std::string &err)
     return true;
 }
 
+bool read_labels_bin(const std::string &path, size_t n, std::vector<uint16_t> &lab, uint32_t &maxlab,
+                     std::string &err)
+{
+    std::ifstream f(path, std::ios::in | std::ios::binary | std::ios::ate);
+    if (!f) {
+        err = "cannot open " + path;
+        return false;
+    }
+    const std::streamoff size = f.tellg();
+    f.seekg(0);
+    int bytes = 0;
+    if (size == (std::streamoff)n)
+        bytes = 1;
+    else if (size == (std::streamoff)(2 * n))
+        bytes = 2;
+    else {
+        err = path + ": size " + std::to_string((long long)size) + " is neither " + std::to_string(n) +
+              " (uint8) nor " + std::to_string(2 * n) + " (uint16) bytes";
+        return false;
+    }
+    std::vector<unsigned char> raw((size_t)size);
+    if (!f.read(reinterpret_cast<char *>(raw.data()), size)) {
+        err = "cannot read " + path;
+        return false;
+    }
+    lab.resize(n);
+    maxlab = 0;
+    for (size_t k = 0; k < n; k++) {
+        const uint32_t v = bytes == 1 ? raw[k] : (uint32_t)raw[2 * k] | ((uint32_t)raw[2 * k + 1] << 8);
+        lab[k] = (uint16_t)v;
+        if (v > maxlab) maxlab = v;
+    }
+    return true;
+}
+
 bool read_labels(const std::string &path, size_t n, std::vector<uint16_t> &lab, uint32_t &maxlab, std::string &err)
 {
     std::ifstream f(path, std::ios::in | std::ios::binary);

@@ -29,6 +29,11 @@ bool read_doubles(const std::string &path, size_t n, std::vector<double> &v, std
 // fiber.txt: NPix1*NPix2*NPix3 integer labels, x-major / z fastest
 // (setupRMS.m:17-23, main.cu:460-466).
 bool read_labels(const std::string &path, size_t n, std::vector<uint16_t> &lab, uint32_t &maxlab, std::string &err);
+// fiber.bin (no reference counterpart): the same labels as raw little-endian
+// uint8 (file size n) or uint16 (file size 2n), same order.  Text is
+// impractical above ~1e8 voxels.
+bool read_labels_bin(const std::string &path, size_t n, std::vector<uint16_t> &lab, uint32_t &maxlab,
+                     std::string &err);
 bool read_int(const std::string &path, int64_t &v, std::string &err);
 
 // Output writers (C++ stream formatting of the reference: precision 15 for

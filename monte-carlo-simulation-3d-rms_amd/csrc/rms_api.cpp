@@ -475,9 +475,6 @@ extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t
         }
         D.cells = G.d_cells;
         D.cls = G.d_cls;
-        // measured on the single-axon workload: lockstep (1) is best with FP32
-        // offsets, letting 3 lanes gather before a march is best with FP64
-        D.slow_min = P.slow_lanes > 0 ? P.slow_lanes : (P.precision == RMS_PREC_F64 ? 3 : 1);
         D.nx = G.meta.nx;
         D.ny = G.meta.ny;
         D.nz = G.meta.nz;

@@ -7,13 +7,10 @@
 // voxel (class + which straight moves stay inside same-class voxels) through
 // L2 / Infinity Cache.
 //
-// A step whose straight move only crosses same-class faces is a few adds (the
-// fast path); the others need the face march.  Lanes do not step in lockstep
-// inside a record window: every pass, each lane whose next step is fast takes
-// it, and lanes that need the march wait until enough of them (slow_min) are
-// waiting, then march together.  The window closes when every lane has done
-// the record step.  Each walker's trajectory is the same as in a lockstep
-// walk; only the schedule differs.
+// A step whose straight move meets only same-class faces and faces next to
+// null voxels, in an order-independent way, is a few adds read off a 64-bit
+// cell word (the fast path); the other lanes run the face march in the same
+// step.
 //
 // Per-record moments are summed across the wavefront with a fixed butterfly
 // and written as per-batch partials; a second kernel sums the partials in a
@@ -99,7 +96,7 @@ struct JobLocal {
     const uint64_t *cells;
     const uint8_t *cls;
     int32_t nx, ny, nz, sx;  // sx = ny*nz
-    int32_t slow_min, nclass;
+    int32_t nclass;
     uint32_t key0, key1, pcat;
     R st0, st1, st2, st3;    // step per class (voxels)
     R is0, is1, is2, is3;    // 1/step
@@ -128,7 +125,6 @@ __device__ __forceinline__ JobLocal<R> job_local(const JobDesc &J)
     L.ny = J.ny;
     L.nz = J.nz;
     L.sx = J.ny * J.nz;
-    L.slow_min = J.slow_min;
     L.nclass = J.nclass;
     L.key0 = J.key0;
     L.key1 = J.key1;
@@ -248,40 +244,43 @@ __device__ __forceinline__ void init_walker(Walker<R> &w, const JobLocal<R> &L, 
     w.oi = w.o;
 }
 
+// Exchange draw j (0..3) of step i: f32 layout takes word j of block (i, 1),
+// f64 layout half j&1 of block (i, 1 + j/2).  The block is recomputed for
+// every draw (draws happen only at probabilistic membranes), which keeps no
+// block in registers across the march.
 template <typename R>
-__device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, int64_t i, int j, U4 &blk)
+__device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, int64_t i, int j)
 {
     if constexpr (sizeof(R) == 4) {
-        if (j == 0) blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A, wid);
-        uint32_t word = j == 0 ? blk.x : (j == 1 ? blk.y : (j == 2 ? blk.z : blk.w));
+        const U4 blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A, wid);
+        const uint32_t word = j == 0 ? blk.x : (j == 1 ? blk.y : (j == 2 ? blk.z : blk.w));
         return u01_f32(word);
     } else {
-        if ((j & 1) == 0) blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A + (uint32_t)(j >> 1), wid);
+        const U4 blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A + (uint32_t)(j >> 1), wid);
         return (j & 1) ? u01_f64(blk.z, blk.w) : u01_f64(blk.x, blk.y);
     }
 }
 
 // Direction of step i, main.cu:140-151.  f32 layout: one Philox block serves
-// steps 2m and 2m+1; (b0, b1) are this step's words and (b2, b3) are shifted
-// in for step i+1.
+// steps 2m (words x, y) and 2m+1 (words z, w).
 template <typename R>
 __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, int64_t i, bool fresh_block,
-                                          uint32_t &b0, uint32_t &b1, uint32_t &b2, uint32_t &b3)
+                                          uint32_t &b0, uint32_t &b1)
 {
     R u1, u2;
     if constexpr (sizeof(R) == 4) {
+        // (b0, b1) carry words z, w of block i/2 from step 2m to step 2m+1
+        uint32_t w0 = b0, w1 = b1;
         if (fresh_block) {
             const U4 r = rng_block(L.key0, L.key1, (uint32_t)(i >> 1), STREAM_DIR, wid);
             const bool odd = (i & 1) != 0;
-            b0 = odd ? r.z : r.x;
-            b1 = odd ? r.w : r.y;
-            b2 = r.z;
-            b3 = r.w;
+            w0 = odd ? r.z : r.x;
+            w1 = odd ? r.w : r.y;
+            b0 = r.z;
+            b1 = r.w;
         }
-        u1 = u01_f32(b0);
-        u2 = u01_f32(b1);
-        b0 = b2;
-        b1 = b3;
+        u1 = u01_f32(w0);
+        u2 = u01_f32(w1);
     } else {
         U4 r = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_DIR, wid);
         u1 = u01_f64(r.x, r.y);
@@ -421,7 +420,6 @@ __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const 
     R f = (R)1;
     R trav = (R)0;
     int nd = 0;
-    U4 eblk = {0, 0, 0, 0};
     bool live = true;
     R df[NP > 0 ? NP : 1];
 #pragma unroll
@@ -488,7 +486,7 @@ __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const 
                 const int cat = (int)((L.pcat >> (2 * pidx)) & 3u);
                 bool cross = (cat == 1);
                 if (cat == 2) {
-                    const R u = exch_uniform<R>(L, wid, i, nd, eblk);
+                    const R u = exch_uniform<R>(L, wid, i, nd);
                     nd++;
                     cross = u < L.P[pidx];
                     if constexpr (NP > 0) {
@@ -779,89 +777,46 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
             while (kb <= nbound && J.bounds[kb] <= a.step0) kb++;
             bnext = kb <= nbound ? J.bounds[kb] : -1;
         }
-        uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+        uint32_t b0 = 0, b1 = 0;
         double *part = J.part + lb * (int64_t)a.timepoints * nval;
         int64_t r = rnext / a.Tstep;  // record index of rnext
         uint64_t cell = L.cells[w.lin];
-        int64_t il = a.step0;         // this lane's next step
-        bool pend = false;            // direction drawn, face march outstanding
-        V3<R> pv = {(R)0, (R)0, (R)0};
-        int64_t done = a.step0;       // every lane has executed the steps < done
 
-        while (done < a.step1) {
-            // window: up to the next record step, the step before the next PGSE
-            // boundary, or the end of this launch (all wave-uniform)
-            int64_t tgt = a.step1 - 1;
-            if (rnext < rend && rnext < tgt) tgt = rnext;
-            if ((SIG & 2) && bnext > 0 && bnext - 1 < tgt) tgt = bnext - 1;
-            for (;;) {
-                const bool can = (il <= tgt) && !pend;
-                const uint64_t cmask = __ballot(can);
-                if (cmask == 0 && __ballot(pend) == 0) break;
-                bool adv = false;
-                if (cmask != 0) {
-                    if (can) {
-                        const bool fresh = ((il & 1) == 0) || (il == a.step0);
-                        const V3<R> v = draw_dir<R>(L, wid, il, fresh, b0, b1, b2, b3);
-                        if (fast_step<R, NP>(w, L, cell, v, T)) {
-                            adv = true;
-                        } else {
-                            pend = true;
-                            pv = v;
-                        }
-                    }
-                }
-                // march the waiting lanes once enough of them wait, or when no
-                // other lane can move
-                const uint64_t pmask = __ballot(pend);
-                if (pmask != 0) {
-                    const uint64_t movable = __ballot(!pend && (il + (adv ? 1 : 0)) <= tgt);
-                    if (__popcll(pmask) >= L.slow_min || movable == 0) {
-                        if (pend) {
-                            march<R, NP>(w, L, J, wid, il, pv, T);
-                            pend = false;
-                            adv = true;
-                        }
-                    }
-                }
-                if (adv) {
-                    il++;
-                    cell = L.cells[w.lin];
-                    if (SIG & 2) {
-                        const V3<double> d = displacement<R>(w, L.res);
-                        ps.x = ps.x + d.x;
-                        ps.y = ps.y + d.y;
-                        ps.z = ps.z + d.z;
-                        if constexpr (NP > 0) {
-#pragma unroll
-                            for (int p = 0; p < NP; p++) {
-                                T.PJ[p].x = T.PJ[p].x + (double)T.Jo[p].x * L.res;
-                                T.PJ[p].y = T.PJ[p].y + (double)T.Jo[p].y * L.res;
-                                T.PJ[p].z = T.PJ[p].z + (double)T.Jo[p].z * L.res;
-                            }
-                        }
-                    }
-                }
-            }
-            // every lane has executed step tgt
-            const int64_t i = tgt;
-            done = tgt + 1;
-            if ((SIG & 2) && i + 1 == bnext) {
-                double *sp = J.snap + (int64_t)(kb * 3) * ws + slot;
-                sp[0] = ps.x;
-                sp[ws] = ps.y;
-                sp[2 * ws] = ps.z;
+        for (int64_t i = a.step0; i < a.step1; i++) {
+            const bool fresh = ((i & 1) == 0) || (i == a.step0);
+            const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1);
+            if (!fast_step<R, NP>(w, L, cell, v, T)) march<R, NP>(w, L, J, wid, i, v, T);
+            cell = L.cells[w.lin];
+            if (SIG & 2) {
+                const V3<double> d = displacement<R>(w, L.res);
+                ps.x = ps.x + d.x;
+                ps.y = ps.y + d.y;
+                ps.z = ps.z + d.z;
                 if constexpr (NP > 0) {
 #pragma unroll
                     for (int p = 0; p < NP; p++) {
-                        double *tp = J.tsnap + (int64_t)((kb * NP + p) * 3) * ws + slot;
-                        tp[0] = T.PJ[p].x;
-                        tp[ws] = T.PJ[p].y;
-                        tp[2 * ws] = T.PJ[p].z;
+                        T.PJ[p].x = T.PJ[p].x + (double)T.Jo[p].x * L.res;
+                        T.PJ[p].y = T.PJ[p].y + (double)T.Jo[p].y * L.res;
+                        T.PJ[p].z = T.PJ[p].z + (double)T.Jo[p].z * L.res;
                     }
                 }
-                kb++;
-                bnext = kb <= nbound ? J.bounds[kb] : -1;
+                if (i + 1 == bnext) {
+                    double *sp = J.snap + (int64_t)(kb * 3) * ws + slot;
+                    sp[0] = ps.x;
+                    sp[ws] = ps.y;
+                    sp[2 * ws] = ps.z;
+                    if constexpr (NP > 0) {
+#pragma unroll
+                        for (int p = 0; p < NP; p++) {
+                            double *tp = J.tsnap + (int64_t)((kb * NP + p) * 3) * ws + slot;
+                            tp[0] = T.PJ[p].x;
+                            tp[ws] = T.PJ[p].y;
+                            tp[2 * ws] = T.PJ[p].z;
+                        }
+                    }
+                    kb++;
+                    bnext = kb <= nbound ? J.bounds[kb] : -1;
+                }
             }
             if (i == rnext && i < rend) {
                 // records, main.cu:333-378

@@ -33,7 +33,7 @@ struct JobDesc {
     const uint64_t *cells;     // cell words, see above
     const uint8_t *cls;        // class of each voxel
     int32_t nx, ny, nz;
-    int32_t slow_min;          // lanes that must wait before a face-march pass runs
+    int32_t pad0;
     int32_t nclass;
     int32_t nbvec;
     int32_t ngrad;

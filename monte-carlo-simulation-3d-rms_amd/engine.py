@@ -157,9 +157,9 @@ class Engine:
         return gid.value
 
     def plan(self, jobs, dt, TN, timepoints=1000, precision="f32", replicas=32, chunk_steps=0, use_graph=False,
-             block_threads=0, grid_blocks=0, xcd_split=0, slow_lanes=0):
+             block_threads=0, grid_blocks=0, xcd_split=0):
         return Plan(self, jobs, dt, TN, timepoints, precision, replicas, chunk_steps, use_graph, block_threads,
-                    grid_blocks, xcd_split, slow_lanes)
+                    grid_blocks, xcd_split)
 
     def run(self, jobs, dt, TN, timepoints=1000, precision="f32", replicas=32, final_state=False, **kw):
         p = self.plan(jobs, dt, TN, timepoints, precision, replicas, **kw)
@@ -173,7 +173,7 @@ class Plan:
     """rms_plan_create / rms_plan_execute / rms_plan_adjoint."""
 
     def __init__(self, eng, jobs, dt, TN, timepoints, precision, replicas, chunk_steps, use_graph, block_threads,
-                 grid_blocks, xcd_split, slow_lanes=0):
+                 grid_blocks, xcd_split):
         self.eng = eng
         self.jobs = []
         n = len(jobs)
@@ -206,8 +206,7 @@ class Plan:
         self.params = capi.RunParams(
             dt=float(dt), TN=int(TN), timepoints=int(timepoints), precision=capi.PREC[precision],
             replicas=max(1, int(replicas)), chunk_steps=int(chunk_steps), use_graph=int(bool(use_graph)),
-            block_threads=int(block_threads), grid_blocks=int(grid_blocks), xcd_split=int(xcd_split),
-            slow_lanes=int(slow_lanes), reserved=0)
+            block_threads=int(block_threads), grid_blocks=int(grid_blocks), xcd_split=int(xcd_split))
         h = C.c_void_p()
         eng._check(capi.lib().rms_plan_create(eng._h, cj, n, C.byref(self.params), C.byref(h)), "rms_plan_create")
         self._h = h

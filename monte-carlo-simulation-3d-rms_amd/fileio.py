@@ -56,6 +56,56 @@ def write_fiber(path, fiber):
             f.write(" ".join(map(str, part.tolist())) + " ")
 
 
+def write_fiber_bin(path, fiber):
+    """fiber.bin: the lattice as raw little-endian uint8 (labels < 256) or
+    uint16, x-major / z fastest (the fiber.txt order).  The CLIs read it in
+    place of fiber.txt when present; text is impractical above ~1e8 voxels."""
+    fiber = np.asarray(fiber)
+    if fiber.ndim != 3:
+        raise ValueError("fiber must be 3-D")
+    if fiber.size and (fiber.min() < 0 or fiber.max() > 65535):
+        raise ValueError("labels must be in 0..65535")
+    dt = "<u1" if (fiber.size == 0 or fiber.max() < 256) else "<u2"
+    np.ascontiguousarray(fiber, dtype=dt).tofile(path)
+
+
+def load_fiber_mat(path):
+    """Read a fiber.mat of the reference (a MAT v5 struct array `fiber` with
+    fields `shape` (uint8 label lattice) and `voxel_size`, as loaded by
+    part1_demo3_simulation.m:31-32 and 57, 75).  Returns a list of
+    (labels, voxel_size).  MATLAB's (i, j, k) index maps to labels[i, j, k],
+    the order setupRMS.m:17-23 writes fiber.txt in."""
+    try:
+        from scipy.io import loadmat
+    except ImportError as e:  # pragma: no cover - scipy is part of the image
+        raise RuntimeError("reading .mat files needs scipy") from e
+    m = loadmat(path, squeeze_me=False, struct_as_record=True)
+    if "fiber" not in m:
+        raise ValueError(f"{path}: no variable 'fiber'")
+    f = m["fiber"].ravel()
+    out = []
+    for e in f:
+        shape = np.ascontiguousarray(np.asarray(e["shape"]))
+        if shape.ndim != 3:
+            raise ValueError("fiber(k).shape must be a 3-D array")
+        vs = float(np.asarray(e["voxel_size"]).ravel()[0])
+        out.append((shape, vs))
+    return out
+
+
+def crop_to_content(labels, margin=2):
+    """Crop x and y to the bounding box of the non-null voxels plus `margin`
+    (full z).  The lattice edges are mirrors, so empty null margins beyond
+    one voxel do not change the walk."""
+    labels = np.asarray(labels)
+    nz = np.nonzero(labels)
+    if nz[0].size == 0:
+        return labels
+    lo = [max(0, int(a.min()) - margin) for a in nz[:2]]
+    hi = [min(labels.shape[k], int(nz[k].max()) + margin + 1) for k in range(2)]
+    return np.ascontiguousarray(labels[lo[0]:hi[0], lo[1]:hi[1], :])
+
+
 def input_rms(target, fiber, X):
     """setupRMS.inputRMS: X has time_step, step_num, particle_num, comp_num,
     voxel_size, diffusivity, T2_relaxation, bval, bvec."""

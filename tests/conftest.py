@@ -34,3 +34,16 @@ def engine(rms):
     eng = rms.Engine(0)
     yield eng
     eng.close()
+
+
+@pytest.fixture(scope="session")
+def harness(tmp_path_factory):
+    """Host build of the CLI's file-contract code (rms_io.cpp) with a small driver."""
+    import subprocess
+    d = tmp_path_factory.mktemp("h")
+    exe = str(d / "io_harness")
+    here = os.path.dirname(os.path.abspath(__file__))
+    cli = os.path.join(ROOT, "monte-carlo-simulation-3d-rms_amd", "cli")
+    subprocess.run(["g++", "-O1", "-std=c++17", os.path.join(here, "cpp", "io_harness.cpp"),
+                    os.path.join(cli, "rms_io.cpp"), "-o", exe], check=True)
+    return exe

@@ -1,5 +1,6 @@
 // Test harness for the CLI's file-contract code (rms_io.cpp), no GPU needed.
-// usage: io_harness read <dir> <pgse 0|1>   -> prints parsed parameters/labels
+// usage: io_harness read <dir> <pgse 0|1> [bin] -> prints parsed parameters/labels
+//        (bin: the labels come from fiber.bin instead of fiber.txt)
 //        io_harness write <dir>             -> writes sample outputs
 #include <cstdio>
 #include <cstring>
@@ -26,7 +27,9 @@ int main(int argc, char **argv)
         std::vector<uint16_t> lab;
         uint32_t maxlab = 0;
         size_t n = (size_t)p.NPix1 * p.NPix2 * p.NPix3;
-        if (!rmsio::read_labels(dir + "/fiber.txt", n, lab, maxlab, err)) {
+        const bool bin = argc > 4 && std::string(argv[4]) == "bin";
+        if (!(bin ? rmsio::read_labels_bin(dir + "/fiber.bin", n, lab, maxlab, err)
+                  : rmsio::read_labels(dir + "/fiber.txt", n, lab, maxlab, err))) {
             std::cerr << err << std::endl;
             return 1;
         }

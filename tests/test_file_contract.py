@@ -12,15 +12,6 @@ ROOT = os.path.dirname(HERE)
 CLI = os.path.join(ROOT, "monte-carlo-simulation-3d-rms_amd", "cli")
 
 
-@pytest.fixture(scope="module")
-def harness(tmp_path_factory):
-    d = tmp_path_factory.mktemp("h")
-    exe = str(d / "io_harness")
-    subprocess.run(["g++", "-O1", "-std=c++17", os.path.join(HERE, "cpp", "io_harness.cpp"),
-                    os.path.join(CLI, "rms_io.cpp"), "-o", exe], check=True)
-    return exe
-
-
 def _X(nbvec=3):
     rng = np.random.default_rng(0)
     bvec = rng.normal(size=(nbvec, 3))

@@ -134,6 +134,9 @@ def lib():
         L.or_pgse_lobes.argtypes = [C.c_double, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]
         L.or_march_f64.argtypes = [C.POINTER(_Geom), C.POINTER(_Derived)] + [C.c_void_p] * 8
         L.or_march_f64.restype = C.c_int
+        L.or_fast_step_f64.argtypes = [C.POINTER(_Geom), C.POINTER(_Derived)] + [C.c_void_p] * 3 + \
+            [C.c_int32, C.c_void_p]
+        L.or_fast_step_f64.restype = C.c_int
         L.or_run.argtypes = [C.POINTER(_Job), C.c_int32, C.POINTER(_Params), C.POINTER(_Out)]
         L.or_run.restype = C.c_int
         L.or_run_adjoint.argtypes = [C.POINTER(_Job), C.c_int32, C.POINTER(_Params), C.POINTER(_Adj),
@@ -278,6 +281,20 @@ def march_f64(geom: Geometry, dt, g, o, v, cls, u_exch=(0.5, 0.5, 0.5, 0.5), gi=
     nd = lib().or_march_f64(C.byref(gc), C.byref(d), _ptr(g), _ptr(o), _ptr(v), _ptr(c), _ptr(u), _ptr(cnt),
                             _ptr(gi), _ptr(oi))
     return dict(g=g, o=o, v=v, cls=int(c[0]), cnt=cnt, draws=nd, gi=gi, oi=oi)
+
+
+def fast_step_f64(geom: Geometry, dt, g, o, v, cls):
+    """One straight-move fast-path step; None when the march is needed."""
+    gc = geom._c()
+    d = _Derived()
+    if not lib().or_derive(C.byref(gc), dt, C.byref(d)):
+        raise ValueError(lib().or_last_error().decode())
+    g = np.array(g, np.int32)
+    o = np.array(o, np.float64)
+    v = np.array(v, np.float64)
+    cnt = np.zeros(2, np.int32)
+    ok = lib().or_fast_step_f64(C.byref(gc), C.byref(d), _ptr(g), _ptr(o), _ptr(v), int(cls), _ptr(cnt))
+    return dict(g=g, o=o, cnt=cnt) if ok else None
 
 
 # ---------------------------------------------------------------- simulation

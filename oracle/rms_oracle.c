@@ -575,6 +575,30 @@ int or_march_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double 
     return nd;
 }
 
+/* One straight-move fast-path step from a given state (test entry): returns
+ * 1 and updates the state when the fast path applies, 0 otherwise. */
+int or_fast_step_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double o[3], const double v[3],
+                int32_t cls, int32_t cnt[2])
+{
+    or_walker_f64 w;
+    memset(&w, 0, sizeof(w));
+    for (int k = 0; k < 3; k++) {
+        w.g[k] = g[k];
+        w.o[k] = o[k];
+    }
+    w.c = cls;
+    or_dreal_f64 dr;
+    or_dreal_make_f64(d, &dr);
+    int r = or_fast_f64(&w, geom, &dr, v, NULL, NULL);
+    for (int k = 0; k < 3; k++) {
+        g[k] = w.g[k];
+        o[k] = w.o[k];
+    }
+    cnt[0] = (int32_t)w.cnt[0];
+    cnt[1] = (int32_t)w.cnt[1];
+    return r;
+}
+
 /* ------------------------------------------------------------------ */
 /* Driver                                                                */
 /* ------------------------------------------------------------------ */

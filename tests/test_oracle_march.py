@@ -160,3 +160,44 @@ def test_step_length_is_conserved(orc, D):
         assert abs(np.linalg.norm(r["v"]) - 1.0) < 1e-14
         assert lab[tuple(r["g"])] != 0
         assert np.linalg.norm(_pos(r) - x0) <= s + 1e-12
+
+
+def test_fast_path_equals_march(orc):
+    """The straight-move fast path against the face march on random states in
+    a lattice with null voxels, a second class and lattice edges: whenever the
+    fast path applies it ends in the same voxel and class with the same
+    crossing/reflection counts, at the same position up to rounding; and it
+    applies to most steps away from boundaries."""
+    rng = np.random.default_rng(11)
+    lab = rng.integers(0, 3, (7, 6, 8)).astype(np.uint8)
+    lab[2:5, 2:5, 2:6] = 1
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0, 2.0], T2=[np.inf] * 3, kappa=0.0)  # classes 1, 2 impermeable
+    dt = 2e-4
+    step = orc.derive(g, dt)["step"][1]
+    applied = 0
+    n = 4000
+    for _ in range(n):
+        while True:
+            gv = rng.integers(0, [7, 6, 8])
+            cls = int(lab[tuple(gv)])
+            if cls != 0:
+                break
+        o = rng.random(3)
+        u1, u2 = rng.random(2)
+        cth = 1 - 2 * u1
+        sth = 2 * np.sqrt(u1 * (1 - u1))
+        v = np.array([sth * np.cos(2 * np.pi * u2), sth * np.sin(2 * np.pi * u2), cth])
+        f = orc.fast_step_f64(g, dt, gv, o, v, cls)
+        m = orc.march_f64(g, dt, gv, o, v, cls)
+        if f is None:
+            continue
+        applied += 1
+        np.testing.assert_array_equal(f["g"], m["g"])
+        assert m["cls"] == cls
+        np.testing.assert_allclose(f["o"], m["o"], atol=1e-12)
+        assert f["cnt"][0] == m["cnt"][0] and f["cnt"][1] == m["cnt"][1]
+        assert m["cnt"][2] == 0  # no lattice-edge mirror on a fast step
+    assert applied > 0.3 * n
+    # pure interior: always applies and is the straight move
+    f = orc.fast_step_f64(g, dt, [3, 3, 3], [0.5, 0.5, 0.5], [0.6, 0.0, 0.8], 1)
+    np.testing.assert_allclose(f["o"], [0.5 + step * 0.6, 0.5, 0.5 + step * 0.8], atol=1e-15)

@@ -234,3 +234,29 @@ def test_autograd_matches_adjoint(rms, orc, engine):
     np.testing.assert_array_equal(D.grad.numpy(), ref["grad"][:3])
     np.testing.assert_array_equal(T2.grad.numpy(), ref["grad"][4:7])
     assert float(kap.grad) == ref["grad"][8]
+
+
+def test_autotuned_launch_gives_identical_results(rms, engine):
+    """The autotuner's launch shape and job order change the schedule only:
+    every job's records are bit-identical to the default launch."""
+    import importlib
+    at = importlib.import_module(rms.__name__ + ".autotune")
+    print("case autotune", flush=True)
+    labs = [random_blobs((9, 8, 10), 3, seed=s, dtype=np.uint16) for s in (1, 2, 3)] + [planes(6, 6, 20)]
+    jobs = []
+    for k, lab in enumerate(labs):
+        nc = 3 if lab.dtype == np.uint16 else 2
+        g = rms.Geometry(lab, 0.1, [0.0, 2.0, 1.5][:nc], T2=[np.inf, 80.0, 60.0][:nc],
+                         kappa=0.02 if nc == 3 else None)
+        jobs.append(dict(geom=g, nwalk=200 + 97 * k, seed=100 + k))
+    best = at.tune(engine, jobs, 2e-4, timepoints=10, precision="f32", probe_steps=50, reps=1, use_cache=False)
+    assert best["block_threads"] in (64, 128, 256) and best["grid_blocks"] >= 8
+    assert sorted(best["order"]) == list(range(len(jobs)))
+    base = engine.run(jobs, 2e-4, 300, 30, precision="f32", replicas=4)
+    ordered = [jobs[k] for k in best["order"]]
+    tuned = engine.run(ordered, 2e-4, 300, 30, precision="f32", replicas=4, block_threads=best["block_threads"],
+                       grid_blocks=best["grid_blocks"], xcd_split=best["xcd_split"])
+    for pos, k in enumerate(best["order"]):
+        np.testing.assert_array_equal(tuned[pos]["rec"], base[k]["rec"])
+        np.testing.assert_array_equal(tuned[pos]["rec_rep"], base[k]["rec_rep"])
+        np.testing.assert_array_equal(tuned[pos]["counters"], base[k]["counters"])

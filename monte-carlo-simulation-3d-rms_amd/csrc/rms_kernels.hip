@@ -295,11 +295,12 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
 
 // Straight-move fast path.  The straight step ends at e = o + s*v and crosses
 // the faces of the axes where e leaves [0,1] (c_k = +-1).  The cell word says,
-// for this pattern of crossed faces, whether the outcome is independent of
-// the order the faces are met in and which of them reflect: same-class faces
+// for this pattern of crossed faces, which of them reflect: same-class faces
 // are crossed without a draw or a turn, faces next to a null voxel reflect
-// without a draw (main.cu:274-289; P = 0 next to label 0, :450-451).  The step
-// is then the straight move folded back at the reflecting faces.  Returns
+// without a draw (main.cu:274-289; P = 0 next to label 0, :450-451).  For two
+// crossed faces whose outcome depends on the order they are met in, the order
+// comes from comparing the distances to the faces, d_a |v_b| <= d_b |v_a|.
+// The step is the straight move folded back at the reflecting faces.  Returns
 // false when the march is needed.
 template <typename R, int NP>
 __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, uint64_t cell, const V3<R> &v,
@@ -332,7 +333,17 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, ui
         const int32_t idx = 4 * (2 - m) + 2 * sa + sb;
         const uint32_t code = (uint32_t)(cell >> (CELL_PAIR_SHIFT + 3 * idx)) & 7u;
         ok = code != 0u;
-        rmask = (code == 2u || code == 4u ? (1u << a) : 0u) | (code == 3u || code == 4u ? (1u << b) : 0u);
+        if (code <= 4u) {
+            rmask = (code == 2u || code == 4u ? (1u << a) : 0u) | (code == 3u || code == 4u ? (1u << b) : 0u);
+        } else {
+            // the order matters: a is met first when d_a |v_b| <= d_b |v_a|
+            const R oa = a == 0 ? w.o.x : w.o.y, ob = b == 1 ? w.o.y : w.o.z;
+            const R va = a == 0 ? v.x : v.y, vb = b == 1 ? v.y : v.z;
+            const R da = sa ? (R)1 - oa : oa, db = sb ? (R)1 - ob : ob;
+            const bool afirst = da * fabs(vb) <= db * fabs(va);
+            const uint32_t A = 1u << a, B = 1u << b;
+            rmask = code == 5u ? (afirst ? B : A) : (code == 6u ? (afirst ? A : 0u) : (afirst ? 0u : B));
+        }
     } else if (nk == 3) {
         const int32_t idx = 4 * px + 2 * py + pz;
         ok = ((cell >> (CELL_TRIPLE_SHIFT + idx)) & 1ull) != 0ull;
@@ -785,8 +796,9 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
         for (int64_t i = a.step0; i < a.step1; i++) {
             const bool fresh = ((i & 1) == 0) || (i == a.step0);
             const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1);
+            const int32_t lin0 = w.lin;
             if (!fast_step<R, NP>(w, L, cell, v, T)) march<R, NP>(w, L, J, wid, i, v, T);
-            cell = L.cells[w.lin];
+            if (w.lin != lin0) cell = L.cells[w.lin];  // only lanes that changed voxel read
             if (SIG & 2) {
                 const V3<double> d = displacement<R>(w, L.res);
                 ps.x = ps.x + d.x;
@@ -1042,8 +1054,8 @@ __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)
 }
 
 // Cell words and classes of a label lattice: one thread per voxel reads its
-// 27-voxel neighbourhood and decides, for every pattern of crossed faces,
-// the order-independent straight-move outcome (see rms_types.h).
+// 27-voxel neighbourhood and decides, for every pattern of crossed faces and
+// order, the straight-move outcome (see rms_types.h).
 __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
                                    int32_t ny, int32_t nz, uint64_t *cells, uint8_t *cls_out)
 {
@@ -1071,37 +1083,32 @@ __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, cons
                 }
         const int32_t c0 = nb[13];
         auto at = [&](int qx, int qy, int qz) { return nb[(qx + 1) * 9 + (qy + 1) * 3 + (qz + 1)]; };
-        // outcome of crossing pattern c: reflect set R (bits), or -1 for the march
-        auto outcome = [&](const int c[3]) -> int {
-            const int nk = (c[0] != 0) + (c[1] != 0) + (c[2] != 0);
-            for (int R = 0; R < 8; R++) {
-                bool ok = !(nk == 3 && R != 0);
-                for (int k = 0; k < 3; k++)
-                    if (((R >> k) & 1) && c[k] == 0) ok = false;
-                for (int S = 0; S < 8 && ok; S++) {
-                    bool valid = true;
-                    for (int k = 0; k < 3; k++)
-                        if (((S >> k) & 1) && (c[k] == 0 || ((R >> k) & 1))) valid = false;
-                    if (!valid) continue;
-                    const int q[3] = {(S & 1) ? c[0] : 0, (S & 2) ? c[1] : 0, (S & 4) ? c[2] : 0};
-                    if (at(q[0], q[1], q[2]) != c0) ok = false;
-                    for (int a = 0; a < 3 && ok; a++) {
-                        if (!((R >> a) & 1)) continue;
-                        int qa[3] = {q[0], q[1], q[2]};
-                        qa[a] += c[a];
-                        if (at(qa[0], qa[1], qa[2]) != 0) ok = false;
-                    }
+        // faces met in the order ord[0..n-1] from this voxel: reflect set R
+        // (bits), or -1 when a face needs the march
+        auto order_outcome = [&](const int c[3], const int *ord, int n) -> int {
+            int p[3] = {0, 0, 0};
+            int R = 0;
+            for (int e = 0; e < n; e++) {
+                const int h = ord[e];
+                int q[3] = {p[0], p[1], p[2]};
+                q[h] += c[h];
+                const int32_t cq = at(q[0], q[1], q[2]);
+                if (cq == c0) {
+                    p[h] = q[h];
+                } else if (cq == 0) {
+                    R |= 1 << h;
+                } else {
+                    return -1;
                 }
-                if (ok) return R;
             }
-            return -1;
+            return R;
         };
         uint64_t word = 0;
         for (int h = 0; h < 3; h++)
             for (int sgn = 0; sgn < 2; sgn++) {
                 int c[3] = {0, 0, 0};
                 c[h] = sgn ? 1 : -1;
-                const int R = outcome(c);
+                const int R = order_outcome(c, &h, 1);
                 const uint64_t code = R < 0 ? 0u : (R == 0 ? 1u : 2u);
                 word |= code << (2 * (2 * h + sgn));
             }
@@ -1112,20 +1119,26 @@ __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, cons
                     int c[3] = {0, 0, 0};
                     c[a] = sa ? 1 : -1;
                     c[b] = sb ? 1 : -1;
-                    const int R = outcome(c);
+                    const int ab[2] = {a, b}, ba[2] = {b, a};
+                    const int oa = order_outcome(c, ab, 2), ob = order_outcome(c, ba, 2);
+                    const int A = 1 << a, B = 1 << b;
                     uint64_t code = 0;
-                    if (R == 0) code = 1;
-                    else if (R == (1 << a)) code = 2;
-                    else if (R == (1 << b)) code = 3;
-                    else if (R == ((1 << a) | (1 << b))) code = 4;
+                    if (oa >= 0 && ob >= 0) {
+                        if (oa == ob) code = oa == 0 ? 1 : (oa == A ? 2 : (oa == B ? 3 : 4));
+                        else if (oa == B && ob == A) code = 5;
+                        else if (oa == A && ob == 0) code = 6;
+                        else if (oa == 0 && ob == B) code = 7;
+                    }
                     word |= code << (CELL_PAIR_SHIFT + 3 * (4 * (2 - m) + 2 * sa + sb));
                 }
         }
         for (int idx = 0; idx < 8; idx++) {
             const int c[3] = {(idx & 4) ? 1 : -1, (idx & 2) ? 1 : -1, (idx & 1) ? 1 : -1};
-            if (outcome(c) == 0) word |= 1ull << (CELL_TRIPLE_SHIFT + idx);
+            bool all = true;
+            for (int S = 1; S < 8; S++)
+                all = all && at((S & 1) ? c[0] : 0, (S & 2) ? c[1] : 0, (S & 4) ? c[2] : 0) == c0;
+            if (all) word |= 1ull << (CELL_TRIPLE_SHIFT + idx);
         }
-        (void)at;
         cells[lin] = word;
         cls_out[lin] = (uint8_t)c0;
     }

@@ -16,14 +16,14 @@ constexpr uint32_t INIT_TRIALS = 1u << 26;
 
 // Cell word of a voxel (one uint64 per voxel, index (x*ny + y)*nz + z): what
 // a straight step from this voxel does for each pattern of crossed faces
-// (cx, cy, cz) in {-1,0,1}^3, when that outcome does not depend on the order
-// the faces are met in.  A = crossed axes, split into X (crossed) and R
-// (reflected): every voxel g + any subset of the X crossings is inside the
-// lattice with the class of g, and from each of them the face across every R
-// axis has a null voxel inside the lattice behind it.
+// (cx, cy, cz) in {-1,0,1}^3.  Met in a given order, a face whose neighbour
+// has the class of the path's voxel is crossed, one whose neighbour is a null
+// voxel inside the lattice reflects, anything else needs the face march.
 //   |A| = 1, f = 2*axis + (c > 0):  bits 2f..2f+1      0 march, 1 cross, 2 reflect
 //   |A| = 2, axes a < b, missing axis m, idx = 4*(2-m) + 2*(ca > 0) + (cb > 0):
-//            bits 12+3*idx..+2   0 march, 1 cross both, 2 reflect a, 3 reflect b, 4 reflect both
+//            bits 12+3*idx..+2   0 march, 1 cross both, 2 reflect a, 3 reflect b,
+//            4 reflect both, and when the order matters (a met first / b first):
+//            5 {b} / {a},  6 {a} / {},  7 {} / {b}
 //   |A| = 3, idx = 4*(cx > 0) + 2*(cy > 0) + (cz > 0):  bit 48+idx  1 cross all
 constexpr int CELL_PAIR_SHIFT = 12;
 constexpr int CELL_TRIPLE_SHIFT = 48;

@@ -3,14 +3,14 @@
 // One lane = one walker (hpc_code/lib/main.cu:64-70), but the grid is
 // persistent: each wavefront walks batches of 64 walkers taken from a batch
 // list that can span many geometries (jobs).  Walker state lives in VGPRs for
-// the whole chunk of steps; the lattice is read as one 32-bit cell word per
-// voxel (class + which straight moves stay inside same-class voxels) through
-// L2 / Infinity Cache.
+// the whole chunk of steps; the lattice is read as one 64-bit cell word per
+// voxel (how each of its 26 neighbours relates to its class) through L2 /
+// Infinity Cache, plus a class byte per voxel for the march and the start.
 //
 // A step whose straight move meets only same-class faces and faces next to
-// null voxels, in an order-independent way, is a few adds read off a 64-bit
-// cell word (the fast path); the other lanes run the face march in the same
-// step.
+// null voxels is the straight move folded back at the null faces, decided by
+// lookups in the cell word along the sequence of faces (the fast path); the
+// other lanes run the face march in the same step.
 //
 // Per-record moments are summed across the wavefront with a fixed butterfly
 // and written as per-batch partials; a second kernel sums the partials in a
@@ -294,14 +294,17 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
 }
 
 // Straight-move fast path.  The straight step ends at e = o + s*v and crosses
-// the faces of the axes where e leaves [0,1] (c_k = +-1).  The cell word says,
-// for this pattern of crossed faces, which of them reflect: same-class faces
-// are crossed without a draw or a turn, faces next to a null voxel reflect
-// without a draw (main.cu:274-289; P = 0 next to label 0, :450-451).  For two
-// crossed faces whose outcome depends on the order they are met in, the order
-// comes from comparing the distances to the faces, d_a |v_b| <= d_b |v_a|.
-// The step is the straight move folded back at the reflecting faces.  Returns
-// false when the march is needed.
+// the faces of the axes where e leaves [0,1] (c_k = +-1).  The crossed faces
+// are met in increasing d_k / |v_k| (d_k = distance to the face), compared
+// without a division: face a comes before face b (a < b) when
+// d_a |v_b| <= d_b |v_a|.  Met in that order, a face whose neighbour has the
+// voxel's class is crossed without a draw or a turn, one next to a null voxel
+// reflects without a draw (main.cu:274-289; P = 0 next to label 0,
+// :450-451).  The cell word holds every neighbour's relation to the voxel's
+// class, so the step is one lookup per face along the sequence and the
+// straight move folded back at the reflecting faces.  Returns false when a
+// face needs the march (a neighbour of another non-null class, or outside the
+// lattice) or when the three comparisons are not transitive (a near tie).
 template <typename R, int NP>
 __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, uint64_t cell, const V3<R> &v,
                                           Tan<R, NP> &T)
@@ -313,42 +316,31 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, ui
     const int32_t cx = ex > (R)1 ? 1 : (ex < (R)0 ? -1 : 0);
     const int32_t cy = ey > (R)1 ? 1 : (ey < (R)0 ? -1 : 0);
     const int32_t cz = ez > (R)1 ? 1 : (ez < (R)0 ? -1 : 0);
-    const int32_t ax = cx != 0, ay = cy != 0, az = cz != 0;
-    const int32_t nk = ax + ay + az;
-    const int32_t px = cx > 0, py = cy > 0, pz = cz > 0;
-    uint32_t rmask = 0;  // reflected axes (bit k)
-    bool ok = true;
-    if (nk == 1) {
-        const int32_t h = ax ? 0 : (ay ? 1 : 2);
-        const int32_t f = 2 * h + (px | py | pz);
-        const uint32_t code = (uint32_t)(cell >> (2 * f)) & 3u;
-        ok = code != 0u;
-        rmask = code == 2u ? (1u << h) : 0u;
-    } else if (nk == 2) {
-        const int32_t m = !ax ? 0 : (!ay ? 1 : 2);  // the axis not crossed
-        const int32_t a = m == 0 ? 1 : 0;           // first crossed axis
-        const int32_t b = m == 2 ? 1 : 2;           // second crossed axis
-        const int32_t sa = a == 0 ? px : py;
-        const int32_t sb = b == 1 ? py : pz;
-        const int32_t idx = 4 * (2 - m) + 2 * sa + sb;
-        const uint32_t code = (uint32_t)(cell >> (CELL_PAIR_SHIFT + 3 * idx)) & 7u;
-        ok = code != 0u;
-        if (code <= 4u) {
-            rmask = (code == 2u || code == 4u ? (1u << a) : 0u) | (code == 3u || code == 4u ? (1u << b) : 0u);
-        } else {
-            // the order matters: a is met first when d_a |v_b| <= d_b |v_a|
-            const R oa = a == 0 ? w.o.x : w.o.y, ob = b == 1 ? w.o.y : w.o.z;
-            const R va = a == 0 ? v.x : v.y, vb = b == 1 ? v.y : v.z;
-            const R da = sa ? (R)1 - oa : oa, db = sb ? (R)1 - ob : ob;
-            const bool afirst = da * fabs(vb) <= db * fabs(va);
-            const uint32_t A = 1u << a, B = 1u << b;
-            rmask = code == 5u ? (afirst ? B : A) : (code == 6u ? (afirst ? A : 0u) : (afirst ? 0u : B));
-        }
-    } else if (nk == 3) {
-        const int32_t idx = 4 * px + 2 * py + pz;
-        ok = ((cell >> (CELL_TRIPLE_SHIFT + idx)) & 1ull) != 0ull;
+    const R dx = cx > 0 ? (R)1 - w.o.x : w.o.x;
+    const R dy = cy > 0 ? (R)1 - w.o.y : w.o.y;
+    const R dz = cz > 0 ? (R)1 - w.o.z : w.o.z;
+    const R avx = fabs(v.x), avy = fabs(v.y), avz = fabs(v.z);
+    // face a before face b: a crossed, and b not crossed or met later
+    const bool xy = cx != 0 && (cy == 0 || dx * avy <= dy * avx);
+    const bool xz = cx != 0 && (cz == 0 || dx * avz <= dz * avx);
+    const bool yz = cy != 0 && (cz == 0 || dy * avz <= dz * avy);
+    // ranks in the sequence; a non-transitive triple gives all three rank 1
+    const int32_t kx = (int32_t)!xy + (int32_t)!xz;
+    const int32_t ky = (int32_t)xy + (int32_t)!yz;
+    // neighbour index 9*(qx+1) + 3*(qy+1) + (qz+1), the voxel itself is 13;
+    // an axis not crossed adds nothing and reads relation 0 of the current voxel
+    const int32_t ox = 9 * cx, oy = 3 * cy, oz = cz;
+    int32_t cur = 13;
+    uint32_t any = 0u, rmask = 0u;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        const int32_t q = cur + (kx == s ? ox : (ky == s ? oy : oz));
+        const uint32_t rel = (uint32_t)(cell >> (2 * q)) & 3u;
+        any |= rel;
+        rmask |= rel == 1u ? (kx == s ? 1u : (ky == s ? 2u : 4u)) : 0u;
+        cur = rel == 0u ? q : cur;
     }
-    if (!ok) return false;
+    if ((any & 2u) != 0u || kx == ky) return false;
     const bool rx = (rmask & 1u) != 0u, ry = (rmask & 2u) != 0u, rz = (rmask & 4u) != 0u;
     if constexpr (NP > 0) {
 #pragma unroll
@@ -1054,8 +1046,8 @@ __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)
 }
 
 // Cell words and classes of a label lattice: one thread per voxel reads its
-// 27-voxel neighbourhood and decides, for every pattern of crossed faces and
-// order, the straight-move outcome (see rms_types.h).
+// 27-voxel neighbourhood and writes each neighbour's relation to the voxel's
+// class (see rms_types.h).
 __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
                                    int32_t ny, int32_t nz, uint64_t *cells, uint8_t *cls_out)
 {
@@ -1066,79 +1058,22 @@ __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, cons
         const int64_t t = lin / nz;
         const int32_t y = (int32_t)(t % ny);
         const int32_t x = (int32_t)(t / ny);
-        // classes of the 3x3x3 neighbourhood, -1 outside the lattice
-        int32_t nb[27];
+        auto class_of = [&](int32_t xx, int32_t yy, int32_t zz) -> int32_t {
+            if (xx < 0 || xx >= nx || yy < 0 || yy >= ny || zz < 0 || zz >= nz) return -1;
+            const int64_t k = ((int64_t)xx * ny + yy) * nz + zz;
+            const uint32_t lab = label_bytes == 1 ? (uint32_t)((const uint8_t *)labels)[k]
+                                                  : (uint32_t)((const uint16_t *)labels)[k];
+            return lclass ? (int32_t)lclass[lab] : (int32_t)lab;
+        };
+        const int32_t c0 = class_of(x, y, z);
+        uint64_t word = 0;
         for (int dx = -1; dx <= 1; dx++)
             for (int dy = -1; dy <= 1; dy++)
                 for (int dz = -1; dz <= 1; dz++) {
-                    const int32_t xx = x + dx, yy = y + dy, zz = z + dz;
-                    int32_t c = -1;
-                    if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
-                        const int64_t k = ((int64_t)xx * ny + yy) * nz + zz;
-                        const uint32_t lab = label_bytes == 1 ? (uint32_t)((const uint8_t *)labels)[k]
-                                                              : (uint32_t)((const uint16_t *)labels)[k];
-                        c = lclass ? (int32_t)lclass[lab] : (int32_t)lab;
-                    }
-                    nb[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)] = c;
+                    const int32_t cq = class_of(x + dx, y + dy, z + dz);
+                    const uint64_t rel = cq == c0 ? CELL_SAME : (cq == 0 ? CELL_NULL : CELL_OTHER);
+                    word |= rel << (2 * ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)));
                 }
-        const int32_t c0 = nb[13];
-        auto at = [&](int qx, int qy, int qz) { return nb[(qx + 1) * 9 + (qy + 1) * 3 + (qz + 1)]; };
-        // faces met in the order ord[0..n-1] from this voxel: reflect set R
-        // (bits), or -1 when a face needs the march
-        auto order_outcome = [&](const int c[3], const int *ord, int n) -> int {
-            int p[3] = {0, 0, 0};
-            int R = 0;
-            for (int e = 0; e < n; e++) {
-                const int h = ord[e];
-                int q[3] = {p[0], p[1], p[2]};
-                q[h] += c[h];
-                const int32_t cq = at(q[0], q[1], q[2]);
-                if (cq == c0) {
-                    p[h] = q[h];
-                } else if (cq == 0) {
-                    R |= 1 << h;
-                } else {
-                    return -1;
-                }
-            }
-            return R;
-        };
-        uint64_t word = 0;
-        for (int h = 0; h < 3; h++)
-            for (int sgn = 0; sgn < 2; sgn++) {
-                int c[3] = {0, 0, 0};
-                c[h] = sgn ? 1 : -1;
-                const int R = order_outcome(c, &h, 1);
-                const uint64_t code = R < 0 ? 0u : (R == 0 ? 1u : 2u);
-                word |= code << (2 * (2 * h + sgn));
-            }
-        for (int m = 0; m < 3; m++) {
-            const int a = m == 0 ? 1 : 0, b = m == 2 ? 1 : 2;
-            for (int sa = 0; sa < 2; sa++)
-                for (int sb = 0; sb < 2; sb++) {
-                    int c[3] = {0, 0, 0};
-                    c[a] = sa ? 1 : -1;
-                    c[b] = sb ? 1 : -1;
-                    const int ab[2] = {a, b}, ba[2] = {b, a};
-                    const int oa = order_outcome(c, ab, 2), ob = order_outcome(c, ba, 2);
-                    const int A = 1 << a, B = 1 << b;
-                    uint64_t code = 0;
-                    if (oa >= 0 && ob >= 0) {
-                        if (oa == ob) code = oa == 0 ? 1 : (oa == A ? 2 : (oa == B ? 3 : 4));
-                        else if (oa == B && ob == A) code = 5;
-                        else if (oa == A && ob == 0) code = 6;
-                        else if (oa == 0 && ob == B) code = 7;
-                    }
-                    word |= code << (CELL_PAIR_SHIFT + 3 * (4 * (2 - m) + 2 * sa + sb));
-                }
-        }
-        for (int idx = 0; idx < 8; idx++) {
-            const int c[3] = {(idx & 4) ? 1 : -1, (idx & 2) ? 1 : -1, (idx & 1) ? 1 : -1};
-            bool all = true;
-            for (int S = 1; S < 8; S++)
-                all = all && at((S & 1) ? c[0] : 0, (S & 2) ? c[1] : 0, (S & 4) ? c[2] : 0) == c0;
-            if (all) word |= 1ull << (CELL_TRIPLE_SHIFT + idx);
-        }
         cells[lin] = word;
         cls_out[lin] = (uint8_t)c0;
     }

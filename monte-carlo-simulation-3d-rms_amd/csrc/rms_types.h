@@ -14,19 +14,16 @@ constexpr int WAVE = 64;       // walkers per batch
 constexpr int MAX_BOUNDS = 64; // distinct PGSE lobe boundaries
 constexpr uint32_t INIT_TRIALS = 1u << 26;
 
-// Cell word of a voxel (one uint64 per voxel, index (x*ny + y)*nz + z): what
-// a straight step from this voxel does for each pattern of crossed faces
-// (cx, cy, cz) in {-1,0,1}^3.  Met in a given order, a face whose neighbour
-// has the class of the path's voxel is crossed, one whose neighbour is a null
-// voxel inside the lattice reflects, anything else needs the face march.
-//   |A| = 1, f = 2*axis + (c > 0):  bits 2f..2f+1      0 march, 1 cross, 2 reflect
-//   |A| = 2, axes a < b, missing axis m, idx = 4*(2-m) + 2*(ca > 0) + (cb > 0):
-//            bits 12+3*idx..+2   0 march, 1 cross both, 2 reflect a, 3 reflect b,
-//            4 reflect both, and when the order matters (a met first / b first):
-//            5 {b} / {a},  6 {a} / {},  7 {} / {b}
-//   |A| = 3, idx = 4*(cx > 0) + 2*(cy > 0) + (cz > 0):  bit 48+idx  1 cross all
-constexpr int CELL_PAIR_SHIFT = 12;
-constexpr int CELL_TRIPLE_SHIFT = 48;
+// Cell word of a voxel (one uint64 per voxel, index (x*ny + y)*nz + z): the
+// relation of each voxel of its 3x3x3 neighbourhood to the voxel's class, two
+// bits at 2*(9*(qx+1) + 3*(qy+1) + (qz+1)) for the neighbour at offset
+// (qx, qy, qz) in {-1,0,1}^3 (the voxel itself is entry 13, bits 26-27, and
+// holds CELL_SAME).  A straight move crosses a CELL_SAME face, reflects off a
+// CELL_NULL face and needs the face march at a CELL_OTHER face (another
+// non-null class, or outside the lattice).  Bits 54-63 are zero.
+constexpr uint64_t CELL_SAME = 0;
+constexpr uint64_t CELL_NULL = 1;
+constexpr uint64_t CELL_OTHER = 2;
 
 struct JobDesc {
     // geometry

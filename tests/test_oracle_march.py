@@ -201,3 +201,33 @@ def test_fast_path_equals_march(orc):
     # pure interior: always applies and is the straight move
     f = orc.fast_step_f64(g, dt, [3, 3, 3], [0.5, 0.5, 0.5], [0.6, 0.0, 0.8], 1)
     np.testing.assert_allclose(f["o"], [0.5 + step * 0.6, 0.5, 0.5 + step * 0.8], atol=1e-15)
+
+
+def test_fast_path_three_faces_in_order(orc):
+    """Steps that cross three faces next to random null voxels of one octant:
+    the fast path meets them in the order of the face distances and agrees
+    with the face march (same voxel, counts, position up to rounding) on every
+    step whose comparisons are transitive, including the steps whose outcome
+    depends on that order."""
+    rng = np.random.default_rng(5)
+    dt = 2e-4
+    n_applied = n_mixed = 0
+    for trial in range(400):
+        lab = np.ones((5, 5, 5), np.uint8)
+        for S in range(1, 8):
+            if rng.random() < 0.45:
+                lab[2 + (S & 1), 2 + ((S >> 1) & 1), 2 + ((S >> 2) & 1)] = 0
+        g = orc.Geometry(lab, 0.1, [0.0, 2.0], T2=[np.inf] * 2)
+        o = rng.uniform(0.8, 0.99, 3)
+        v = rng.uniform(0.45, 0.7, 3)
+        v /= np.linalg.norm(v)
+        f = orc.fast_step_f64(g, dt, [2, 2, 2], o, v, 1)
+        m = orc.march_f64(g, dt, [2, 2, 2], o, v, 1)
+        assert f is not None, (trial, lab[2:4, 2:4, 2:4])
+        n_applied += 1
+        np.testing.assert_array_equal(f["g"], m["g"])
+        np.testing.assert_allclose(f["o"], m["o"], atol=1e-12)
+        assert f["cnt"][0] == m["cnt"][0] and f["cnt"][1] == m["cnt"][1]
+        assert f["cnt"][0] + f["cnt"][1] == 3  # three faces met
+        n_mixed += 0 < f["cnt"][1] < 3
+    assert n_applied == 400 and n_mixed > 100

@@ -3,7 +3,9 @@
 hipcc compiles the kernels and the C ABI into one shared library; the two
 command-line programs that honour the reference file contract
 (hpc_code/lib/main.cu and main_PGSE.cu) are plain C++ clients of that ABI.
-Outputs stay in-tree: lib/librms_amd.so, bin/rms_main, bin/rms_main_pgse.
+Outputs stay in-tree: lib/librms_amd.so, bin/rms_main, bin/rms_main_pgse,
+and bin/rms_math_check (exhaustive checks of the kernel's FP32 math shortcuts,
+a GPU program the tests run).
 """
 from __future__ import annotations
 
@@ -68,10 +70,22 @@ def build_cli(verbose=False):
     return outs
 
 
+def build_math_check(verbose=False):
+    os.makedirs(BIN_DIR, exist_ok=True)
+    exe = os.path.join(BIN_DIR, "rms_math_check")
+    flags = [f for f in HIP_FLAGS if f != "-fPIC"]
+    out = _run([HIPCC, *flags, "-o", exe + ".tmp", os.path.join(CSRC, "rms_math_check.hip")])
+    os.replace(exe + ".tmp", exe)
+    if verbose and out:
+        print(out)
+    return exe
+
+
 def build(verbose=False):
     lib = build_lib(verbose)
     clis = build_cli(verbose)
-    return [lib, *clis]
+    chk = build_math_check(verbose)
+    return [lib, *clis, chk]
 
 
 if __name__ == "__main__":

@@ -31,8 +31,9 @@ __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
         const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
         const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        const uint32_t n0 = hi1 ^ c1 ^ k0;
-        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        // three-input xor in one v_bitop3_b32 (truth table 0x96)
+        const uint32_t n0 = r == 0 ? hi1 ^ (c1 ^ k0) : __builtin_amdgcn_bitop3_b32(hi1, c1, k0, 0x96);
+        const uint32_t n2 = r == 0 ? hi0 ^ (c3 ^ k1) : __builtin_amdgcn_bitop3_b32(hi0, c3, k1, 0x96);
         c0 = n0;
         c1 = lo1;
         c2 = n2;
@@ -135,6 +136,37 @@ __device__ __forceinline__ void sincos_turns(float t, float &s, float &c)
     float sx, cx;
     sc_poly(x, sx, cx);
     quadrant<float>((int)k, sx, cx, s, c);
+}
+
+// sin(2*pi*t), cos(2*pi*t) for t = u01_f32(w) = m * 2^-24, m = (w >> 8) | 1,
+// with the reduction done on the integer m: the quarter-turn index
+// k = floor(4t + 1/2) = (m + 2^21) >> 22 and t - k/4 = (m - k*2^22) * 2^-24 are
+// the exact values the float reduction above computes, so the result is the
+// same bit for bit.
+__device__ __forceinline__ void sincos_turns_word(uint32_t w, float &s, float &c)
+{
+    const uint32_t m = (w >> 8) | 1u;
+    const uint32_t k = (m + (1u << 21)) >> 22;
+    const int32_t num = (int32_t)m - (int32_t)(k << 22);
+    const float x = (float)num * (6.2831854820251465f * 5.9604644775390625e-08f);
+    float sx, cx;
+    sc_poly(x, sx, cx);
+    quadrant<float>((int)k, sx, cx, s, c);
+}
+
+// Correctly rounded square root of a normal x far from the ends of the float
+// range (the direction draw takes it of u(1-u) in [2^-25, 1/4]): the hardware
+// estimate, within one ulp, moves to the neighbour that the square residuals
+// x - s_down*s and x - s_up*s show to be the rounded root.
+__device__ __forceinline__ float sqrt_rn_mid(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    const float r = rd <= 0.0f ? sd : s;
+    return ru > 0.0f ? su : r;
 }
 
 // exp(x), x <= 0: Cody-Waite reduction by ln 2, Taylor to r^13.

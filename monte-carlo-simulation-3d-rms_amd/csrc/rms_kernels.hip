@@ -280,17 +280,21 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
             b1 = r.w;
         }
         u1 = u01_f32(w0);
-        u2 = u01_f32(w1);
+        const R cth = (R)1 - (R)2 * u1;
+        const R sth = (R)2 * sqrt_rn_mid(u1 * ((R)1 - u1));
+        R sp, cp;
+        sincos_turns_word(w1, sp, cp);  // = sincos_turns(u01_f32(w1))
+        return V3<R>{sth * cp, sth * sp, cth};
     } else {
         U4 r = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_DIR, wid);
         u1 = u01_f64(r.x, r.y);
         u2 = u01_f64(r.z, r.w);
+        const R cth = (R)1 - (R)2 * u1;
+        const R sth = (R)2 * sqrt(u1 * ((R)1 - u1));
+        R sp, cp;
+        sincos_turns(u2 * PiRef<R>::v, sp, cp);
+        return V3<R>{sth * cp, sth * sp, cth};
     }
-    const R cth = (R)1 - (R)2 * u1;
-    const R sth = (R)2 * sqrt(u1 * ((R)1 - u1));
-    R sp, cp;
-    sincos_turns(u2 * PiRef<R>::v, sp, cp);
-    return V3<R>{sth * cp, sth * sp, cth};
 }
 
 // Straight-move fast path.  The straight step ends at e = o + s*v and crosses
@@ -329,19 +333,24 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, ui
     const int32_t ky = (int32_t)xy + (int32_t)!yz;
     // neighbour index 9*(qx+1) + 3*(qy+1) + (qz+1), the voxel itself is 13;
     // an axis not crossed adds nothing and reads relation 0 of the current voxel
-    const int32_t ox = 9 * cx, oy = 3 * cy, oz = cz;
-    int32_t cur = 13;
-    uint32_t any = 0u, rmask = 0u;
+    // (bit offsets 2*index are tracked)
+    const int32_t ox = 18 * cx, oy = 6 * cy, oz = 2 * cz;
+    int32_t cur = 26;
+    uint32_t rels = 0u;  // relation met in slot s, bits 2s..2s+1
 #pragma unroll
     for (int s = 0; s < 3; s++) {
         const int32_t q = cur + (kx == s ? ox : (ky == s ? oy : oz));
-        const uint32_t rel = (uint32_t)(cell >> (2 * q)) & 3u;
-        any |= rel;
-        rmask |= rel == 1u ? (kx == s ? 1u : (ky == s ? 2u : 4u)) : 0u;
+        const uint32_t rel = (uint32_t)(cell >> q) & 3u;
+        rels |= rel << (2 * s);
         cur = rel == 0u ? q : cur;
     }
-    if ((any & 2u) != 0u || kx == ky) return false;
-    const bool rx = (rmask & 1u) != 0u, ry = (rmask & 2u) != 0u, rz = (rmask & 4u) != 0u;
+    if ((rels & 0x2Au) != 0u || kx == ky) return false;
+    // every relation met is CELL_SAME (crossed, or axis not crossed) or
+    // CELL_NULL (reflected); the ranks are a permutation of 0, 1, 2
+    const int32_t kz = 3 - kx - ky;
+    const bool rx = ((rels >> (2 * kx)) & 1u) != 0u;
+    const bool ry = ((rels >> (2 * ky)) & 1u) != 0u;
+    const bool rz = ((rels >> (2 * kz)) & 1u) != 0u;
     if constexpr (NP > 0) {
 #pragma unroll
         for (int p = 0; p < NP; p++) {
@@ -362,9 +371,10 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, ui
     w.g.x += gx;
     w.g.y += gy;
     w.g.z += gz;
-    w.lin += gx * L.sx + gy * L.nz + gz;
-    w.ncross += (uint32_t)(gx != 0) + (uint32_t)(gy != 0) + (uint32_t)(gz != 0);
-    w.nrefl += (uint32_t)rx + (uint32_t)ry + (uint32_t)rz;
+    w.lin += (gx == 0 ? 0 : (gx > 0 ? L.sx : -L.sx)) + (gy == 0 ? 0 : (gy > 0 ? L.nz : -L.nz)) + gz;
+    const uint32_t nref = (uint32_t)__builtin_popcount(rels);
+    w.nrefl += nref;
+    w.ncross += (uint32_t)(__mul24(cx, cx) + __mul24(cy, cy) + __mul24(cz, cz)) - nref;
     w.tcur = w.tcur + (R)1;
     return true;
 }

@@ -70,6 +70,7 @@ struct Walker {
     V3<R> o;         // offset inside the voxel, [0,1]
     V3<int32_t> gi;  // origin voxel (mirror-unfolded)
     V3<R> oi;        // origin offset
+    V3<double> xi;   // gi + oi (PGSE sums)
     int32_t c;       // class
     int32_t lin;     // linear voxel index of g
     R tcur;          // steps spent in class c since the last flush
@@ -208,6 +209,13 @@ __device__ __forceinline__ void flush(Walker<R> &w)
     w.tcur = (R)0;
 }
 
+// The origin as one double per axis, the term the PGSE sums subtract.
+template <typename R>
+__device__ __forceinline__ void set_origin_sum(Walker<R> &w)
+{
+    w.xi = {(double)w.gi.x + (double)w.oi.x, (double)w.gi.y + (double)w.oi.y, (double)w.gi.z + (double)w.oi.z};
+}
+
 // Rejection-sampled start voxel, main.cu:117-132 (uniforms over (0,1)).
 template <typename R>
 __device__ __forceinline__ void init_walker(Walker<R> &w, const JobLocal<R> &L, uint64_t wid)
@@ -242,6 +250,7 @@ __device__ __forceinline__ void init_walker(Walker<R> &w, const JobLocal<R> &L, 
     // lattices without a non-null voxel are rejected on the host
     w.gi = w.g;
     w.oi = w.o;
+    set_origin_sum(w);
 }
 
 // Exchange draw j (0..3) of step i: f32 layout takes word j of block (i, 1),
@@ -489,6 +498,7 @@ __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const 
                     at<k>(w.gi) = h ? gm : at<k>(w.gi);
                     at<k>(w.oi) = h ? -at<k>(w.oi) : at<k>(w.oi);
                 });
+                set_origin_sum(w);
                 w.nmirror++;
                 flip = true;
             } else {
@@ -611,6 +621,7 @@ __device__ __forceinline__ void load_state(Walker<R> &w, const StateBuf &st, int
         w.oi = {st.od[3 * ws + s], st.od[4 * ws + s], st.od[5 * ws + s]};
         w.tcur = st.tcd[s];
     }
+    set_origin_sum(w);
     w.c = st.c[s];
     w.ts0 = st.tsum[s];
     w.ts1 = st.tsum[ws + s];
@@ -802,30 +813,31 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
             if (!fast_step<R, NP>(w, L, cell, v, T)) march<R, NP>(w, L, J, wid, i, v, T);
             if (w.lin != lin0) cell = L.cells[w.lin];  // only lanes that changed voxel read
             if (SIG & 2) {
-                const V3<double> d = displacement<R>(w, L.res);
-                ps.x = ps.x + d.x;
-                ps.y = ps.y + d.y;
-                ps.z = ps.z + d.z;
+                // running sums of the displacement in voxels, (g + o) - (gi + oi);
+                // the snapshots at lobe boundaries are scaled to um
+                ps.x = ps.x + (((double)w.g.x + (double)w.o.x) - w.xi.x);
+                ps.y = ps.y + (((double)w.g.y + (double)w.o.y) - w.xi.y);
+                ps.z = ps.z + (((double)w.g.z + (double)w.o.z) - w.xi.z);
                 if constexpr (NP > 0) {
 #pragma unroll
                     for (int p = 0; p < NP; p++) {
-                        T.PJ[p].x = T.PJ[p].x + (double)T.Jo[p].x * L.res;
-                        T.PJ[p].y = T.PJ[p].y + (double)T.Jo[p].y * L.res;
-                        T.PJ[p].z = T.PJ[p].z + (double)T.Jo[p].z * L.res;
+                        T.PJ[p].x = T.PJ[p].x + (double)T.Jo[p].x;
+                        T.PJ[p].y = T.PJ[p].y + (double)T.Jo[p].y;
+                        T.PJ[p].z = T.PJ[p].z + (double)T.Jo[p].z;
                     }
                 }
                 if (i + 1 == bnext) {
                     double *sp = J.snap + (int64_t)(kb * 3) * ws + slot;
-                    sp[0] = ps.x;
-                    sp[ws] = ps.y;
-                    sp[2 * ws] = ps.z;
+                    sp[0] = ps.x * L.res;
+                    sp[ws] = ps.y * L.res;
+                    sp[2 * ws] = ps.z * L.res;
                     if constexpr (NP > 0) {
 #pragma unroll
                         for (int p = 0; p < NP; p++) {
                             double *tp = J.tsnap + (int64_t)((kb * NP + p) * 3) * ws + slot;
-                            tp[0] = T.PJ[p].x;
-                            tp[ws] = T.PJ[p].y;
-                            tp[2 * ws] = T.PJ[p].z;
+                            tp[0] = T.PJ[p].x * L.res;
+                            tp[ws] = T.PJ[p].y * L.res;
+                            tp[2 * ws] = T.PJ[p].z * L.res;
                         }
                     }
                     kb++;

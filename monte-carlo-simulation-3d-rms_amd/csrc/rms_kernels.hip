@@ -306,7 +306,7 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
     }
 }
 
-// Straight-move fast path.  The straight step ends at e = o + s*v and crosses
+// Straight-move fast path.  The straight step ends at e = fma(s, v, o) and crosses
 // the faces of the axes where e leaves [0,1] (c_k = +-1).  The crossed faces
 // are met in increasing d_k / |v_k| (d_k = distance to the face), compared
 // without a division: face a comes before face b (a < b) when
@@ -323,9 +323,9 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, ui
                                           Tan<R, NP> &T)
 {
     const R sc = w.sc;
-    const R ex = w.o.x + sc * v.x;
-    const R ey = w.o.y + sc * v.y;
-    const R ez = w.o.z + sc * v.z;
+    const R ex = fma(sc, v.x, w.o.x);
+    const R ey = fma(sc, v.y, w.o.y);
+    const R ez = fma(sc, v.z, w.o.z);
     const int32_t cx = ex > (R)1 ? 1 : (ex < (R)0 ? -1 : 0);
     const int32_t cy = ey > (R)1 ? 1 : (ey < (R)0 ? -1 : 0);
     const int32_t cz = ez > (R)1 ? 1 : (ez < (R)0 ? -1 : 0);
@@ -792,22 +792,24 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
             w.isc = sel4(w.c, L.is0, L.is1, L.is2, L.is3);
             if (SIG & 2) ps = {a.st.ps[slot], a.st.ps[ws + slot], a.st.ps[2 * ws + slot]};
         }
-        // next record step and next PGSE boundary at or after step0
-        int64_t rnext = ((a.step0 + a.Tstep - 1) / a.Tstep) * a.Tstep;
-        const int64_t rend = (int64_t)a.timepoints * a.Tstep;  // records only for r < timepoints
+        // step counters fit int32 (TN < 2^31 is checked on the host) and are
+        // wave-uniform: next record step and next PGSE boundary at or after step0
+        const int32_t step0 = (int32_t)a.step0, step1 = (int32_t)a.step1, Tstep = (int32_t)a.Tstep;
+        int32_t rnext = ((step0 + Tstep - 1) / Tstep) * Tstep;
+        const int32_t rend = a.timepoints * Tstep;  // records only for r < timepoints
         int kb = 1;
-        int64_t bnext = -1;
+        int32_t bnext = -1;
         if (SIG & 2) {
             while (kb <= nbound && J.bounds[kb] <= a.step0) kb++;
-            bnext = kb <= nbound ? J.bounds[kb] : -1;
+            bnext = __builtin_amdgcn_readfirstlane(kb <= nbound ? (int32_t)J.bounds[kb] : -1);
         }
         uint32_t b0 = 0, b1 = 0;
         double *part = J.part + lb * (int64_t)a.timepoints * nval;
-        int64_t r = rnext / a.Tstep;  // record index of rnext
+        int32_t r = rnext / Tstep;  // record index of rnext
         uint64_t cell = L.cells[w.lin];
 
-        for (int64_t i = a.step0; i < a.step1; i++) {
-            const bool fresh = ((i & 1) == 0) || (i == a.step0);
+        for (int32_t i = step0; i < step1; i++) {
+            const bool fresh = ((i & 1) == 0) || (i == step0);
             const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1);
             const int32_t lin0 = w.lin;
             if (!fast_step<R, NP>(w, L, cell, v, T)) march<R, NP>(w, L, J, wid, i, v, T);
@@ -841,17 +843,17 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
                         }
                     }
                     kb++;
-                    bnext = kb <= nbound ? J.bounds[kb] : -1;
+                    bnext = __builtin_amdgcn_readfirstlane(kb <= nbound ? (int32_t)J.bounds[kb] : -1);
                 }
             }
             if (i == rnext && i < rend) {
                 // records, main.cu:333-378
-                rnext += a.Tstep;
+                rnext += Tstep;
                 const double s0 = t2_weight<R>(w, L);
                 const V3<double> d = displacement<R>(w, L.res);
                 const double x = d.x, y = d.y, z = d.z;
                 if constexpr (NP == 0) {
-                    double *dst = part + r * nval;
+                    double *dst = part + (int64_t)r * nval;
                     put(s0, active, dst + 0, lane);
                     for (int cc = 0; cc < L.nclass; cc++) put(cc == w.c ? 1.0 : 0.0, active, dst + 1 + cc, lane);
                     double *m = dst + 1 + L.nclass;
@@ -877,7 +879,7 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
                     put(s0 * y * z * z * z, active, m + 19, lane);
                     put(s0 * z * z * z * z, active, m + 20, lane);
                     if (SIG & 1) {
-                        const double *qs = J.qscale + r * nbvec;
+                        const double *qs = J.qscale + (int64_t)r * nbvec;
                         for (int jb = 0; jb < nbvec; jb++) {
                             const double *bv = J.bvec + 3 * jb;
                             const double qx = qs[jb] * (x * bv[0] + y * bv[1] + z * bv[2]);
@@ -887,12 +889,12 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
                 } else if (J.cot_rec) {
                     // adjoint: contract this record row with its cotangent
                     double F, g0, g1, g2;
-                    adj_record(J.cot_rec + r * nval, L.nclass, nbvec, J.bvec,
-                               (SIG & 1) ? J.qscale + r * nbvec : nullptr, x, y, z, F, g0, g1, g2);
+                    adj_record(J.cot_rec + (int64_t)r * nval, L.nclass, nbvec, J.bvec,
+                               (SIG & 1) ? J.qscale + (int64_t)r * nbvec : nullptr, x, y, z, F, g0, g1, g2);
                     const double sF = s0 * F;
                     const double s0r = s0 * L.res;
                     // NPar_count is unweighted: only the score terms
-                    const double fc = J.cot_rec[r * nval + 1 + w.c];
+                    const double fc = J.cot_rec[(int64_t)r * nval + 1 + w.c];
 #pragma unroll
                     for (int p = 0; p < NP; p++) {
                         if (p < L.nclass - 1) {

@@ -511,6 +511,7 @@ static inline double or_pgse_phase(const or_jobaux *aux, const double *gr, int j
 
 #define FABS fabs
 #define SQRT sqrt
+#define FMA fma
 #define SINCOS_TURNS or_sincos_turns_f64
 #define PIREF 0.9999999988573335
 #define REAL double
@@ -519,6 +520,7 @@ static inline double or_pgse_phase(const or_jobaux *aux, const double *gr, int j
 #include "rms_oracle_core.inc"
 #undef FABS
 #undef SQRT
+#undef FMA
 #undef SINCOS_TURNS
 #undef PIREF
 #undef REAL
@@ -527,6 +529,7 @@ static inline double or_pgse_phase(const or_jobaux *aux, const double *gr, int j
 
 #define FABS fabsf
 #define SQRT sqrtf
+#define FMA fmaf
 #define SINCOS_TURNS or_sincos_turns_f32
 #define PIREF 1.0f
 #define REAL float
@@ -535,6 +538,7 @@ static inline double or_pgse_phase(const or_jobaux *aux, const double *gr, int j
 #include "rms_oracle_core.inc"
 #undef FABS
 #undef SQRT
+#undef FMA
 #undef SINCOS_TURNS
 #undef PIREF
 #undef REAL

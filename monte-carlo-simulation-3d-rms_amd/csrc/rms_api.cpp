@@ -505,6 +505,9 @@ extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t
         for (int a = 1; a < G.meta.nclass; a++)
             for (int b = 1; b < G.meta.nclass; b++)
                 if (a != b && Pcat[a * NC + b] != 0) H.exchange = true;
+        // the walk kernel with faces between the two non-null classes on the
+        // fast path (reflect at P = 0, draw at 0 < P < 1)
+        if (G.meta.nclass == 3) p->sig |= 4;
         H.geom_id = J.geom_id;
         H.nwalk = J.nwalk;
         H.nbatch = (J.nwalk + WAVE - 1) / WAVE;
@@ -673,7 +676,7 @@ extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t
     p->block = P.block_threads > 0 ? P.block_threads : 256;
     if (p->block % WAVE != 0 || p->block > 256) BAD("block_threads must be a multiple of 64 and <= 256");
     int bpc = 0;
-    if (walk_occupancy(P.precision == RMS_PREC_F64, sig, p->block, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+    if (walk_occupancy(P.precision == RMS_PREC_F64, p->sig, p->block, &bpc) != hipSuccess || bpc < 1) bpc = 1;
     const int wpb = p->block / WAVE;
     int64_t need = (nbatch + wpb - 1) / wpb;
     int64_t resident = (int64_t)bpc * ctx->ncu;

@@ -104,6 +104,8 @@ struct JobLocal {
     double T0, T1, T2, T3;   // T2 in steps
     double res;
     const R *P;              // permeation probabilities [NC*NC]
+    R P12, P21;              // P between classes 1 and 2
+    int32_t exch_fast;       // two non-null classes: a CELL_OTHER face is the other one
     // adjoint constants
     R ds1, ds2, ds3;         // d step_c / d D_c, classes 1..3
     R di1, di2, di3;         // d (1/step_c) / d D_c
@@ -151,6 +153,9 @@ __device__ __forceinline__ JobLocal<R> job_local(const JobDesc &J)
         L.is3 = J.inv_step_d[3];
         L.P = J.P_d;
     }
+    L.P12 = L.P[1 * NC + 2];
+    L.P21 = L.P[2 * NC + 1];
+    L.exch_fast = J.nclass <= 3;
     L.T0 = J.T2s[0];
     L.T1 = J.T2s[1];
     L.T2 = J.T2s[2];
@@ -318,9 +323,9 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
 // straight move folded back at the reflecting faces.  Returns false when a
 // face needs the march (a neighbour of another non-null class, or outside the
 // lattice) or when the three comparisons are not transitive (a near tie).
-template <typename R, int NP>
-__device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, uint64_t cell, const V3<R> &v,
-                                          Tan<R, NP> &T)
+template <typename R, int NP, bool EXCH>
+__device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t cell,
+                                          const V3<R> &v, uint64_t wid, int32_t i, Tan<R, NP> &T)
 {
     const R sc = w.sc;
     const R ex = fma(sc, v.x, w.o.x);
@@ -353,13 +358,64 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, ui
         rels |= rel << (2 * s);
         cur = rel == 0u ? q : cur;
     }
-    if ((rels & 0x2Au) != 0u || kx == ky) return false;
-    // every relation met is CELL_SAME (crossed, or axis not crossed) or
-    // CELL_NULL (reflected); the ranks are a permutation of 0, 1, 2
+    uint32_t ndraw = 0u, refl = rels;  // reflected slots, bits 2s
+    if ((rels & 0x2Au) != 0u || kx == ky) {
+        // a near tie or a lattice edge (mirror) goes to the march.  Exchange
+        // faces (the other non-null class when there are two): P = 0 reflects
+        // without a draw, P = 1 crosses into the other class (march);
+        // otherwise the k-th exchange face of the step takes draw k (the
+        // march's numbering) and reflects when u >= P, a crossing draw sends
+        // the step to the march, which replays the same draws
+        if constexpr (!EXCH) return false;
+        if (kx == ky || (rels & (rels >> 1) & 0x15u) != 0u || !L.exch_fast) return false;
+        const int32_t pidx = w.c * NC + (3 - w.c);
+        const uint32_t cat = (L.pcat >> (2 * pidx)) & 3u;
+        if (cat == 1u) return false;
+        if (cat == 2u) {
+            const R P = w.c == 1 ? L.P12 : L.P21;
+            bool cross = false;
+            if constexpr (sizeof(R) == 4) {
+                const U4 blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A, wid);
+#pragma unroll
+                for (int s = 0; s < 3; s++) {
+                    if (((rels >> (2 * s)) & 3u) == 2u) {
+                        const uint32_t word = ndraw == 0u ? blk.x : (ndraw == 1u ? blk.y : blk.z);
+                        cross = cross || u01_f32(word) < P;
+                        ndraw++;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < 3; s++) {
+                    if (((rels >> (2 * s)) & 3u) == 2u) {
+                        cross = cross || exch_uniform<R>(L, wid, i, (int)ndraw) < P;
+                        ndraw++;
+                    }
+                }
+            }
+            if (cross) return false;
+            if constexpr (NP > 0) {
+                // scores of the reflecting Bernoulli(P) draws, one add per draw
+                const double *sc4 = J.dlog1mP + 4 * pidx;
+#pragma unroll
+                for (uint32_t k = 0; k < 3u; k++) {
+                    if (k < ndraw) {
+#pragma unroll
+                        for (int p = 0; p < NP; p++) T.S[p] = T.S[p] + sc4[p];
+                        T.Sk = T.Sk + sc4[3];
+                    }
+                }
+            }
+        }
+        refl = (rels | (rels >> 1)) & 0x15u;
+    }
+    // every face met is now crossed (CELL_SAME, or axis not crossed) or
+    // reflected (CELL_NULL, or a reflecting exchange face); the ranks are a
+    // permutation of 0, 1, 2
     const int32_t kz = 3 - kx - ky;
-    const bool rx = ((rels >> (2 * kx)) & 1u) != 0u;
-    const bool ry = ((rels >> (2 * ky)) & 1u) != 0u;
-    const bool rz = ((rels >> (2 * kz)) & 1u) != 0u;
+    const bool rx = ((refl >> (2 * kx)) & 1u) != 0u;
+    const bool ry = ((refl >> (2 * ky)) & 1u) != 0u;
+    const bool rz = ((refl >> (2 * kz)) & 1u) != 0u;
     if constexpr (NP > 0) {
 #pragma unroll
         for (int p = 0; p < NP; p++) {
@@ -381,8 +437,9 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, ui
     w.g.y += gy;
     w.g.z += gz;
     w.lin += (gx == 0 ? 0 : (gx > 0 ? L.sx : -L.sx)) + (gy == 0 ? 0 : (gy > 0 ? L.nz : -L.nz)) + gz;
-    const uint32_t nref = (uint32_t)__builtin_popcount(rels);
+    const uint32_t nref = (uint32_t)__builtin_popcount(refl);
     w.nrefl += nref;
+    w.ndraw += ndraw;
     w.ncross += (uint32_t)(__mul24(cx, cx) + __mul24(cy, cy) + __mul24(cz, cz)) - nref;
     w.tcur = w.tcur + (R)1;
     return true;
@@ -728,8 +785,13 @@ __device__ __forceinline__ void adj_record(const double *cot, int nclass, int nb
 // bit 1 PGSE signal.  NP > 0 instantiates the adjoint pass: the same walk
 // with tangents for D of classes 1..NP, contracted with the cotangents of the
 // record rows and PGSE signal into per-batch parameter gradients.
+//
+// Occupancy targets (waves per SIMD, 512 VGPRs each): 3 for the forward walk
+// and the FP32 narrow-pulse tangent pass (at most a few spills outside the
+// step loop), 2 for the PGSE and FP64 tangent passes.
 template <typename R, int SIG, int NP>
-__global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
+__global__ __launch_bounds__(256, (NP == 0 || (sizeof(R) == 4 && (SIG & 2) == 0)) ? 3 : 2)
+void walk_kernel(LaunchArgs a)
 {
     const int lane = (int)(threadIdx.x & 63u);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -812,7 +874,7 @@ __global__ __launch_bounds__(256) void walk_kernel(LaunchArgs a)
             const bool fresh = ((i & 1) == 0) || (i == step0);
             const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1);
             const int32_t lin0 = w.lin;
-            if (!fast_step<R, NP>(w, L, cell, v, T)) march<R, NP>(w, L, J, wid, i, v, T);
+            if (!fast_step<R, NP, (SIG & 4) != 0>(w, L, J, cell, v, wid, i, T)) march<R, NP>(w, L, J, wid, i, v, T);
             if (w.lin != lin0) cell = L.cells[w.lin];  // only lanes that changed voxel read
             if (SIG & 2) {
                 // running sums of the displacement in voxels, (g + o) - (gi + oi);
@@ -1095,7 +1157,8 @@ __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, cons
             for (int dy = -1; dy <= 1; dy++)
                 for (int dz = -1; dz <= 1; dz++) {
                     const int32_t cq = class_of(x + dx, y + dy, z + dz);
-                    const uint64_t rel = cq == c0 ? CELL_SAME : (cq == 0 ? CELL_NULL : CELL_OTHER);
+                    const uint64_t rel =
+                        cq == c0 ? CELL_SAME : (cq == 0 ? CELL_NULL : (cq < 0 ? CELL_EDGE : CELL_OTHER));
                     word |= rel << (2 * ((dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)));
                 }
         cells[lin] = word;
@@ -1107,19 +1170,29 @@ __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, cons
 template <int NP>
 static const void *walk_fn(int prec_f64, int sig)
 {
+    // sig bit 0: narrow-pulse b-table, bit 1: PGSE table, bit 2: exchange
+    // faces on the fast path
     if (prec_f64) {
-        switch (sig) {
+        switch (sig & 7) {
         case 0: return (const void *)walk_kernel<double, 0, NP>;
         case 1: return (const void *)walk_kernel<double, 1, NP>;
         case 2: return (const void *)walk_kernel<double, 2, NP>;
-        default: return (const void *)walk_kernel<double, 3, NP>;
+        case 3: return (const void *)walk_kernel<double, 3, NP>;
+        case 4: return (const void *)walk_kernel<double, 4, NP>;
+        case 5: return (const void *)walk_kernel<double, 5, NP>;
+        case 6: return (const void *)walk_kernel<double, 6, NP>;
+        default: return (const void *)walk_kernel<double, 7, NP>;
         }
     }
-    switch (sig) {
+    switch (sig & 7) {
     case 0: return (const void *)walk_kernel<float, 0, NP>;
     case 1: return (const void *)walk_kernel<float, 1, NP>;
     case 2: return (const void *)walk_kernel<float, 2, NP>;
-    default: return (const void *)walk_kernel<float, 3, NP>;
+    case 3: return (const void *)walk_kernel<float, 3, NP>;
+    case 4: return (const void *)walk_kernel<float, 4, NP>;
+    case 5: return (const void *)walk_kernel<float, 5, NP>;
+    case 6: return (const void *)walk_kernel<float, 6, NP>;
+    default: return (const void *)walk_kernel<float, 7, NP>;
     }
 }
 

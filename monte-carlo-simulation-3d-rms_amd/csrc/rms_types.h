@@ -18,12 +18,14 @@ constexpr uint32_t INIT_TRIALS = 1u << 26;
 // relation of each voxel of its 3x3x3 neighbourhood to the voxel's class, two
 // bits at 2*(9*(qx+1) + 3*(qy+1) + (qz+1)) for the neighbour at offset
 // (qx, qy, qz) in {-1,0,1}^3 (the voxel itself is entry 13, bits 26-27, and
-// holds CELL_SAME).  A straight move crosses a CELL_SAME face, reflects off a
-// CELL_NULL face and needs the face march at a CELL_OTHER face (another
-// non-null class, or outside the lattice).  Bits 54-63 are zero.
+// holds CELL_SAME).  A straight move crosses a CELL_SAME face and reflects off
+// a CELL_NULL face; a CELL_OTHER face (another non-null class) is an exchange
+// face, and a CELL_EDGE face (outside the lattice) a mirror.  Bits 54-63 are
+// zero.
 constexpr uint64_t CELL_SAME = 0;
 constexpr uint64_t CELL_NULL = 1;
 constexpr uint64_t CELL_OTHER = 2;
+constexpr uint64_t CELL_EDGE = 3;
 
 struct JobDesc {
     // geometry

@@ -580,9 +580,10 @@ int or_march_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double 
 }
 
 /* One straight-move fast-path step from a given state (test entry): returns
- * 1 and updates the state when the fast path applies, 0 otherwise. */
+ * 1 and updates the state when the fast path applies, 0 otherwise.  Exchange
+ * draws j take u_exch[j]; cnt = crossings, reflections, draws. */
 int or_fast_step_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double o[3], const double v[3],
-                int32_t cls, int32_t cnt[2])
+                     int32_t cls, double u_exch[4], int32_t cnt[3])
 {
     or_walker_f64 w;
     memset(&w, 0, sizeof(w));
@@ -593,13 +594,15 @@ int or_fast_step_f64(const or_geom *geom, const or_derived *d, int32_t g[3], dou
     w.c = cls;
     or_dreal_f64 dr;
     or_dreal_make_f64(d, &dr);
-    int r = or_fast_f64(&w, geom, &dr, v, NULL, NULL);
+    or_fixed_draws fd = {u_exch};
+    int r = or_fast_f64(&w, geom, d, &dr, v, or_draw_fixed, &fd, NULL, NULL, NULL);
     for (int k = 0; k < 3; k++) {
         g[k] = w.g[k];
         o[k] = w.o[k];
     }
     cnt[0] = (int32_t)w.cnt[0];
     cnt[1] = (int32_t)w.cnt[1];
+    cnt[2] = (int32_t)w.cnt[4];
     return r;
 }
 

@@ -231,3 +231,42 @@ def test_fast_path_three_faces_in_order(orc):
         assert f["cnt"][0] + f["cnt"][1] == 3  # three faces met
         n_mixed += 0 < f["cnt"][1] < 3
     assert n_applied == 400 and n_mixed > 100
+
+
+def test_fast_path_exchange_faces_equal_march(orc):
+    """With two non-null classes and a permeable membrane, a step meeting
+    exchange faces takes the fast path when every draw reflects (u >= P) and
+    then equals the march given the same draws (same voxel, class, counts,
+    draws, position up to rounding); a crossing draw leaves the step to the
+    march."""
+    rng = np.random.default_rng(17)
+    lab = rng.integers(1, 3, (7, 6, 8)).astype(np.uint8)
+    lab[rng.random(lab.shape) < 0.15] = 0
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0, 1.5], T2=[np.inf] * 3, kappa=0.05)
+    dt = 2e-4
+    P = orc.derive(g, dt)["P"]
+    assert 0.0 < P[1][2] < 1.0 and 0.0 < P[2][1] < 1.0
+    n_draw = n_cross_declined = 0
+    for _ in range(3000):
+        while True:
+            gv = rng.integers(1, [6, 5, 7])  # interior voxels: no lattice edge next door
+            cls = int(lab[tuple(gv)])
+            if cls != 0:
+                break
+        o = rng.random(3)
+        u1, u2 = rng.random(2)
+        sth = 2 * np.sqrt(u1 * (1 - u1))
+        v = np.array([sth * np.cos(2 * np.pi * u2), sth * np.sin(2 * np.pi * u2), 1 - 2 * u1])
+        for u in (0.999, 0.0):
+            f = orc.fast_step_f64(g, dt, gv, o, v, cls, u_exch=(u,) * 4)
+            m = orc.march_f64(g, dt, gv, o, v, cls, u_exch=(u,) * 4)
+            if f is None:
+                n_cross_declined += u == 0.0 and m["draws"] > 0
+                continue
+            np.testing.assert_array_equal(f["g"], m["g"])
+            assert m["cls"] == cls
+            np.testing.assert_allclose(f["o"], m["o"], atol=1e-12)
+            assert list(f["cnt"]) == [m["cnt"][0], m["cnt"][1], m["draws"]]
+            assert u == 0.999 or f["cnt"][2] == 0  # a crossing draw never stays on the fast path
+            n_draw += f["cnt"][2] > 0
+    assert n_draw > 100 and n_cross_declined > 100

@@ -311,6 +311,60 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
     }
 }
 
+// Exchange faces met by a straight move (slots with CELL_OTHER in rels; the
+// other non-null class when a geometry has two, main.cu:272-289): P = 0
+// reflects without a draw, P = 1 crosses into the other class (march);
+// otherwise the k-th exchange face of the step takes draw k (the march's
+// numbering) and reflects when u >= P.  Returns false when the step needs the
+// march (which replays the same draws); otherwise every exchange face
+// reflects, ndraw counts the draws and the tangent pass adds their scores.
+template <typename R, int NP>
+__device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLocal<R> &L, const JobDesc &J,
+                                               uint32_t rels, uint64_t wid, int32_t i, Tan<R, NP> &T,
+                                               uint32_t &ndraw)
+{
+    if (!L.exch_fast) return false;
+    const int32_t pidx = w.c * NC + (3 - w.c);
+    const uint32_t cat = (L.pcat >> (2 * pidx)) & 3u;
+    if (cat == 1u) return false;
+    if (cat != 2u) return true;
+    const R P = w.c == 1 ? L.P12 : L.P21;
+    bool cross = false;
+    if constexpr (sizeof(R) == 4) {
+        const U4 blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A, wid);
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            if (((rels >> (2 * s)) & 3u) == 2u) {
+                const uint32_t word = ndraw == 0u ? blk.x : (ndraw == 1u ? blk.y : blk.z);
+                cross = cross || u01_f32(word) < P;
+                ndraw++;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            if (((rels >> (2 * s)) & 3u) == 2u) {
+                cross = cross || exch_uniform<R>(L, wid, i, (int)ndraw) < P;
+                ndraw++;
+            }
+        }
+    }
+    if (cross) return false;
+    if constexpr (NP > 0) {
+        // scores of the reflecting Bernoulli(P) draws, one add per draw
+        const double *sc4 = J.dlog1mP + 4 * pidx;
+#pragma unroll
+        for (uint32_t k = 0; k < 3u; k++) {
+            if (k < ndraw) {
+#pragma unroll
+                for (int p = 0; p < NP; p++) T.S[p] = T.S[p] + sc4[p];
+                T.Sk = T.Sk + sc4[3];
+            }
+        }
+    }
+    return true;
+}
+
 // Straight-move fast path.  The straight step ends at e = fma(s, v, o) and crosses
 // the faces of the axes where e leaves [0,1] (c_k = +-1).  The crossed faces
 // are met in increasing d_k / |v_k| (d_k = distance to the face), compared
@@ -318,11 +372,12 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
 // d_a |v_b| <= d_b |v_a|.  Met in that order, a face whose neighbour has the
 // voxel's class is crossed without a draw or a turn, one next to a null voxel
 // reflects without a draw (main.cu:274-289; P = 0 next to label 0,
-// :450-451).  The cell word holds every neighbour's relation to the voxel's
-// class, so the step is one lookup per face along the sequence and the
-// straight move folded back at the reflecting faces.  Returns false when a
-// face needs the march (a neighbour of another non-null class, or outside the
-// lattice) or when the three comparisons are not transitive (a near tie).
+// :450-451), a lattice edge is a mirror, and with EXCH an exchange face is
+// decided by exchange_faces.  The cell word holds every neighbour's relation
+// to the voxel's class, so the step is one lookup per face along the sequence
+// and the straight move folded back at the reflecting faces.  Returns false
+// when the step needs the march (a crossing into another class, or three
+// comparisons that are not transitive: a near tie).
 template <typename R, int NP, bool EXCH>
 __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t cell,
                                           const V3<R> &v, uint64_t wid, int32_t i, Tan<R, NP> &T)
@@ -358,60 +413,42 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
         rels |= rel << (2 * s);
         cur = rel == 0u ? q : cur;
     }
-    uint32_t ndraw = 0u, refl = rels;  // reflected slots, bits 2s
+    uint32_t ndraw = 0u, nedge = 0u, refl = rels;  // reflected slots, bits 2s
     if ((rels & 0x2Au) != 0u || kx == ky) {
-        // a near tie or a lattice edge (mirror) goes to the march.  Exchange
-        // faces (the other non-null class when there are two): P = 0 reflects
-        // without a draw, P = 1 crosses into the other class (march);
-        // otherwise the k-th exchange face of the step takes draw k (the
-        // march's numbering) and reflects when u >= P, a crossing draw sends
-        // the step to the march, which replays the same draws
-        if constexpr (!EXCH) return false;
-        if (kx == ky || (rels & (rels >> 1) & 0x15u) != 0u || !L.exch_fast) return false;
-        const int32_t pidx = w.c * NC + (3 - w.c);
-        const uint32_t cat = (L.pcat >> (2 * pidx)) & 3u;
-        if (cat == 1u) return false;
-        if (cat == 2u) {
-            const R P = w.c == 1 ? L.P12 : L.P21;
-            bool cross = false;
-            if constexpr (sizeof(R) == 4) {
-                const U4 blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A, wid);
-#pragma unroll
-                for (int s = 0; s < 3; s++) {
-                    if (((rels >> (2 * s)) & 3u) == 2u) {
-                        const uint32_t word = ndraw == 0u ? blk.x : (ndraw == 1u ? blk.y : blk.z);
-                        cross = cross || u01_f32(word) < P;
-                        ndraw++;
-                    }
-                }
+        // a near tie goes to the march.  A lattice edge is a mirror: the
+        // walker reflects and its origin is mirrored (main.cu:246-265,
+        // 302-310).
+        if (kx == ky) return false;
+        const uint32_t edge = rels & (rels >> 1) & 0x15u;  // CELL_EDGE slots
+        if (((rels >> 1) & ~rels & 0x15u) != 0u) {  // CELL_OTHER slots
+            if constexpr (EXCH) {
+                if (!exchange_faces<R, NP>(w, L, J, rels, wid, i, T, ndraw)) return false;
             } else {
-#pragma unroll
-                for (int s = 0; s < 3; s++) {
-                    if (((rels >> (2 * s)) & 3u) == 2u) {
-                        cross = cross || exch_uniform<R>(L, wid, i, (int)ndraw) < P;
-                        ndraw++;
-                    }
-                }
+                return false;
             }
-            if (cross) return false;
-            if constexpr (NP > 0) {
-                // scores of the reflecting Bernoulli(P) draws, one add per draw
-                const double *sc4 = J.dlog1mP + 4 * pidx;
-#pragma unroll
-                for (uint32_t k = 0; k < 3u; k++) {
-                    if (k < ndraw) {
-#pragma unroll
-                        for (int p = 0; p < NP; p++) T.S[p] = T.S[p] + sc4[p];
-                        T.Sk = T.Sk + sc4[3];
-                    }
-                }
+        }
+        if (edge != 0u) {
+            const int32_t kz = 3 - kx - ky;
+            if (((edge >> (2 * kx)) & 1u) != 0u) {
+                w.gi.x = cx < 0 ? -w.gi.x : 2 * L.nx - w.gi.x;
+                w.oi.x = -w.oi.x;
             }
+            if (((edge >> (2 * ky)) & 1u) != 0u) {
+                w.gi.y = cy < 0 ? -w.gi.y : 2 * L.ny - w.gi.y;
+                w.oi.y = -w.oi.y;
+            }
+            if (((edge >> (2 * kz)) & 1u) != 0u) {
+                w.gi.z = cz < 0 ? -w.gi.z : 2 * L.nz - w.gi.z;
+                w.oi.z = -w.oi.z;
+            }
+            set_origin_sum(w);
+            nedge = (uint32_t)__builtin_popcount(edge);
         }
         refl = (rels | (rels >> 1)) & 0x15u;
     }
     // every face met is now crossed (CELL_SAME, or axis not crossed) or
-    // reflected (CELL_NULL, or a reflecting exchange face); the ranks are a
-    // permutation of 0, 1, 2
+    // reflected (CELL_NULL, a reflecting exchange face, a mirror); the ranks
+    // are a permutation of 0, 1, 2
     const int32_t kz = 3 - kx - ky;
     const bool rx = ((refl >> (2 * kx)) & 1u) != 0u;
     const bool ry = ((refl >> (2 * ky)) & 1u) != 0u;
@@ -438,7 +475,8 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
     w.g.z += gz;
     w.lin += (gx == 0 ? 0 : (gx > 0 ? L.sx : -L.sx)) + (gy == 0 ? 0 : (gy > 0 ? L.nz : -L.nz)) + gz;
     const uint32_t nref = (uint32_t)__builtin_popcount(refl);
-    w.nrefl += nref;
+    w.nrefl += nref - nedge;
+    w.nmirror += nedge;
     w.ndraw += ndraw;
     w.ncross += (uint32_t)(__mul24(cx, cx) + __mul24(cy, cy) + __mul24(cz, cz)) - nref;
     w.tcur = w.tcur + (R)1;

@@ -135,7 +135,7 @@ def lib():
         L.or_march_f64.argtypes = [C.POINTER(_Geom), C.POINTER(_Derived)] + [C.c_void_p] * 8
         L.or_march_f64.restype = C.c_int
         L.or_fast_step_f64.argtypes = [C.POINTER(_Geom), C.POINTER(_Derived)] + [C.c_void_p] * 3 + \
-            [C.c_int32, C.c_void_p, C.c_void_p]
+            [C.c_int32] + [C.c_void_p] * 4
         L.or_fast_step_f64.restype = C.c_int
         L.or_run.argtypes = [C.POINTER(_Job), C.c_int32, C.POINTER(_Params), C.POINTER(_Out)]
         L.or_run.restype = C.c_int
@@ -283,9 +283,10 @@ def march_f64(geom: Geometry, dt, g, o, v, cls, u_exch=(0.5, 0.5, 0.5, 0.5), gi=
     return dict(g=g, o=o, v=v, cls=int(c[0]), cnt=cnt, draws=nd, gi=gi, oi=oi)
 
 
-def fast_step_f64(geom: Geometry, dt, g, o, v, cls, u_exch=(0.5, 0.5, 0.5, 0.5)):
+def fast_step_f64(geom: Geometry, dt, g, o, v, cls, u_exch=(0.5, 0.5, 0.5, 0.5), gi=None, oi=None):
     """One straight-move fast-path step; None when the march is needed.
-    cnt = crossings, reflections, exchange draws (draw j is u_exch[j])."""
+    cnt = crossings, reflections, mirrors, exchange draws (draw j is
+    u_exch[j]); gi, oi = the (mirror-unfolded) origin, default the start."""
     gc = geom._c()
     d = _Derived()
     if not lib().or_derive(C.byref(gc), dt, C.byref(d)):
@@ -294,9 +295,12 @@ def fast_step_f64(geom: Geometry, dt, g, o, v, cls, u_exch=(0.5, 0.5, 0.5, 0.5))
     o = np.array(o, np.float64)
     v = np.array(v, np.float64)
     u = np.array(u_exch, np.float64)
-    cnt = np.zeros(3, np.int32)
-    ok = lib().or_fast_step_f64(C.byref(gc), C.byref(d), _ptr(g), _ptr(o), _ptr(v), int(cls), _ptr(u), _ptr(cnt))
-    return dict(g=g, o=o, cnt=cnt) if ok else None
+    cnt = np.zeros(4, np.int32)
+    gi = np.array(g if gi is None else gi, np.int32)
+    oi = np.array(o if oi is None else oi, np.float64)
+    ok = lib().or_fast_step_f64(C.byref(gc), C.byref(d), _ptr(g), _ptr(o), _ptr(v), int(cls), _ptr(u), _ptr(cnt),
+                                _ptr(gi), _ptr(oi))
+    return dict(g=g, o=o, cnt=cnt, gi=gi, oi=oi) if ok else None
 
 
 # ---------------------------------------------------------------- simulation

@@ -580,16 +580,19 @@ int or_march_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double 
 }
 
 /* One straight-move fast-path step from a given state (test entry): returns
- * 1 and updates the state when the fast path applies, 0 otherwise.  Exchange
- * draws j take u_exch[j]; cnt = crossings, reflections, draws. */
+ * 1 and updates the state (voxel, offset, origin) when the fast path
+ * applies, 0 otherwise.  Exchange draws j take u_exch[j]; cnt = crossings,
+ * reflections, mirrors, draws. */
 int or_fast_step_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double o[3], const double v[3],
-                     int32_t cls, double u_exch[4], int32_t cnt[3])
+                     int32_t cls, double u_exch[4], int32_t cnt[4], int32_t gi[3], double oi[3])
 {
     or_walker_f64 w;
     memset(&w, 0, sizeof(w));
     for (int k = 0; k < 3; k++) {
         w.g[k] = g[k];
         w.o[k] = o[k];
+        w.gi[k] = gi[k];
+        w.oi[k] = oi[k];
     }
     w.c = cls;
     or_dreal_f64 dr;
@@ -599,10 +602,13 @@ int or_fast_step_f64(const or_geom *geom, const or_derived *d, int32_t g[3], dou
     for (int k = 0; k < 3; k++) {
         g[k] = w.g[k];
         o[k] = w.o[k];
+        gi[k] = w.gi[k];
+        oi[k] = w.oi[k];
     }
     cnt[0] = (int32_t)w.cnt[0];
     cnt[1] = (int32_t)w.cnt[1];
-    cnt[2] = (int32_t)w.cnt[4];
+    cnt[2] = (int32_t)w.cnt[2];
+    cnt[3] = (int32_t)w.cnt[4];
     return r;
 }
 

@@ -114,11 +114,11 @@ int or_march_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double 
                  int32_t gi[3], double oi[3]);
 
 /* ---- one straight-move fast-path step (for the fast-path vs march tests):
- * returns 1 (state updated, cnt = {crossings, reflections, draws}) when the
- * step is taken without the march, 0 otherwise; exchange draw j is
+ * returns 1 (state updated, cnt = {crossings, reflections, mirrors, draws})
+ * when the step is taken without the march, 0 otherwise; exchange draw j is
  * u_exch[j]. ---- */
 int or_fast_step_f64(const or_geom *geom, const or_derived *d, int32_t g[3], double o[3], const double v[3],
-                     int32_t cls, double u_exch[4], int32_t cnt[3]);
+                     int32_t cls, double u_exch[4], int32_t cnt[4], int32_t gi[3], double oi[3]);
 
 /* ---- full simulation of a list of jobs ---- */
 int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs);

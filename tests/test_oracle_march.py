@@ -166,15 +166,15 @@ def test_fast_path_equals_march(orc):
     """The straight-move fast path against the face march on random states in
     a lattice with null voxels, a second class and lattice edges: whenever the
     fast path applies it ends in the same voxel and class with the same
-    crossing/reflection counts, at the same position up to rounding; and it
-    applies to most steps away from boundaries."""
+    crossing/reflection/mirror counts and the same mirrored origin, at the
+    same position up to rounding; and it applies to most steps."""
     rng = np.random.default_rng(11)
     lab = rng.integers(0, 3, (7, 6, 8)).astype(np.uint8)
     lab[2:5, 2:5, 2:6] = 1
     g = orc.Geometry(lab, 0.1, [0.0, 2.0, 2.0], T2=[np.inf] * 3, kappa=0.0)  # classes 1, 2 impermeable
     dt = 2e-4
     step = orc.derive(g, dt)["step"][1]
-    applied = 0
+    applied = mirrored = 0
     n = 4000
     for _ in range(n):
         while True:
@@ -195,9 +195,11 @@ def test_fast_path_equals_march(orc):
         np.testing.assert_array_equal(f["g"], m["g"])
         assert m["cls"] == cls
         np.testing.assert_allclose(f["o"], m["o"], atol=1e-12)
-        assert f["cnt"][0] == m["cnt"][0] and f["cnt"][1] == m["cnt"][1]
-        assert m["cnt"][2] == 0  # no lattice-edge mirror on a fast step
-    assert applied > 0.3 * n
+        np.testing.assert_array_equal(f["cnt"][:3], m["cnt"][:3])  # crossings, reflections, mirrors
+        np.testing.assert_array_equal(f["gi"], m["gi"])
+        np.testing.assert_array_equal(f["oi"], m["oi"])
+        mirrored += f["cnt"][2] > 0
+    assert applied > 0.3 * n and mirrored > 50
     # pure interior: always applies and is the straight move
     f = orc.fast_step_f64(g, dt, [3, 3, 3], [0.5, 0.5, 0.5], [0.6, 0.0, 0.8], 1)
     np.testing.assert_allclose(f["o"], [0.5 + step * 0.6, 0.5, 0.5 + step * 0.8], atol=1e-15)
@@ -266,7 +268,7 @@ def test_fast_path_exchange_faces_equal_march(orc):
             np.testing.assert_array_equal(f["g"], m["g"])
             assert m["cls"] == cls
             np.testing.assert_allclose(f["o"], m["o"], atol=1e-12)
-            assert list(f["cnt"]) == [m["cnt"][0], m["cnt"][1], m["draws"]]
-            assert u == 0.999 or f["cnt"][2] == 0  # a crossing draw never stays on the fast path
-            n_draw += f["cnt"][2] > 0
+            assert list(f["cnt"]) == [m["cnt"][0], m["cnt"][1], m["cnt"][2], m["draws"]]
+            assert u == 0.999 or f["cnt"][3] == 0  # a crossing draw never stays on the fast path
+            n_draw += f["cnt"][3] > 0
     assert n_draw > 100 and n_cross_declined > 100

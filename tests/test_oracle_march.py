@@ -198,8 +198,27 @@ def test_fast_path_equals_march(orc):
         np.testing.assert_array_equal(f["cnt"][:3], m["cnt"][:3])  # crossings, reflections, mirrors
         np.testing.assert_array_equal(f["gi"], m["gi"])
         np.testing.assert_array_equal(f["oi"], m["oi"])
+    assert applied > 0.3 * n
+    # lattice edges of a voxel whose other neighbours are all outside: mirrors
+    # on the fast path, same voxel, counts and mirrored origin as the march
+    g1 = orc.Geometry(np.ones((4, 5, 6), np.uint8), 0.1, [0.0, 2.0], T2=[np.inf] * 2)
+    for _ in range(2000):
+        gv = rng.integers(0, [4, 5, 6])
+        o = rng.random(3)
+        u1, u2 = rng.random(2)
+        sth = 2 * np.sqrt(u1 * (1 - u1))
+        v = np.array([sth * np.cos(2 * np.pi * u2), sth * np.sin(2 * np.pi * u2), 1 - 2 * u1])
+        gi, oi = rng.integers(-3, 9, 3), rng.random(3)
+        f = orc.fast_step_f64(g1, dt, gv, o, v, 1, gi=gi, oi=oi)
+        m = orc.march_f64(g1, dt, gv, o, v, 1, gi=gi, oi=oi)
+        assert f is not None
+        np.testing.assert_array_equal(f["g"], m["g"])
+        np.testing.assert_allclose(f["o"], m["o"], atol=1e-12)
+        np.testing.assert_array_equal(f["cnt"][:3], m["cnt"][:3])
+        np.testing.assert_array_equal(f["gi"], m["gi"])
+        np.testing.assert_array_equal(f["oi"], m["oi"])
         mirrored += f["cnt"][2] > 0
-    assert applied > 0.3 * n and mirrored > 50
+    assert mirrored > 300
     # pure interior: always applies and is the straight move
     f = orc.fast_step_f64(g, dt, [3, 3, 3], [0.5, 0.5, 0.5], [0.6, 0.0, 0.8], 1)
     np.testing.assert_allclose(f["o"], [0.5 + step * 0.6, 0.5, 0.5 + step * 0.8], atol=1e-15)
@@ -242,8 +261,7 @@ def test_fast_path_exchange_faces_equal_march(orc):
     draws, position up to rounding); a crossing draw leaves the step to the
     march."""
     rng = np.random.default_rng(17)
-    lab = rng.integers(1, 3, (7, 6, 8)).astype(np.uint8)
-    lab[rng.random(lab.shape) < 0.15] = 0
+    lab = rng.integers(1, 3, (7, 6, 8)).astype(np.uint8)  # two classes, no null voxel
     g = orc.Geometry(lab, 0.1, [0.0, 2.0, 1.5], T2=[np.inf] * 3, kappa=0.05)
     dt = 2e-4
     P = orc.derive(g, dt)["P"]

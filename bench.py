@@ -24,6 +24,10 @@ batch-means replicas:
             0.01 um/ms, 1e6 walkers, 5e4 steps, 20 records, narrow-pulse
             b-table of 64 directions x b in {0.5,1,2,3} ms/um^2: forward pass
             plus the adjoint pass of S(b, g, t) and D(t), K(t) along z.
+On request (--forms sweep, not in the default run) the parameter sweep: 4096
+axons with extra-axonal space in 70x70x1000 uint16 lattices (caliber CV,
+undulation and kappa drawn per geometry, ~20e9 voxels, ~180 GB of cell words
+and classes in HBM), 4e7 walkers, 2e4 steps, batched forward plus adjoint.
 """
 from __future__ import annotations
 
@@ -86,11 +90,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay the run as a captured hipGraph")
     ap.add_argument("--forms", default="plain,batched,gradient",
-                    help="comma list of forms to time beside the plain form")
+                    help="comma list of forms to time beside the plain form (also: sweep)")
     ap.add_argument("--batched-walkers", type=int, default=10_000_000)
     ap.add_argument("--batched-tn", type=int, default=50_000)
     ap.add_argument("--grad-walkers", type=int, default=1_000_000)
     ap.add_argument("--grad-tn", type=int, default=50_000)
+    ap.add_argument("--sweep-geoms", type=int, default=4096)
+    ap.add_argument("--sweep-walkers", type=int, default=40_000_000)
+    ap.add_argument("--sweep-tn", type=int, default=20_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -188,6 +195,8 @@ def main():
         line["forms"]["batched"] = form_batched(rms, eng, args, dt, grad)
     if "gradient" in forms:
         line["forms"]["gradient"] = form_gradient(rms, eng, args, dt)
+    if "sweep" in forms:
+        line["forms"]["sweep"] = form_sweep(rms, eng, args, dt)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(lab, res, D0, dt, grad, args)
     print(json.dumps(line), flush=True)
@@ -345,6 +354,77 @@ def form_gradient(rms, eng, args, dt):
         "D_end": float(est[0]), "D_end_se": float(ses[0]), "K_end": float(est[1]), "K_end_se": float(ses[1]),
         "S_end_b1_first": float(est[2]), "S_end_b1_first_se": float(ses[2]),
         "S_end_b2_first": float(est[3]), "S_end_b2_first_se": float(ses[3]),
+    }
+
+
+def form_sweep(rms, eng, args, dt):
+    """Parameter sweep: many axons with extra-axonal space, caliber CV in
+    [0, 0.4], undulation amplitude in [0, 1] um and kappa in [0, 0.03] um/ms
+    drawn per geometry, 70x70x1000 uint16 lattices, all in one batched forward
+    launch plus its adjoint (D(t), K(t) along z and narrow-pulse S per
+    geometry, 20 records, 16 directions x b in {1, 2} ms/um^2)."""
+    import torch
+    from rms_amd import analysis as A
+    from rms_amd import geometry as G
+
+    ng = args.sweep_geoms
+    rng = np.random.default_rng(1238)
+    cv, und, kap = rng.uniform(0.0, 0.4, ng), rng.uniform(0.0, 1.0, ng), rng.uniform(0.0, 0.03, ng)
+    n = 16
+    i = np.arange(n) + 0.5
+    phi = np.arccos(1 - 2 * i / n)
+    th = np.pi * (1 + 5 ** 0.5) * i
+    dirs = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
+    bvals, bvecs = np.repeat([1.0, 2.0], n), np.tile(dirs, (2, 1))
+    nb = bvals.size
+    nw = max(64, args.sweep_walkers // ng)
+    TN, T = args.sweep_tn, 20
+    jobs, voxels = [], 0
+    t0 = time.perf_counter()
+    for k in range(ng):
+        lab = G.axon(n=70, nz=1000, res=0.1, r0=0.8, cv=float(cv[k]), undulation=float(und[k]), seed=1238 + k,
+                     ecs=True, dtype=np.uint16)
+        geom = rms.Geometry(lab, 0.1, [0.0, 2.0, 1.5], T2=[math.inf, 80.0, 60.0], kappa=float(kap[k]))
+        eng.add_geometry(geom)
+        voxels += lab.size
+        jobs.append(dict(geom=geom, nwalk=nw, seed=0x5EED0004 | (k << 32), bval=bvals, bvec=bvecs))
+        if (k + 1) % 256 == 0:
+            print(f"sweep: {k + 1}/{ng} geometries uploaded ({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
+                  flush=True)
+    setup_s = time.perf_counter() - t0
+    TD = A.diff_times(dt, TN, T)
+    i_re = 1 + 3 + 21
+    plan = eng.plan(jobs, dt, TN, timepoints=T, precision=args.precision, replicas=32)
+    t0 = time.perf_counter()
+    outs = plan.execute(want_replicas=False)
+    cots = []
+    for o in outs:
+        rec = o["rec"]
+        cot = A.akc_cotangent(rec, 3, TD, [0.0, 0.0, 1.0])
+        cot[:, i_re:i_re + nb] += (1.0 / rec[:, 0])[:, None]
+        cot[:, 0] += -rec[:, i_re:i_re + nb].sum(1) / rec[:, 0] ** 2
+        cots.append(cot)
+    adj = plan.adjoint(cots, None)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    fwd_ms, adj_ms = plan.timing["walk_ms"], plan.adj_timing["walk_ms"]
+    plan.close()
+    dk = np.array([a["grad"][8] for a in adj])
+    dT2 = np.array([a["grad"][5] for a in adj])
+    Dend = []
+    for o in outs:
+        s = A.split(o["rec"], 3, nb)
+        K, D = A.akc_mom(s["sig0"], s["dx2"], s["dx4"], TD, [0.0, 0.0, 1.0])
+        Dend.append(D[-1, 0])
+    total = float(nw) * ng * TN
+    return {
+        "walker_steps_per_s": total / el, "forward_walker_steps_per_s": total / (fwd_ms / 1e3),
+        "forward_kernel_ms": float(fwd_ms), "adjoint_kernel_ms": float(adj_ms),
+        "geometries": ng, "walkers": int(nw * ng), "TN": TN, "records": T, "bvecs": int(nb),
+        "voxels": int(voxels), "label_bytes": 2, "lattice_bytes_in_hbm": int(voxels * 9),
+        "setup_s": setup_s,
+        "D_end_median": float(np.median(Dend)), "D_end_range": [float(np.min(Dend)), float(np.max(Dend))],
+        "dobj_dkappa_median": float(np.median(dk)), "dobj_dT2_intra_median": float(np.median(dT2)),
     }
 
 

@@ -139,19 +139,22 @@ __device__ __forceinline__ void sincos_turns(float t, float &s, float &c)
 }
 
 // sin(2*pi*t), cos(2*pi*t) for t = u01_f32(w) = m * 2^-24, m = (w >> 8) | 1,
-// with the reduction done on the integer m: the quarter-turn index
-// k = floor(4t + 1/2) = (m + 2^21) >> 22 and t - k/4 = (m - k*2^22) * 2^-24 are
-// the exact values the float reduction above computes, so the result is the
-// same bit for bit.
-__device__ __forceinline__ void sincos_turns_word(uint32_t w, float &s, float &c)
+// from a table of 256 angles: k = round(256 t) (as the integer (m + 2^15) >>
+// 16), x = 2*pi*(t - k/256) = 2*pi*(m - k*2^16)*2^-24 with |x| <= pi/256, and
+// the angle sum with cos x = 1 - x^2/2, sin x = x - x^3/6 (truncation below
+// 1e-9).  tab = {sin, cos} of 2*pi*k/256 (rms_turn_table.h, staged in LDS).
+__device__ __forceinline__ void sincos_turns_table(uint32_t w, const float2 *tab, float &s, float &c)
 {
     const uint32_t m = (w >> 8) | 1u;
-    const uint32_t k = (m + (1u << 21)) >> 22;
-    const int32_t num = (int32_t)m - (int32_t)(k << 22);
+    const uint32_t k = (m + (1u << 15)) >> 16;
+    const int32_t num = (int32_t)m - (int32_t)(k << 16);
     const float x = (float)num * (6.2831854820251465f * 5.9604644775390625e-08f);
-    float sx, cx;
-    sc_poly(x, sx, cx);
-    quadrant<float>((int)k, sx, cx, s, c);
+    const float2 a = tab[k & 255u];
+    const float x2 = x * x;
+    const float cr = __builtin_fmaf(x2, -0.5f, 1.0f);
+    const float sr = __builtin_fmaf(x * x2, -0.16666667163372040f, x);
+    s = __builtin_fmaf(a.x, cr, a.y * sr);
+    c = __builtin_fmaf(a.y, cr, -(a.x * sr));
 }
 
 // Correctly rounded square root of a normal x far from the ends of the float

@@ -26,6 +26,7 @@
 
 #include "rms_device.h"
 #include "rms_internal.h"
+#include "rms_turn_table.h"
 #include "rms_types.h"
 
 namespace rms {
@@ -276,10 +277,11 @@ __device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, in
 }
 
 // Direction of step i, main.cu:140-151.  f32 layout: one Philox block serves
-// steps 2m (words x, y) and 2m+1 (words z, w).
+// steps 2m (words x, y) and 2m+1 (words z, w); the azimuth's sin/cos come from
+// the 256-angle table tab.
 template <typename R>
 __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, int64_t i, bool fresh_block,
-                                          uint32_t &b0, uint32_t &b1)
+                                          uint32_t &b0, uint32_t &b1, const float2 *tab)
 {
     R u1, u2;
     if constexpr (sizeof(R) == 4) {
@@ -297,7 +299,7 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
         const R cth = (R)1 - (R)2 * u1;
         const R sth = (R)2 * sqrt_rn_mid(u1 * ((R)1 - u1));
         R sp, cp;
-        sincos_turns_word(w1, sp, cp);  // = sincos_turns(u01_f32(w1))
+        sincos_turns_table(w1, tab, sp, cp);
         return V3<R>{sth * cp, sth * sp, cth};
     } else {
         U4 r = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_DIR, wid);
@@ -831,6 +833,10 @@ template <typename R, int SIG, int NP>
 __global__ __launch_bounds__(256, (NP == 0 || (sizeof(R) == 4 && (SIG & 2) == 0)) ? 3 : 2)
 void walk_kernel(LaunchArgs a)
 {
+    // sin/cos table of the FP32 direction draw, staged in LDS
+    __shared__ float2 tab[256];
+    for (int k = (int)threadIdx.x; k < 256; k += (int)blockDim.x) tab[k] = turn_table[k];
+    __syncthreads();
     const int lane = (int)(threadIdx.x & 63u);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int wpb = (int)(blockDim.x >> 6);
@@ -910,7 +916,7 @@ void walk_kernel(LaunchArgs a)
 
         for (int32_t i = step0; i < step1; i++) {
             const bool fresh = ((i & 1) == 0) || (i == step0);
-            const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1);
+            const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1, tab);
             const int32_t lin0 = w.lin;
             if (!fast_step<R, NP, (SIG & 4) != 0>(w, L, J, cell, v, wid, i, T)) march<R, NP>(w, L, J, wid, i, v, T);
             if (w.lin != lin0) cell = L.cells[w.lin];  // only lanes that changed voxel read

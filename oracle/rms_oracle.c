@@ -151,17 +151,43 @@ void or_sincos_turns_f64(double t, double *s, double *c)
     quadrant((int)k, sx, cx, s, c);
 }
 
+/* sin and cos of 2*pi*k/256, k = 0..255, as float32 roundings of the C
+ * library's double sin/cos (the engine's table, rms_turn_table.h, holds the
+ * same values). */
+static float turn_sin[256], turn_cos[256];
+
+__attribute__((constructor)) static void or_turn_table_init(void)
+{
+    for (int k = 0; k < 256; k++) {
+        const double a = 2 * 3.141592653589793 * k / 256;
+        turn_sin[k] = (float)sin(a);
+        turn_cos[k] = (float)cos(a);
+    }
+}
+
+void or_turn_table(float s[256], float c[256])
+{
+    for (int k = 0; k < 256; k++) {
+        s[k] = turn_sin[k];
+        c[k] = turn_cos[k];
+    }
+}
+
+/* sin(2 pi t), cos(2 pi t) for t = m * 2^-24 (FP32 direction draw): k =
+ * round(256 t), x = 2 pi (t - k/256), |x| <= pi/256, and the angle sum with
+ * cos x = 1 - x^2/2, sin x = x - x^3/6. */
 void or_sincos_turns_f32(float t, float *s, float *c)
 {
-    float k = floorf(4.0f * t + 0.5f);
-    float r = t - k * 0.25f;
-    float x = r * 6.2831854820251465f;
-    float sx, cx;
-    sc_poly_f32(x, &sx, &cx);
-    double so, co;
-    quadrant((int)k, (double)sx, (double)cx, &so, &co);
-    *s = (float)so;
-    *c = (float)co;
+    const uint32_t m = (uint32_t)(t * 16777216.0f);
+    const uint32_t k = (m + (1u << 15)) >> 16;
+    const int32_t num = (int32_t)m - (int32_t)(k << 16);
+    const float x = (float)num * (6.2831854820251465f * 5.9604644775390625e-08f);
+    const float as = turn_sin[k & 255u], ac = turn_cos[k & 255u];
+    const float x2 = x * x;
+    const float cr = fmaf(x2, -0.5f, 1.0f);
+    const float sr = fmaf(x * x2, -0.16666667163372040f, x);
+    *s = fmaf(as, cr, ac * sr);
+    *c = fmaf(ac, cr, -(as * sr));
 }
 
 /* exp(x) for x <= 0 (T2 weights). */

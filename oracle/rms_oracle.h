@@ -93,6 +93,7 @@ double or_u01_f64(uint32_t a, uint32_t b);
 float or_u01_f32(uint32_t w);
 void or_sincos_turns_f64(double t, double *s, double *c);
 void or_sincos_turns_f32(float t, float *s, float *c);
+void or_turn_table(float s[256], float c[256]);
 double or_exp_f64(double x);
 void or_sincos_f64(double x, double *s, double *c);
 double or_cos_f64(double x);

@@ -82,14 +82,39 @@ def test_sincos_turns_f64_accuracy(orc):
 
 
 def test_sincos_turns_f32_accuracy(orc):
-    t = np.linspace(1e-6, 1 - 1e-6, 5001).astype(np.float32)
+    """The FP32 direction's sin/cos (256-angle table + angle sum) on the
+    draw's grid t = (2k+1) 2^-24 against long double: absolute error below
+    2.5 ulp of 1, unit length within 3e-7."""
+    rng = np.random.default_rng(3)
+    m = np.concatenate([[1, (1 << 24) - 1, 1 << 23 | 1, (1 << 22) + 1, (3 << 22) + 1],
+                        2 * rng.integers(0, 1 << 23, 6000) + 1, 2 * np.arange(0, 1 << 23, 1 << 13) + 1])
+    t = (m * 2.0 ** -24).astype(np.float32)
     s, c = zip(*[orc.sincos_turns(float(x), "f32") for x in t])
-    rs, rc = _turns_ref(t, np.float32)
-    assert _ulp_err(s, rs, np.float32) < 4
-    assert _ulp_err(c, rc, np.float32) < 4
-    # unit vector to within 2 ulp
+    tl = m.astype(np.longdouble) * np.longdouble(2.0) ** -24
+    rs, rc = np.sin(2 * PI_LD * tl), np.cos(2 * PI_LD * tl)
+    assert np.max(np.abs(np.asarray(s, np.longdouble) - rs)) < 2.5 * 2.0 ** -24
+    assert np.max(np.abs(np.asarray(c, np.longdouble) - rc)) < 2.5 * 2.0 ** -24
     n = np.asarray(s, np.float64) ** 2 + np.asarray(c, np.float64) ** 2
     assert np.max(np.abs(n - 1)) < 3e-7
+
+
+def test_turn_table_matches_engine_header(orc):
+    """The oracle's 256-angle table equals the engine's (rms_turn_table.h), and
+    both are the float32 roundings of double sin/cos(2 pi k / 256)."""
+    import ctypes as C
+    import math
+    import os
+    import re
+    ts, tc = (C.c_float * 256)(), (C.c_float * 256)()
+    orc.lib().or_turn_table(ts, tc)
+    hdr = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "monte-carlo-simulation-3d-rms_amd", "csrc", "rms_turn_table.h")
+    pairs = re.findall(r"\{(-?0x[0-9a-fp.+-]+)f, (-?0x[0-9a-fp.+-]+)f\}", open(hdr).read())
+    assert len(pairs) == 256
+    for k, (hs, hc) in enumerate(pairs):
+        a = 2 * math.pi * k / 256
+        assert float.fromhex(hs) == ts[k] == float(np.float32(math.sin(a)))
+        assert float.fromhex(hc) == tc[k] == float(np.float32(math.cos(a)))
 
 
 def test_exp_accuracy(orc):

@@ -368,7 +368,7 @@ __device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLoca
 }
 
 // Straight-move fast path.  The straight step ends at e = fma(s, v, o) and crosses
-// the faces of the axes where e leaves [0,1] (c_k = +-1).  The crossed faces
+// the faces of the axes where floor(e) = c_k is +-1 (e outside [0,1)).  The crossed faces
 // are met in increasing d_k / |v_k| (d_k = distance to the face), compared
 // without a division: face a comes before face b (a < b) when
 // d_a |v_b| <= d_b |v_a|.  Met in that order, a face whose neighbour has the
@@ -388,9 +388,9 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
     const R ex = fma(sc, v.x, w.o.x);
     const R ey = fma(sc, v.y, w.o.y);
     const R ez = fma(sc, v.z, w.o.z);
-    const int32_t cx = ex > (R)1 ? 1 : (ex < (R)0 ? -1 : 0);
-    const int32_t cy = ey > (R)1 ? 1 : (ey < (R)0 ? -1 : 0);
-    const int32_t cz = ez > (R)1 ? 1 : (ez < (R)0 ? -1 : 0);
+    // e is in (-1, 2): the face crossed along an axis is floor(e) in {-1, 0, 1}
+    const R fx = floor(ex), fy = floor(ey), fz = floor(ez);
+    const int32_t cx = (int32_t)fx, cy = (int32_t)fy, cz = (int32_t)fz;
     const R dx = cx > 0 ? (R)1 - w.o.x : w.o.x;
     const R dy = cy > 0 ? (R)1 - w.o.y : w.o.y;
     const R dz = cz > 0 ? (R)1 - w.o.z : w.o.z;
@@ -469,9 +469,11 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
         }
     }
     const int32_t gx = rx ? 0 : cx, gy = ry ? 0 : cy, gz = rz ? 0 : cz;
-    w.o.x = rx ? (cx > 0 ? (R)2 - ex : -ex) : ex - (R)cx;
-    w.o.y = ry ? (cy > 0 ? (R)2 - ey : -ey) : ey - (R)cy;
-    w.o.z = rz ? (cz > 0 ? (R)2 - ez : -ez) : ez - (R)cz;
+    // y = e - c is the offset past the face; a reflection folds it back to 1 - y
+    const R px = ex - fx, py = ey - fy, pz = ez - fz;
+    w.o.x = rx ? (R)1 - px : px;
+    w.o.y = ry ? (R)1 - py : py;
+    w.o.z = rz ? (R)1 - pz : pz;
     w.g.x += gx;
     w.g.y += gy;
     w.g.z += gz;

@@ -538,6 +538,7 @@ static inline double or_pgse_phase(const or_jobaux *aux, const double *gr, int j
 #define FABS fabs
 #define SQRT sqrt
 #define FMA fma
+#define FLOOR floor
 #define SINCOS_TURNS or_sincos_turns_f64
 #define PIREF 0.9999999988573335
 #define REAL double
@@ -547,6 +548,7 @@ static inline double or_pgse_phase(const or_jobaux *aux, const double *gr, int j
 #undef FABS
 #undef SQRT
 #undef FMA
+#undef FLOOR
 #undef SINCOS_TURNS
 #undef PIREF
 #undef REAL
@@ -556,6 +558,7 @@ static inline double or_pgse_phase(const or_jobaux *aux, const double *gr, int j
 #define FABS fabsf
 #define SQRT sqrtf
 #define FMA fmaf
+#define FLOOR floorf
 #define SINCOS_TURNS or_sincos_turns_f32
 #define PIREF 1.0f
 #define REAL float
@@ -565,6 +568,7 @@ static inline double or_pgse_phase(const or_jobaux *aux, const double *gr, int j
 #undef FABS
 #undef SQRT
 #undef FMA
+#undef FLOOR
 #undef SINCOS_TURNS
 #undef PIREF
 #undef REAL

@@ -141,12 +141,12 @@ extern "C" rms_status rms_create(const rms_device_opts *opts, rms_ctx **out)
 extern "C" rms_status rms_destroy(rms_ctx *ctx)
 {
     if (!ctx) return RMS_ERR_INVALID_ARG;
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     for (auto &g : ctx->geoms) {
-        if (g.d_cells) hipFree(g.d_cells);
-        if (g.d_cls) hipFree(g.d_cls);
+        if (g.d_cells) (void)hipFree(g.d_cells);
+        if (g.d_cls) (void)hipFree(g.d_cls);
     }
-    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RMS_OK;
 }
@@ -317,11 +317,11 @@ extern "C" rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, i
     if (e == hipSuccess)
         e = launch_build_cells(d_lab, g.label_bytes, d_map, g.nx, g.ny, g.nz, gd.d_cells, gd.d_cls, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (d_lab) hipFree(d_lab);
-    if (d_map) hipFree(d_map);
+    if (d_lab) (void)hipFree(d_lab);
+    if (d_map) (void)hipFree(d_map);
     if (e != hipSuccess) {
-        if (gd.d_cells) hipFree(gd.d_cells);
-        if (gd.d_cls) hipFree(gd.d_cls);
+        if (gd.d_cells) (void)hipFree(gd.d_cells);
+        if (gd.d_cls) (void)hipFree(gd.d_cls);
         ctx->err = std::string("geometry upload: ") + hipGetErrorString(e);
         return e == hipErrorOutOfMemory ? RMS_ERR_OUT_OF_MEMORY : RMS_ERR_HIP;
     }
@@ -389,12 +389,12 @@ static int64_t first_true_ge(double dt, int64_t TN, double thr)
 static void plan_free(rms_plan *p)
 {
     if (!p) return;
-    hipSetDevice(p->ctx->device);
-    if (p->gexec) hipGraphExecDestroy(p->gexec);
-    if (p->graph) hipGraphDestroy(p->graph);
+    (void)hipSetDevice(p->ctx->device);
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
     for (auto &e : p->ev)
-        if (e) hipEventDestroy(e);
-    for (void *q : p->allocs) hipFree(q);
+        if (e) (void)hipEventDestroy(e);
+    for (void *q : p->allocs) (void)hipFree(q);
     delete p;
 }
 
@@ -759,8 +759,8 @@ extern "C" rms_status rms_plan_execute(rms_plan *p, rms_job_out *outs, rms_timin
         if ((st = palloc(p, &p->fin.cnt, sizeof(uint32_t) * 5 * ws)) != RMS_OK) return st;
         p->fin_alloc = true;
         if (p->gexec) {  // the captured graph does not write the final state
-            hipGraphExecDestroy(p->gexec);
-            hipGraphDestroy(p->graph);
+            (void)hipGraphExecDestroy(p->gexec);
+            (void)hipGraphDestroy(p->graph);
             p->gexec = nullptr;
             p->graph = nullptr;
         }
@@ -773,7 +773,7 @@ extern "C" rms_status rms_plan_execute(rms_plan *p, rms_job_out *outs, rms_timin
             hipGraph_t g = nullptr;
             hipError_t e = hipStreamEndCapture(s, &g);
             if (st != RMS_OK) {
-                if (g) hipGraphDestroy(g);
+                if (g) (void)hipGraphDestroy(g);
                 return st;
             }
             HIPCHK(ctx, e);
@@ -791,12 +791,12 @@ extern "C" rms_status rms_plan_execute(rms_plan *p, rms_job_out *outs, rms_timin
     if (timing) {
         std::memset(timing, 0, sizeof(*timing));
         float ms = 0.f;
-        hipEventElapsedTime(&ms, p->ev[0], p->ev[2]);
+        HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev[0], p->ev[2]));
         timing->total_ms = ms;
         if (!p->prm.use_graph) {
-            hipEventElapsedTime(&ms, p->ev[0], p->ev[1]);
+            HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
             timing->walk_ms = ms;
-            hipEventElapsedTime(&ms, p->ev[1], p->ev[2]);
+            HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev[1], p->ev[2]));
             timing->reduce_ms = ms;
         } else {
             timing->walk_ms = timing->total_ms;
@@ -944,11 +944,11 @@ extern "C" rms_status rms_plan_adjoint(rms_plan *p, const rms_job_adj *adj, rms_
     if (timing) {
         std::memset(timing, 0, sizeof(*timing));
         float ms = 0.f;
-        hipEventElapsedTime(&ms, p->ev[0], p->ev[2]);
+        HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev[0], p->ev[2]));
         timing->total_ms = ms;
-        hipEventElapsedTime(&ms, p->ev[0], p->ev[1]);
+        HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
         timing->walk_ms = ms;
-        hipEventElapsedTime(&ms, p->ev[1], p->ev[2]);
+        HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev[1], p->ev[2]));
         timing->reduce_ms = ms;
         timing->grid = p->adj_grid;
         timing->block = p->block;

@@ -30,7 +30,7 @@ ARCH = os.environ.get("RMS_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-Wall", "-Wno-unused-function", "-Wno-unused-result", "-Wno-unused-value",
+    "-Wall", "-Wno-unused-function",
 ]
 
 

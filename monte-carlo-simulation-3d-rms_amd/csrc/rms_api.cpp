@@ -506,8 +506,11 @@ extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t
             for (int b = 1; b < G.meta.nclass; b++)
                 if (a != b && Pcat[a * NC + b] != 0) H.exchange = true;
         // the walk kernel with faces between the two non-null classes on the
-        // fast path (reflect at P = 0, draw at 0 < P < 1)
-        if (G.meta.nclass == 3) p->sig |= 4;
+        // fast path (reflect at P = 0, draw at 0 < P < 1); it is also the only
+        // variant whose march compiles exchange draws (category 2)
+        bool draws = false;
+        for (int i = 0; i < NC * NC; i++) draws = draws || Pcat[i] == 2;
+        if (G.meta.nclass == 3 || draws) p->sig |= 4;
         H.geom_id = J.geom_id;
         H.nwalk = J.nwalk;
         H.nbatch = (J.nwalk + WAVE - 1) / WAVE;

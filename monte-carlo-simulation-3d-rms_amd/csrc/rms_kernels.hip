@@ -259,16 +259,23 @@ __device__ __forceinline__ void init_walker(Walker<R> &w, const JobLocal<R> &L, 
     set_origin_sum(w);
 }
 
-// Exchange draw j (0..3) of step i: f32 layout takes word j of block (i, 1),
-// f64 layout half j&1 of block (i, 1 + j/2).  The block is recomputed for
-// every draw (draws happen only at probabilistic membranes), which keeps no
-// block in registers across the march.
+// Exchange draw j (0..3) of step i.  f32 layout: draw 0 is u01_f32 of the
+// word (xhi << 16) | (x >> 16), where xhi holds the 16 low bits of the step's
+// direction words that the direction does not use (draw_dir) and x is word x
+// of block (i, 1); draws 1..3 take words y, z, w of that block.  The high half
+// alone settles u < P except when P falls inside its 2^-16 interval, so the
+// fast path computes the block only then or for a second draw.  f64 layout:
+// half j&1 of block (i, 1 + j/2).  The block is recomputed for every draw
+// (draws happen only at probabilistic membranes), which keeps no block in
+// registers across the march.
+__device__ __forceinline__ uint32_t exch_word0(uint32_t xhi, uint32_t x) { return (xhi << 16) | (x >> 16); }
+
 template <typename R>
-__device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, int64_t i, int j)
+__device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, int64_t i, int j, uint32_t xhi)
 {
     if constexpr (sizeof(R) == 4) {
         const U4 blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A, wid);
-        const uint32_t word = j == 0 ? blk.x : (j == 1 ? blk.y : (j == 2 ? blk.z : blk.w));
+        const uint32_t word = j == 0 ? exch_word0(xhi, blk.x) : (j == 1 ? blk.y : (j == 2 ? blk.z : blk.w));
         return u01_f32(word);
     } else {
         const U4 blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A + (uint32_t)(j >> 1), wid);
@@ -278,10 +285,13 @@ __device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, in
 
 // Direction of step i, main.cu:140-151.  f32 layout: one Philox block serves
 // steps 2m (words x, y) and 2m+1 (words z, w); the azimuth's sin/cos come from
-// the 256-angle table tab.
+// the 256-angle table tab.  The polar uniform uses bits 9..31 of the first
+// word and the azimuth bits 8..31 of the second; xhi returns the low bytes of
+// the two words (first word's in bits 8..15), the high half of the step's
+// first exchange draw.  f64 layout: xhi = 0 (unused).
 template <typename R>
 __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, int64_t i, bool fresh_block,
-                                          uint32_t &b0, uint32_t &b1, const float2 *tab)
+                                          uint32_t &b0, uint32_t &b1, const float2 *tab, uint32_t &xhi)
 {
     R u1, u2;
     if constexpr (sizeof(R) == 4) {
@@ -295,6 +305,7 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
             b0 = r.z;
             b1 = r.w;
         }
+        xhi = ((w0 & 0xFFu) << 8) | (w1 & 0xFFu);
         u1 = u01_f32(w0);
         const R cth = (R)1 - (R)2 * u1;
         const R sth = (R)2 * sqrt_rn_mid(u1 * ((R)1 - u1));
@@ -302,6 +313,7 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
         sincos_turns_table(w1, tab, sp, cp);
         return V3<R>{sth * cp, sth * sp, cth};
     } else {
+        xhi = 0u;
         U4 r = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_DIR, wid);
         u1 = u01_f64(r.x, r.y);
         u2 = u01_f64(r.z, r.w);
@@ -320,10 +332,13 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
 // numbering) and reflects when u >= P.  Returns false when the step needs the
 // march (which replays the same draws); otherwise every exchange face
 // reflects, ndraw counts the draws and the tangent pass adds their scores.
+// With f32 draws the first draw's high half xhi bounds u to
+// [u01_f32(xhi << 16), u01_f32((xhi << 16) | 0xFFFF)]; the Philox block is
+// computed only when P lies inside those bounds or a second face draws.
 template <typename R, int NP>
 __device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLocal<R> &L, const JobDesc &J,
-                                               uint32_t rels, uint64_t wid, int32_t i, Tan<R, NP> &T,
-                                               uint32_t &ndraw)
+                                               uint32_t rels, uint64_t wid, int32_t i, uint32_t xhi,
+                                               Tan<R, NP> &T, uint32_t &ndraw)
 {
     if (!L.exch_fast) return false;
     const int32_t pidx = w.c * NC + (3 - w.c);
@@ -333,20 +348,21 @@ __device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLoca
     const R P = w.c == 1 ? L.P12 : L.P21;
     bool cross = false;
     if constexpr (sizeof(R) == 4) {
-        const U4 blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A, wid);
-#pragma unroll
-        for (int s = 0; s < 3; s++) {
-            if (((rels >> (2 * s)) & 3u) == 2u) {
-                const uint32_t word = ndraw == 0u ? blk.x : (ndraw == 1u ? blk.y : blk.z);
-                cross = cross || u01_f32(word) < P;
-                ndraw++;
-            }
+        ndraw = (uint32_t)__builtin_popcount((rels >> 1) & ~rels & 0x15u);  // CELL_OTHER slots
+        const bool lo_cross = u01_f32((xhi << 16) | 0xFFFFu) < P;  // u < P for every low half
+        const bool hi_refl = u01_f32(xhi << 16) >= P;              // u >= P for every low half
+        cross = lo_cross;
+        if ((!lo_cross && !hi_refl) || ndraw > 1u) {
+            const U4 blk = rng_block(L.key0, L.key1, (uint32_t)i, STREAM_EXCH_A, wid);
+            cross = cross || u01_f32(exch_word0(xhi, blk.x)) < P;
+            cross = cross || (ndraw > 1u && u01_f32(blk.y) < P);
+            cross = cross || (ndraw > 2u && u01_f32(blk.z) < P);
         }
     } else {
 #pragma unroll
         for (int s = 0; s < 3; s++) {
             if (((rels >> (2 * s)) & 3u) == 2u) {
-                cross = cross || exch_uniform<R>(L, wid, i, (int)ndraw) < P;
+                cross = cross || exch_uniform<R>(L, wid, i, (int)ndraw, 0u) < P;
                 ndraw++;
             }
         }
@@ -382,7 +398,7 @@ __device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLoca
 // comparisons that are not transitive: a near tie).
 template <typename R, int NP, bool EXCH>
 __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t cell,
-                                          const V3<R> &v, uint64_t wid, int32_t i, Tan<R, NP> &T)
+                                          const V3<R> &v, uint64_t wid, int32_t i, uint32_t xhi, Tan<R, NP> &T)
 {
     const R sc = w.sc;
     const R ex = fma(sc, v.x, w.o.x);
@@ -424,7 +440,7 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
         const uint32_t edge = rels & (rels >> 1) & 0x15u;  // CELL_EDGE slots
         if (((rels >> 1) & ~rels & 0x15u) != 0u) {  // CELL_OTHER slots
             if constexpr (EXCH) {
-                if (!exchange_faces<R, NP>(w, L, J, rels, wid, i, T, ndraw)) return false;
+                if (!exchange_faces<R, NP>(w, L, J, rels, wid, i, xhi, T, ndraw)) return false;
             } else {
                 return false;
             }
@@ -496,9 +512,12 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
 // reached when f * step >= t_e - t_{e-1} (main.cu:231); the remaining
 // fraction finishes the step (main.cu:237-243).  The three event slots are
 // unrolled and predicated, so a wavefront never loops on its slowest lane.
-template <typename R, int NP>
+// Exchange draws (0 < P < 1 between two non-null classes) exist only in
+// geometries with two non-null classes, which the host runs with EXCH set;
+// without it the draw is not compiled.
+template <typename R, int NP, bool EXCH>
 __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t wid,
-                                      int64_t i, V3<R> v, Tan<R, NP> &T)
+                                      int64_t i, V3<R> v, uint32_t xhi, Tan<R, NP> &T)
 {
     V3<R> iv = {(R)1 / v.x, (R)1 / v.y, (R)1 / v.z};
     // face distances in units of path, main.cu:199-219
@@ -607,8 +626,8 @@ __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const 
                 const int pidx = w.c * NC + cn;
                 const int cat = (int)((L.pcat >> (2 * pidx)) & 3u);
                 bool cross = (cat == 1);
-                if (cat == 2) {
-                    const R u = exch_uniform<R>(L, wid, i, nd);
+                if (EXCH && cat == 2) {
+                    const R u = exch_uniform<R>(L, wid, i, nd, xhi);
                     nd++;
                     cross = u < L.P[pidx];
                     if constexpr (NP > 0) {
@@ -918,9 +937,11 @@ void walk_kernel(LaunchArgs a)
 
         for (int32_t i = step0; i < step1; i++) {
             const bool fresh = ((i & 1) == 0) || (i == step0);
-            const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1, tab);
+            uint32_t xhi;
+            const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1, tab, xhi);
             const int32_t lin0 = w.lin;
-            if (!fast_step<R, NP, (SIG & 4) != 0>(w, L, J, cell, v, wid, i, T)) march<R, NP>(w, L, J, wid, i, v, T);
+            if (!fast_step<R, NP, (SIG & 4) != 0>(w, L, J, cell, v, wid, i, xhi, T))
+                march<R, NP, (SIG & 4) != 0>(w, L, J, wid, i, v, xhi, T);
             if (w.lin != lin0) cell = L.cells[w.lin];  // only lanes that changed voxel read
             if (SIG & 2) {
                 // running sums of the displacement in voxels, (g + o) - (gi + oi);

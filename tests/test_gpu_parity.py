@@ -66,6 +66,21 @@ def test_exchange_three_classes_bitexact(rms, orc, engine, prec):
     assert gpu[0]["counters"][4] > 0  # exchange draws happened
 
 
+def test_exchange_draws_split_uniform_bitexact(rms, orc, engine):
+    """f32 exchange draw 0 takes its high 16 bits from the direction words and
+    its low half from the exchange block only when P lies inside the high
+    half's 2^-16 interval.  With kappa = 0.05 both directions draw (P ~ 1e-3),
+    6.5e6 draws give ~100 such steps, and noisy blobs give steps with two or
+    three exchange faces (draws 1, 2)."""
+    print("case exchange draws split uniform f32", flush=True)
+    lab = random_blobs((12, 12, 12), 3, seed=7)
+    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0, 0.6], T2=[50.0, 80.0, 20.0], kappa=0.05)
+    jobs = [dict(geom=geom, nwalk=16384, seed=77)]
+    gpu, ref = _both(rms, orc, engine, jobs, "f32", dt=2e-4, TN=2000, timepoints=20)
+    _assert_same(gpu, ref)
+    assert gpu[0]["counters"][4] > 6_000_000
+
+
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_pgse_bitexact(rms, orc, engine, prec):
     print(f"case pgse {prec}", flush=True)

@@ -151,6 +151,7 @@ def main():
     kern_ms = float(np.mean(walk_ms))
     kern_rate = steps_total / (kern_ms / 1e3)
     achieved = kern_rate * ops / 1e12
+    traffic = MEASURED_TRAFFIC.get((args.precision, args.walkers, TN))
     res0 = r[0]
     counters = res0["counters"].tolist()
     # physics summary of the last run (SE from 32 batch-means replicas)
@@ -181,8 +182,12 @@ def main():
         "roofline": {
             "bound": "valu", "achieved": achieved, "peak": VALU_PEAK_TOPS, "unit": "Tlane-op/s",
             "frac": achieved / VALU_PEAK_TOPS,
-            "traffic": MEASURED_TRAFFIC.get((args.precision, args.walkers, TN)),
+            "traffic": traffic,
             "ops_per_walker_step": ops, "kernel_ms_avg": kern_ms, "kernel_walker_steps_per_s": kern_rate,
+            # the same launch against the HBM roofline: measured bytes / kernel time
+            "hbm": None if traffic is None else {
+                "achieved": traffic / (kern_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": traffic / (kern_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
         },
         "physics": {"D_axial_end": float(D[-1, 0]), "K_axial_end": float(K[-1, 0]),
                     "S_first_rows": [float(x) for x in S[:3]], "counters": counters},

@@ -18,6 +18,8 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -60,9 +62,26 @@ uint64_t urandom_seed()
     return s;
 }
 
+int run_body(int argc, char **argv, bool pgse);
+
 }  // namespace
 
+// Exceptions (a lattice too large for host memory, a failed stream) end the
+// program with a message and exit status 1, like the other errors.
 int run_cli(int argc, char **argv, bool pgse)
+{
+    try {
+        return run_body(argc, argv, pgse);
+    } catch (const std::bad_alloc &) {
+        return die("host memory allocation failed");
+    } catch (const std::exception &e) {
+        return die(e.what());
+    }
+}
+
+namespace {
+
+int run_body(int argc, char **argv, bool pgse)
 {
     Opts o;
     for (int i = 1; i < argc; i++) {
@@ -248,3 +267,5 @@ int run_cli(int argc, char **argv, bool pgse)
     if (!ok) return die(err);
     return 0;
 }
+
+}  // namespace

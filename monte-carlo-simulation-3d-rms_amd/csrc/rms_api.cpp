@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <new>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -106,11 +107,37 @@ static rms_status fail(rms_ctx *ctx, rms_status s, const std::string &msg)
     return s;
 }
 
+// C++ exceptions (host allocation failures in the std containers) must not
+// cross the C ABI: every entry point that allocates on the host runs its body
+// through guard(), which turns them into status codes.
+template <typename F>
+static rms_status guard(rms_ctx *ctx, F &&body) noexcept
+{
+    const char *msg;
+    rms_status s;
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        msg = "host memory allocation failed";
+        s = RMS_ERR_OUT_OF_MEMORY;
+    } catch (...) {
+        msg = "unexpected host exception";
+        s = RMS_ERR_STATE;
+    }
+    if (ctx) {
+        try {
+            ctx->err = msg;
+        } catch (...) {
+        }
+    }
+    return s;
+}
+
 extern "C" const char *rms_version(void) { return kVersion; }
 
 extern "C" int32_t rms_nval(int32_t nclass, int32_t nbvec) { return 1 + nclass + 6 + 15 + nbvec; }
 
-extern "C" rms_status rms_create(const rms_device_opts *opts, rms_ctx **out)
+static rms_status create_impl(const rms_device_opts *opts, rms_ctx **out)
 {
     if (!out) return RMS_ERR_INVALID_ARG;
     *out = nullptr;
@@ -136,6 +163,11 @@ extern "C" rms_status rms_create(const rms_device_opts *opts, rms_ctx **out)
     }
     *out = c;
     return RMS_OK;
+}
+
+extern "C" rms_status rms_create(const rms_device_opts *opts, rms_ctx **out)
+{
+    return guard(nullptr, [&] { return create_impl(opts, out); });
 }
 
 extern "C" rms_status rms_destroy(rms_ctx *ctx)
@@ -198,7 +230,7 @@ static bool derive(const rms_geometry &g, double dt, double step[NC], double inv
     return true;
 }
 
-extern "C" rms_status rms_derive(const rms_geometry *geom, double dt, double *step, double *T2s, double *P)
+static rms_status derive_impl(const rms_geometry *geom, double dt, double *step, double *T2s, double *P)
 {
     if (!geom || !(dt > 0.0)) return RMS_ERR_INVALID_ARG;
     double st[NC], ist[NC], t2[NC], pp[NC * NC];
@@ -209,6 +241,11 @@ extern "C" rms_status rms_derive(const rms_geometry *geom, double dt, double *st
     if (T2s) std::memcpy(T2s, t2, sizeof(t2));
     if (P) std::memcpy(P, pp, sizeof(pp));
     return RMS_OK;
+}
+
+extern "C" rms_status rms_derive(const rms_geometry *geom, double dt, double *step, double *T2s, double *P)
+{
+    return guard(nullptr, [&] { return derive_impl(geom, dt, step, T2s, P); });
 }
 
 // Constants of the tangent (adjoint) pass, from the same quantities as
@@ -263,7 +300,7 @@ static void derive_tangent(const rms_geometry &g, double dt, const double step[N
     }
 }
 
-extern "C" rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id)
+static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id)
 {
     if (!ctx || !geom || !geom_id) return fail(ctx, RMS_ERR_INVALID_ARG, "null argument");
     const rms_geometry &g = *geom;
@@ -325,9 +362,20 @@ extern "C" rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, i
         ctx->err = std::string("geometry upload: ") + hipGetErrorString(e);
         return e == hipErrorOutOfMemory ? RMS_ERR_OUT_OF_MEMORY : RMS_ERR_HIP;
     }
-    ctx->geoms.push_back(gd);
+    try {
+        ctx->geoms.push_back(gd);
+    } catch (...) {
+        (void)hipFree(gd.d_cells);
+        (void)hipFree(gd.d_cls);
+        throw;
+    }
     *geom_id = (int32_t)ctx->geoms.size() - 1;
     return RMS_OK;
+}
+
+extern "C" rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id)
+{
+    return guard(ctx, [&] { return add_geometry_impl(ctx, geom, geom_id); });
 }
 
 // ---------------------------------------------------------------- plans
@@ -405,8 +453,16 @@ extern "C" rms_status rms_plan_destroy(rms_plan *plan)
     return RMS_OK;
 }
 
-extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_run_params *prm,
-                                      rms_plan **out)
+namespace {
+// owns a plan under construction; released into *out on success
+struct PlanHolder {
+    rms_plan *p = nullptr;
+    ~PlanHolder() { plan_free(p); }
+};
+}  // namespace
+
+static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_run_params *prm,
+                                   rms_plan **out)
 {
     if (!ctx || !jobs || njobs < 1 || !prm || !out) return fail(ctx, RMS_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
@@ -423,22 +479,17 @@ extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t
     const int64_t T = P.timepoints;
     const int64_t Tstep = P.TN / P.timepoints;
 
-    rms_plan *p = new rms_plan();
+    if (hipSetDevice(ctx->device) != hipSuccess) return fail(ctx, RMS_ERR_HIP, "hipSetDevice failed");
+    PlanHolder holder;
+    rms_plan *p = holder.p = new rms_plan();
     p->ctx = ctx;
     p->prm = P;
     p->prm.replicas = R;
-    if (hipSetDevice(ctx->device) != hipSuccess) {
-        delete p;
-        return fail(ctx, RMS_ERR_HIP, "hipSetDevice failed");
-    }
     rms_status st = RMS_OK;
-#define TRY(x)                   \
-    do {                         \
-        st = (x);                \
-        if (st != RMS_OK) {      \
-            plan_free(p);        \
-            return st;           \
-        }                        \
+#define TRY(x)                       \
+    do {                             \
+        st = (x);                    \
+        if (st != RMS_OK) return st; \
     } while (0)
 #define BAD(msg) TRY(fail(ctx, RMS_ERR_INVALID_ARG, msg))
 
@@ -698,9 +749,16 @@ extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t
         if (hipEventCreate(&e) != hipSuccess) TRY(fail(ctx, RMS_ERR_HIP, "hipEventCreate failed"));
     }
     *out = p;
+    holder.p = nullptr;
     return RMS_OK;
 #undef TRY
 #undef BAD
+}
+
+extern "C" rms_status rms_plan_create(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_run_params *prm,
+                                      rms_plan **out)
+{
+    return guard(ctx, [&] { return plan_create_impl(ctx, jobs, njobs, prm, out); });
 }
 
 static LaunchArgs make_args(rms_plan *p, int64_t s0, int64_t s1)
@@ -741,9 +799,8 @@ static rms_status enqueue(rms_plan *p, bool with_events)
     return RMS_OK;
 }
 
-extern "C" rms_status rms_plan_execute(rms_plan *p, rms_job_out *outs, rms_timing *timing)
+static rms_status plan_execute_impl(rms_plan *p, rms_job_out *outs, rms_timing *timing)
 {
-    if (!p) return RMS_ERR_INVALID_ARG;
     rms_ctx *ctx = p->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const int nj = (int)p->jobs.size();
@@ -845,6 +902,12 @@ extern "C" rms_status rms_plan_execute(rms_plan *p, rms_job_out *outs, rms_timin
     return RMS_OK;
 }
 
+extern "C" rms_status rms_plan_execute(rms_plan *p, rms_job_out *outs, rms_timing *timing)
+{
+    if (!p) return RMS_ERR_INVALID_ARG;
+    return guard(p->ctx, [&] { return plan_execute_impl(p, outs, timing); });
+}
+
 extern "C" rms_status rms_run(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_run_params *params,
                               rms_job_out *outs, rms_timing *timing)
 {
@@ -869,6 +932,7 @@ static rms_status adjoint_setup(rms_plan *p)
     p->adj_np = std::min(np, 3);
     p->cot_rec.assign(nj, nullptr);
     p->cot_sig.assign(nj, nullptr);
+    p->red_adj.clear();  // a failed earlier setup may have left entries
     rms_status st;
     for (int j = 0; j < nj; j++) {
         const JobHost &H = p->jobs[j];
@@ -899,9 +963,8 @@ static rms_status adjoint_setup(rms_plan *p)
     return RMS_OK;
 }
 
-extern "C" rms_status rms_plan_adjoint(rms_plan *p, const rms_job_adj *adj, rms_timing *timing)
+static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, rms_timing *timing)
 {
-    if (!p || !adj) return RMS_ERR_INVALID_ARG;
     rms_ctx *ctx = p->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const int nj = (int)p->jobs.size();
@@ -974,4 +1037,10 @@ extern "C" rms_status rms_plan_adjoint(rms_plan *p, const rms_job_adj *adj, rms_
         }
     }
     return RMS_OK;
+}
+
+extern "C" rms_status rms_plan_adjoint(rms_plan *p, const rms_job_adj *adj, rms_timing *timing)
+{
+    if (!p || !adj) return RMS_ERR_INVALID_ARG;
+    return guard(p->ctx, [&] { return plan_adjoint_impl(p, adj, timing); });
 }

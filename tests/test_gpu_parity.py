@@ -66,6 +66,19 @@ def test_exchange_three_classes_bitexact(rms, orc, engine, prec):
     assert gpu[0]["counters"][4] > 0  # exchange draws happened
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_exchange_four_classes_bitexact(rms, orc, engine, prec):
+    """Three non-null classes: a neighbour of another class is not one fixed
+    class, so every exchange face goes through the march and its draws."""
+    print(f"case exchange 4 classes {prec}", flush=True)
+    lab = random_blobs((10, 9, 11), 4, seed=8)
+    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0, 0.6, 1.2], T2=[50.0, 80.0, 20.0, 40.0])
+    jobs = [dict(geom=geom, nwalk=640, seed=31)]
+    gpu, ref = _both(rms, orc, engine, jobs, prec, dt=2e-4, TN=400, timepoints=40)
+    _assert_same(gpu, ref)
+    assert gpu[0]["counters"][4] > 0
+
+
 def test_exchange_draws_split_uniform_bitexact(rms, orc, engine):
     """f32 exchange draw 0 takes its high 16 bits from the direction words and
     its low half from the exchange block only when P lies inside the high

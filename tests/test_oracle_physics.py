@@ -228,3 +228,30 @@ def test_f32_and_f64_agree_statistically(orc):
     d = out["f32"][0] - out["f64"][0]
     se = np.hypot(out["f32"][1], out["f64"][1])
     assert np.max(np.abs(d / se)) < 4.5
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_initial_positions_uniform_over_walkable_voxels(orc, prec):
+    """main.cu:117-132: start points are uniform over the non-null voxels.
+    A border of null voxels keeps walkers off the lattice edges, so after one
+    step the origin (gi, oi) is still the start point: every start voxel is
+    walkable, the voxel counts pass a chi-square test against the uniform
+    distribution, and the in-voxel offsets are uniform on [0, 1)."""
+    from scipy import stats
+    rng = np.random.default_rng(21)
+    lab = rng.integers(0, 3, size=(12, 10, 8)).astype(np.uint8)
+    lab[[0, -1], :, :] = 0
+    lab[:, [0, -1], :] = 0
+    lab[:, :, [0, -1]] = 0
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0, 0.5])
+    n = 40000
+    r = orc.run([dict(geom=g, nwalk=n, seed=4242)], DT, 1, 1, prec=prec, final_state=True)[0]
+    gi, oi = r["fin_gi"], r["fin_oi"]
+    assert np.all(lab[gi[0], gi[1], gi[2]] != 0)
+    walk = np.flatnonzero(lab.ravel() != 0)
+    lin = np.ravel_multi_index((gi[0], gi[1], gi[2]), lab.shape)
+    counts = np.bincount(lin, minlength=lab.size)[walk]
+    assert stats.chisquare(counts).pvalue > 1e-3
+    assert np.all((oi >= 0) & (oi < 1))
+    for k in range(3):
+        assert stats.kstest(oi[k], "uniform").pvalue > 1e-3

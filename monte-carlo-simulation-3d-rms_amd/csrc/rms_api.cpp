@@ -408,6 +408,17 @@ static rms_status pupload(rms_plan *p, T **ptr, const T *src, size_t n)
     return RMS_OK;
 }
 
+// The PGSE sums keep sum(g) of the steps since the last flush in int32; with
+// voxel indices below max(nx, ny, nz) a flush every 2^k steps, k the largest
+// value <= 16 with 2^k max(nx, ny, nz) < 2^31, keeps it in range.
+static int32_t pgse_flush_log2(int32_t nx, int32_t ny, int32_t nz)
+{
+    const int64_t m = std::max<int64_t>(1, std::max<int64_t>(nx, std::max<int64_t>(ny, nz)));
+    int32_t k = 16;
+    while (k > 0 && (m << k) > 2147483647LL) k--;
+    return k;
+}
+
 static int64_t first_false_lt(double dt, int64_t TN, double thr)
 {
     int64_t lo = 0, hi = TN;
@@ -529,6 +540,7 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         D.nx = G.meta.nx;
         D.ny = G.meta.ny;
         D.nz = G.meta.nz;
+        D.pg_log2 = pgse_flush_log2(G.meta.nx, G.meta.ny, G.meta.nz);
         D.nclass = G.meta.nclass;
         D.nbvec = J.nbvec;
         D.ngrad = J.ngrad;
@@ -716,7 +728,10 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         }
         TRY(palloc(p, &p->st.tsum, sizeof(double) * NC * ws));
         TRY(palloc(p, &p->st.cnt, sizeof(uint32_t) * 5 * ws));
-        if (sig & 2) TRY(palloc(p, &p->st.ps, sizeof(double) * 3 * ws));
+        if (sig & 2) {
+            TRY(palloc(p, &p->st.ps, sizeof(double) * 3 * ws));
+            TRY(palloc(p, &p->st.sg, sizeof(int32_t) * 3 * ws));
+        }
     }
     // batch -> job table
     std::vector<int32_t> bj(nbatch);

@@ -71,7 +71,11 @@ struct Walker {
     V3<R> o;         // offset inside the voxel, [0,1]
     V3<int32_t> gi;  // origin voxel (mirror-unfolded)
     V3<R> oi;        // origin offset
-    V3<double> xi;   // gi + oi (PGSE sums)
+    // PGSE running sums of the displacement in voxels, kept split between
+    // flushes: ps holds the flushed sum plus sum(o) of the later steps, sg the
+    // sum(g) of those steps; a flush adds sg - n (gi + oi)
+    V3<double> ps;
+    V3<int32_t> sg;
     int32_t c;       // class
     int32_t lin;     // linear voxel index of g
     R tcur;          // steps spent in class c since the last flush
@@ -215,11 +219,19 @@ __device__ __forceinline__ void flush(Walker<R> &w)
     w.tcur = (R)0;
 }
 
-// The origin as one double per axis, the term the PGSE sums subtract.
+// The origin of one axis as a double, the term the PGSE sums subtract.
 template <typename R>
-__device__ __forceinline__ void set_origin_sum(Walker<R> &w)
+__device__ __forceinline__ double origin(int32_t gi, R oi)
 {
-    w.xi = {(double)w.gi.x + (double)w.oi.x, (double)w.gi.y + (double)w.oi.y, (double)w.gi.z + (double)w.oi.z};
+    return (double)gi + (double)oi;
+}
+
+// PGSE sums after a mirror moved the origin of one axis from xo to xn: the
+// npg terms since the last flush were taken with xo, the flush subtracts the
+// origin at flush time, so the difference npg (xn - xo) is added now.
+__device__ __forceinline__ double mirror_ps(double ps, int32_t npg, double xo, double xn)
+{
+    return ps + (double)npg * (xn - xo);
 }
 
 // Rejection-sampled start voxel, main.cu:117-132 (uniforms over (0,1)).
@@ -256,7 +268,6 @@ __device__ __forceinline__ void init_walker(Walker<R> &w, const JobLocal<R> &L, 
     // lattices without a non-null voxel are rejected on the host
     w.gi = w.g;
     w.oi = w.o;
-    set_origin_sum(w);
 }
 
 // Exchange draw j (0..3) of step i.  f32 layout: draw 0 is u01_f32 of the
@@ -398,7 +409,8 @@ __device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLoca
 // comparisons that are not transitive: a near tie).
 template <typename R, int NP, bool EXCH>
 __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t cell,
-                                          const V3<R> &v, uint64_t wid, int32_t i, uint32_t xhi, Tan<R, NP> &T)
+                                          const V3<R> &v, uint64_t wid, int32_t i, uint32_t xhi, int32_t npg,
+                                          Tan<R, NP> &T)
 {
     const R sc = w.sc;
     const R ex = fma(sc, v.x, w.o.x);
@@ -448,18 +460,23 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
         if (edge != 0u) {
             const int32_t kz = 3 - kx - ky;
             if (((edge >> (2 * kx)) & 1u) != 0u) {
+                const double xo = origin(w.gi.x, w.oi.x);
                 w.gi.x = cx < 0 ? -w.gi.x : 2 * L.nx - w.gi.x;
                 w.oi.x = -w.oi.x;
+                w.ps.x = mirror_ps(w.ps.x, npg, xo, origin(w.gi.x, w.oi.x));
             }
             if (((edge >> (2 * ky)) & 1u) != 0u) {
+                const double xo = origin(w.gi.y, w.oi.y);
                 w.gi.y = cy < 0 ? -w.gi.y : 2 * L.ny - w.gi.y;
                 w.oi.y = -w.oi.y;
+                w.ps.y = mirror_ps(w.ps.y, npg, xo, origin(w.gi.y, w.oi.y));
             }
             if (((edge >> (2 * kz)) & 1u) != 0u) {
+                const double xo = origin(w.gi.z, w.oi.z);
                 w.gi.z = cz < 0 ? -w.gi.z : 2 * L.nz - w.gi.z;
                 w.oi.z = -w.oi.z;
+                w.ps.z = mirror_ps(w.ps.z, npg, xo, origin(w.gi.z, w.oi.z));
             }
-            set_origin_sum(w);
             nedge = (uint32_t)__builtin_popcount(edge);
         }
         refl = (rels | (rels >> 1)) & 0x15u;
@@ -517,7 +534,7 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
 // without it the draw is not compiled.
 template <typename R, int NP, bool EXCH>
 __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t wid,
-                                      int64_t i, V3<R> v, uint32_t xhi, Tan<R, NP> &T)
+                                      int64_t i, V3<R> v, uint32_t xhi, int32_t npg, Tan<R, NP> &T)
 {
     V3<R> iv = {(R)1 / v.x, (R)1 / v.y, (R)1 / v.z};
     // face distances in units of path, main.cu:199-219
@@ -612,11 +629,12 @@ __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const 
                 for3([&](auto K) __attribute__((always_inline)) {
                     constexpr int k = decltype(K)::value;
                     const bool h = (k == hit);
+                    const double xo = origin(at<k>(w.gi), at<k>(w.oi));
                     const int32_t gm = (nk < 0) ? -at<k>(w.gi) : 2 * Nh - at<k>(w.gi);
                     at<k>(w.gi) = h ? gm : at<k>(w.gi);
                     at<k>(w.oi) = h ? -at<k>(w.oi) : at<k>(w.oi);
+                    if (h) at<k>(w.ps) = mirror_ps(at<k>(w.ps), npg, xo, origin(at<k>(w.gi), at<k>(w.oi)));
                 });
-                set_origin_sum(w);
                 w.nmirror++;
                 flip = true;
             } else {
@@ -739,7 +757,6 @@ __device__ __forceinline__ void load_state(Walker<R> &w, const StateBuf &st, int
         w.oi = {st.od[3 * ws + s], st.od[4 * ws + s], st.od[5 * ws + s]};
         w.tcur = st.tcd[s];
     }
-    set_origin_sum(w);
     w.c = st.c[s];
     w.ts0 = st.tsum[s];
     w.ts1 = st.tsum[ws + s];
@@ -907,7 +924,8 @@ void walk_kernel(LaunchArgs a)
             }
             T.Sk = 0.0;
         }
-        V3<double> ps = {0.0, 0.0, 0.0};
+        w.ps = {0.0, 0.0, 0.0};
+        w.sg = {0, 0, 0};
         if (a.step0 == 0) {
             init_walker<R>(w, L, wid);
             w.sc = sel4(w.c, L.st0, L.st1, L.st2, L.st3);
@@ -917,7 +935,10 @@ void walk_kernel(LaunchArgs a)
             w.lin = w.g.x * L.sx + w.g.y * L.nz + w.g.z;
             w.sc = sel4(w.c, L.st0, L.st1, L.st2, L.st3);
             w.isc = sel4(w.c, L.is0, L.is1, L.is2, L.is3);
-            if (SIG & 2) ps = {a.st.ps[slot], a.st.ps[ws + slot], a.st.ps[2 * ws + slot]};
+            if (SIG & 2) {
+                w.ps = {a.st.ps[slot], a.st.ps[ws + slot], a.st.ps[2 * ws + slot]};
+                w.sg = {a.st.sg[slot], a.st.sg[ws + slot], a.st.sg[2 * ws + slot]};
+            }
         }
         // step counters fit int32 (TN < 2^31 is checked on the host) and are
         // wave-uniform: next record step and next PGSE boundary at or after step0
@@ -926,9 +947,15 @@ void walk_kernel(LaunchArgs a)
         const int32_t rend = a.timepoints * Tstep;  // records only for r < timepoints
         int kb = 1;
         int32_t bnext = -1;
+        // PGSE flushes after the steps i with i + 1 a lobe boundary or a
+        // multiple of 2^pg_log2; i0 = the last flush point at or before step0
+        const int32_t pgmask = (int32_t)((1u << J.pg_log2) - 1u);
+        int32_t i0 = step0 & ~pgmask;
         if (SIG & 2) {
             while (kb <= nbound && J.bounds[kb] <= a.step0) kb++;
             bnext = __builtin_amdgcn_readfirstlane(kb <= nbound ? (int32_t)J.bounds[kb] : -1);
+            const int32_t lastb = (int32_t)J.bounds[kb - 1];
+            i0 = __builtin_amdgcn_readfirstlane(lastb > i0 ? lastb : i0);
         }
         uint32_t b0 = 0, b1 = 0;
         double *part = J.part + lb * (int64_t)a.timepoints * nval;
@@ -940,15 +967,20 @@ void walk_kernel(LaunchArgs a)
             uint32_t xhi;
             const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1, tab, xhi);
             const int32_t lin0 = w.lin;
-            if (!fast_step<R, NP, (SIG & 4) != 0>(w, L, J, cell, v, wid, i, xhi, T))
-                march<R, NP, (SIG & 4) != 0>(w, L, J, wid, i, v, xhi, T);
+            const int32_t npg = i - i0;  // PGSE terms since the last flush
+            if (!fast_step<R, NP, (SIG & 4) != 0>(w, L, J, cell, v, wid, i, xhi, npg, T))
+                march<R, NP, (SIG & 4) != 0>(w, L, J, wid, i, v, xhi, npg, T);
             if (w.lin != lin0) cell = L.cells[w.lin];  // only lanes that changed voxel read
             if (SIG & 2) {
-                // running sums of the displacement in voxels, (g + o) - (gi + oi);
-                // the snapshots at lobe boundaries are scaled to um
-                ps.x = ps.x + (((double)w.g.x + (double)w.o.x) - w.xi.x);
-                ps.y = ps.y + (((double)w.g.y + (double)w.o.y) - w.xi.y);
-                ps.z = ps.z + (((double)w.g.z + (double)w.o.z) - w.xi.z);
+                // running sums of the displacement in voxels, (g + o) - (gi + oi),
+                // as sum(o) in FP64 and sum(g) in int32 until the next flush; the
+                // snapshots at lobe boundaries are scaled to um
+                w.ps.x = w.ps.x + (double)w.o.x;
+                w.ps.y = w.ps.y + (double)w.o.y;
+                w.ps.z = w.ps.z + (double)w.o.z;
+                w.sg.x += w.g.x;
+                w.sg.y += w.g.y;
+                w.sg.z += w.g.z;
                 if constexpr (NP > 0) {
 #pragma unroll
                     for (int p = 0; p < NP; p++) {
@@ -957,11 +989,20 @@ void walk_kernel(LaunchArgs a)
                         T.PJ[p].z = T.PJ[p].z + (double)T.Jo[p].z;
                     }
                 }
-                if (i + 1 == bnext) {
+                const bool snap = i + 1 == bnext;
+                if (__builtin_expect(snap || ((i + 1) & pgmask) == 0, 0)) {
+                    const double n = (double)(i + 1 - i0);
+                    w.ps.x = w.ps.x + ((double)w.sg.x - n * origin(w.gi.x, w.oi.x));
+                    w.ps.y = w.ps.y + ((double)w.sg.y - n * origin(w.gi.y, w.oi.y));
+                    w.ps.z = w.ps.z + ((double)w.sg.z - n * origin(w.gi.z, w.oi.z));
+                    w.sg = {0, 0, 0};
+                    i0 = i + 1;
+                }
+                if (snap) {
                     double *sp = J.snap + (int64_t)(kb * 3) * ws + slot;
-                    sp[0] = ps.x * L.res;
-                    sp[ws] = ps.y * L.res;
-                    sp[2 * ws] = ps.z * L.res;
+                    sp[0] = w.ps.x * L.res;
+                    sp[ws] = w.ps.y * L.res;
+                    sp[2 * ws] = w.ps.z * L.res;
                     if constexpr (NP > 0) {
 #pragma unroll
                         for (int p = 0; p < NP; p++) {
@@ -1152,9 +1193,12 @@ void walk_kernel(LaunchArgs a)
         } else {
             save_state<R>(w, a.st, slot);
             if (SIG & 2) {
-                a.st.ps[slot] = ps.x;
-                a.st.ps[ws + slot] = ps.y;
-                a.st.ps[2 * ws + slot] = ps.z;
+                a.st.ps[slot] = w.ps.x;
+                a.st.ps[ws + slot] = w.ps.y;
+                a.st.ps[2 * ws + slot] = w.ps.z;
+                a.st.sg[slot] = w.sg.x;
+                a.st.sg[ws + slot] = w.sg.y;
+                a.st.sg[2 * ws + slot] = w.sg.z;
             }
         }
     }

@@ -32,7 +32,7 @@ struct JobDesc {
     const uint64_t *cells;     // cell words, see above
     const uint8_t *cls;        // class of each voxel
     int32_t nx, ny, nz;
-    int32_t pad0;
+    int32_t pg_log2;           // PGSE sums flush at least every 2^pg_log2 steps
     int32_t nclass;
     int32_t nbvec;
     int32_t ngrad;
@@ -84,7 +84,8 @@ struct StateBuf {
     float *tcf;       // [ws]
     double *tcd;      // [ws]
     double *tsum;     // [NC][ws]
-    double *ps;       // [3][ws] PGSE running sums
+    double *ps;       // [3][ws] PGSE running sums (FP64 part)
+    int32_t *sg;      // [3][ws] PGSE sums of the voxel index since the last flush
     uint32_t *cnt;    // [5][ws]
     uint32_t *rng;    // [4][ws] cached direction block (f32 mode)
     int64_t ws;       // stride

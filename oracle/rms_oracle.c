@@ -508,6 +508,7 @@ typedef struct {
     double *qscale;   /* [timepoints][nbvec]: sqrt(bval/TD), main.cu:375 */
     int nbound;       /* boundaries 1..nbound (bounds[0] = 0) */
     int64_t bounds[OR_MAX_BOUNDS + 1];
+    int64_t pgmask;   /* PGSE sums flush after step i when (i + 1) & pgmask == 0 */
     int32_t *gk;      /* [ngrad][3] indices (ka, kb, ke) into bounds */
     double *snapbuf;  /* per thread */
     double *phasebuf; /* per thread */
@@ -703,6 +704,17 @@ static int or_run_impl(const or_job *jobs, int32_t njobs, const or_params *p, or
         }
         aux.nval = or_nval(J->geom->nclass, J->nbvec);
         aux.Tstep = p->TN / p->timepoints;
+        {
+            /* sum(g) between flushes stays below 2^31: the largest k <= 16 with
+             * 2^k max(nx, ny, nz) < 2^31 */
+            int64_t m = J->geom->nx;
+            if (J->geom->ny > m) m = J->geom->ny;
+            if (J->geom->nz > m) m = J->geom->nz;
+            if (m < 1) m = 1;
+            int k = 16;
+            while (k > 0 && (m << k) > 2147483647LL) k--;
+            aux.pgmask = ((int64_t)1 << k) - 1;
+        }
         int T = p->timepoints;
         double *TD = (double *)malloc(sizeof(double) * T);
         or_diff_times(p->dt, p->TN, T, TD);

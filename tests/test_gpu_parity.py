@@ -139,6 +139,23 @@ def test_chunked_and_graph_equal_single_launch(rms, orc, engine, prec):
     _assert_same(gr, ref, final_state=False)
 
 
+def test_pgse_sums_flush_period_bitexact(rms, orc, engine):
+    """The PGSE sums keep sum(g) in int32 between flushes at lobe boundaries
+    and every 2^16 steps.  70000 steps cross the 2^16 flush; the z lattice
+    edges of the cylinder mirror the origin between flushes; chunks of 30000
+    steps carry the unflushed sums across launches."""
+    print("case pgse flush period f32", flush=True)
+    lab = cylinder(16, 6, 6.0)
+    grad = pgse_table(fib_dirs(3), [1.0], [0.03], 0.01)
+    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0], T2=[80.0, 80.0])
+    jobs = [dict(geom=geom, nwalk=64, seed=606, grad=grad)]
+    base, ref = _both(rms, orc, engine, jobs, "f32", dt=2e-4, TN=70000, timepoints=7)
+    _assert_same(base, ref)
+    assert base[0]["counters"][2] > 1000  # mirrors happened
+    ch, _ = _both(rms, orc, engine, jobs, "f32", dt=2e-4, TN=70000, timepoints=7, chunk_steps=30000)
+    _assert_same(ch, ref)
+
+
 def _adj_jobs(rms, orc, jobs):
     gj, oj = [], []
     for jb in jobs:

@@ -163,6 +163,23 @@ def test_pgse_window_sums_equal_literal_loop(orc):
     np.testing.assert_allclose(ws["sig"], bf["sig"], rtol=1e-11, atol=1e-9)
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_pgse_split_sums_long_run_equal_literal_loop(orc, prec):
+    """The window sums keep sum(g) in int32 and flush every 2^16 steps; over
+    80000 steps the flush falls inside the second lobe, and the z lattice
+    edges mirror the origin between flushes.  Equal to the literal per-step
+    phase loop of main_PGSE.cu:340-346."""
+    lab = cylinder(12, 6, 4.5)
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0], T2=[70.0, 70.0])
+    grad = pgse_table(fib_dirs(3), [0.5, 2.0], [10.0], 5.0)
+    kw = dict(dt=DT, TN=80000, timepoints=8)
+    jobs = [dict(geom=g, nwalk=64, seed=18, grad=grad)]
+    ws = orc.run(jobs, prec=prec, **kw)[0]
+    bf = orc.run(jobs, prec=prec, pgse_bruteforce=True, **kw)[0]
+    assert ws["counters"][2] > 1000  # mirrors
+    np.testing.assert_allclose(ws["sig"], bf["sig"], rtol=1e-10, atol=1e-9)
+
+
 def test_pgse_free_diffusion_gaussian_phase(orc):
     """Gradients along z in a lattice 40 um long in z: no walker reaches a z
     edge, whose mirror would flip the sign of later displacements

@@ -419,6 +419,28 @@ static int32_t pgse_flush_log2(int32_t nx, int32_t ny, int32_t nz)
     return k;
 }
 
+// Threshold on the high half xhi of an FP32 step's first exchange draw: the
+// draw's uniform is at least u01_f32(xhi << 16) = (256 xhi + 1) 2^-24, so
+// xhi >= the smallest xhi with that bound >= P reflects whatever the low half.
+// -1: the face reflects without a draw (P = 0); 65536: never decided by xhi
+// alone (P = 1 crosses, and with more than two non-null classes the relation
+// word does not name the other class).
+static int32_t exchange_threshold(int32_t nclass, int cat, float P)
+{
+    if (nclass != 3 || cat == 1) return 65536;
+    if (cat == 0) return -1;
+    int32_t lo = 0, hi = 65536;  // smallest xhi in [0, 65536) with the bound >= P, else 65536
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) / 2;
+        const float umin = (2.0f * (float)(mid * 128) + 1.0f) * 5.9604644775390625e-08f;
+        if (umin >= P)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo;
+}
+
 static int64_t first_false_lt(double dt, int64_t TN, double thr)
 {
     int64_t lo = 0, hi = TN;
@@ -562,6 +584,8 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
             D.P_f[i] = (float)Pp[i];
             D.P_d[i] = Pp[i];
         }
+        D.xh12 = exchange_threshold(G.meta.nclass, Pcat[1 * NC + 2], D.P_f[1 * NC + 2]);
+        D.xh21 = exchange_threshold(G.meta.nclass, Pcat[2 * NC + 1], D.P_f[2 * NC + 1]);
         derive_tangent(G.meta, P.dt, step, inv_step, T2s, Pp, Pcat, D);
         JobHost &H = p->jobs[j];
         H.exchange = false;

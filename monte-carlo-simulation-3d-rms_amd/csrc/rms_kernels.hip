@@ -110,6 +110,7 @@ struct JobLocal {
     double res;
     const R *P;              // permeation probabilities [NC*NC]
     R P12, P21;              // P between classes 1 and 2
+    int32_t xh12, xh21;      // reflect-sure thresholds of the first draw's high half
     int32_t exch_fast;       // two non-null classes: a CELL_OTHER face is the other one
     // adjoint constants
     R ds1, ds2, ds3;         // d step_c / d D_c, classes 1..3
@@ -160,6 +161,8 @@ __device__ __forceinline__ JobLocal<R> job_local(const JobDesc &J)
     }
     L.P12 = L.P[1 * NC + 2];
     L.P21 = L.P[2 * NC + 1];
+    L.xh12 = J.xh12;
+    L.xh21 = J.xh21;
     L.exch_fast = J.nclass <= 3;
     L.T0 = J.T2s[0];
     L.T1 = J.T2s[1];
@@ -444,7 +447,19 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
         cur = rel == 0u ? q : cur;
     }
     uint32_t ndraw = 0u, nedge = 0u, refl = rels;  // reflected slots, bits 2s
-    if ((rels & 0x2Au) != 0u || kx == ky) {
+    bool rare = (rels & 0x2Au) != 0u || kx == ky;
+    if constexpr (EXCH && NP == 0 && sizeof(R) == 4) {
+        // one exchange face whose draw reflects whatever its low half (xhi at
+        // or above the class's threshold), or that reflects without a draw, is
+        // folded like a null face; other cases take the branch below
+        const uint32_t om = (rels >> 1) & ~rels & 0x15u;  // CELL_OTHER slots
+        const int32_t hc = w.c == 1 ? L.xh12 : L.xh21;
+        const bool easy = om == 0u || ((om & (om - 1u)) == 0u && (int32_t)xhi >= hc);
+        rare = (rels & (rels >> 1) & 0x15u) != 0u || kx == ky || !easy;
+        refl = (rels | (rels >> 1)) & 0x15u;
+        ndraw = (om != 0u && hc >= 0) ? 1u : 0u;
+    }
+    if (rare) {
         // a near tie goes to the march.  A lattice edge is a mirror: the
         // walker reflects and its origin is mirrored (main.cu:246-265,
         // 302-310).

@@ -42,6 +42,11 @@ struct JobDesc {
     uint32_t key0, key1;       // Philox key = job seed
     uint32_t pcat;             // 2 bits per class pair: 0 reflect, 1 cross, 2 draw
     int32_t nbound;            // PGSE boundaries 1..nbound
+    // FP32 exchange faces between classes 1 and 2 (from 1, from 2): -1 reflects
+    // without a draw (P = 0); otherwise the smallest high half xhi of the first
+    // draw with u01_f32(xhi << 16) >= P, so xhi >= it reflects whatever the low
+    // half (65536: never, P = 1 or more than two non-null classes)
+    int32_t xh12, xh21;
     double res;
     float step_f[NC], inv_step_f[NC], P_f[NC * NC];
     double step_d[NC], inv_step_d[NC], P_d[NC * NC];

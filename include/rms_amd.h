@@ -63,6 +63,10 @@ typedef struct {
     /* permeability between classes (um/ms), row-major [i][j];
      * < 0 selects the reference rule P_ij = min(1, sqrt(D_j/D_i)) */
     double kappa[RMS_NC_MAX * RMS_NC_MAX];
+    /* lattice edges: bit k set = axis k (x, y, z) wraps around (periodic
+     * tiling); clear = the reference's mirror (main.cu:246-265, 302-310) */
+    int32_t periodic;
+    int32_t reserved;
 } rms_geometry;
 
 /* One simulation on one geometry (one reference setup directory). */

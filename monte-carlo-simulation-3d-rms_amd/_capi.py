@@ -49,6 +49,8 @@ class Geometry(C.Structure):
         ("D", C.c_double * NC_MAX),
         ("T2", C.c_double * NC_MAX),
         ("kappa", C.c_double * (NC_MAX * NC_MAX)),
+        ("periodic", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

@@ -9,8 +9,9 @@
 // reference): --seed N (default: 8 bytes of /dev/urandom, as main.cu:491-496
 // reads /dev/random), --precision f64|f32 (default f64, the reference's FP64
 // positions), --timepoints N (default 1000, main.cu:35), --device N,
-// --dir PATH.  Inputs the reference would crash or hang on are rejected with
-// a message and exit status 1.
+// --dir PATH, --periodic AXES (e.g. xy: those lattice edges wrap around
+// instead of being the reference's mirrors).  Inputs the reference would
+// crash or hang on are rejected with a message and exit status 1.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -34,12 +35,15 @@ struct Opts {
     int precision = RMS_PREC_F64;
     int timepoints = RMS_REF_TIMEPOINTS;
     int device = 0;
+    int periodic = 0;  // bit k: axis k wraps around
     std::string dir = ".";
 };
 
 int usage(const char *prog)
 {
-    std::fprintf(stderr, "usage: %s [--seed N] [--precision f64|f32] [--timepoints N] [--device N] [--dir PATH]\n",
+    std::fprintf(stderr,
+                 "usage: %s [--seed N] [--precision f64|f32] [--timepoints N] [--device N] [--dir PATH] "
+                 "[--periodic AXES]\n",
                  prog);
     return 2;
 }
@@ -108,6 +112,13 @@ int run_body(int argc, char **argv, bool pgse)
             o.device = std::atoi(v.c_str());
         } else if (a == "--dir" && next(v)) {
             o.dir = v;
+        } else if (a == "--periodic" && next(v)) {
+            for (char ch : v) {
+                const char *axes = "xyz";
+                const char *q = std::strchr(axes, ch);
+                if (!q || ch == 0) return usage(argv[0]);
+                o.periodic |= 1 << (int)(q - axes);
+            }
         } else {
             return usage(argv[0]);
         }
@@ -162,6 +173,8 @@ int run_body(int argc, char **argv, bool pgse)
         g.T2[c] = T2[c];
     }
     for (int i = 0; i < RMS_NC_MAX * RMS_NC_MAX; i++) g.kappa[i] = -1.0;
+    g.periodic = o.periodic;
+    g.reserved = 0;
     double step[RMS_NC_MAX], T2s[RMS_NC_MAX], P[RMS_NC_MAX * RMS_NC_MAX];
     if (rms_derive(&g, sp.dt, step, T2s, P) != RMS_OK) return die("invalid physical parameters");
     for (int i = 0; i < sp.Nc; i++)

@@ -312,6 +312,7 @@ static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int3
     if (!g.labels) return fail(ctx, RMS_ERR_INVALID_ARG, "labels is null");
     if (g.nclass < 1 || g.nclass > NC) return fail(ctx, RMS_ERR_INVALID_ARG, "nclass must be in 1..4");
     if (!(g.res > 0.0)) return fail(ctx, RMS_ERR_INVALID_ARG, "voxel size must be positive");
+    if (g.periodic & ~7) return fail(ctx, RMS_ERR_INVALID_ARG, "periodic: bits 0..2 (x, y, z) only");
     if (g.label_class) {
         if (g.nlabel < 1) return fail(ctx, RMS_ERR_INVALID_ARG, "nlabel must be >= 1 with a class map");
         for (int i = 0; i < g.nlabel; i++)
@@ -563,6 +564,7 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         D.ny = G.meta.ny;
         D.nz = G.meta.nz;
         D.pg_log2 = pgse_flush_log2(G.meta.nx, G.meta.ny, G.meta.nz);
+        D.periodic = G.meta.periodic;
         D.nclass = G.meta.nclass;
         D.nbvec = J.nbvec;
         D.ngrad = J.ngrad;
@@ -598,6 +600,8 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         bool draws = false;
         for (int i = 0; i < NC * NC; i++) draws = draws || Pcat[i] == 2;
         if (G.meta.nclass == 3 || draws) p->sig |= 4;
+        // the walk kernel whose march wraps periodic lattice edges
+        if (G.meta.periodic != 0) p->sig |= 8;
         H.geom_id = J.geom_id;
         H.nwalk = J.nwalk;
         H.nbatch = (J.nwalk + WAVE - 1) / WAVE;

@@ -24,6 +24,8 @@
 //   setup_kernel              main.cu:58-61   -> counter-based Philox, no state
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "rms_device.h"
 #include "rms_internal.h"
 #include "rms_turn_table.h"
@@ -111,6 +113,7 @@ struct JobLocal {
     const R *P;              // permeation probabilities [NC*NC]
     R P12, P21;              // P between classes 1 and 2
     int32_t xh12, xh21;      // reflect-sure thresholds of the first draw's high half
+    int32_t periodic;        // bit k: axis k wraps around (else the edge is a mirror)
     int32_t exch_fast;       // two non-null classes: a CELL_OTHER face is the other one
     // adjoint constants
     R ds1, ds2, ds3;         // d step_c / d D_c, classes 1..3
@@ -163,6 +166,7 @@ __device__ __forceinline__ JobLocal<R> job_local(const JobDesc &J)
     L.P21 = L.P[2 * NC + 1];
     L.xh12 = J.xh12;
     L.xh21 = J.xh21;
+    L.periodic = J.periodic;
     L.exch_fast = J.nclass <= 3;
     L.T0 = J.T2s[0];
     L.T1 = J.T2s[1];
@@ -229,9 +233,9 @@ __device__ __forceinline__ double origin(int32_t gi, R oi)
     return (double)gi + (double)oi;
 }
 
-// PGSE sums after a mirror moved the origin of one axis from xo to xn: the
-// npg terms since the last flush were taken with xo, the flush subtracts the
-// origin at flush time, so the difference npg (xn - xo) is added now.
+// PGSE sums after a mirror or a periodic wrap moved the origin of one axis
+// from xo to xn: the npg terms since the last flush were taken with xo, the
+// flush subtracts the origin at flush time, so npg (xn - xo) is added now.
 __device__ __forceinline__ double mirror_ps(double ps, int32_t npg, double xo, double xn)
 {
     return ps + (double)npg * (xn - xo);
@@ -410,7 +414,7 @@ __device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLoca
 // and the straight move folded back at the reflecting faces.  Returns false
 // when the step needs the march (a crossing into another class, or three
 // comparisons that are not transitive: a near tie).
-template <typename R, int NP, bool EXCH>
+template <typename R, int NP, bool EXCH, bool PER>
 __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t cell,
                                           const V3<R> &v, uint64_t wid, int32_t i, uint32_t xhi, int32_t npg,
                                           Tan<R, NP> &T)
@@ -465,6 +469,14 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
         // 302-310).
         if (kx == ky) return false;
         const uint32_t edge = rels & (rels >> 1) & 0x15u;  // CELL_EDGE slots
+        const int32_t kz = 3 - kx - ky;
+        if (PER && edge != 0u) {
+            // a periodic edge wraps around: the march crosses into the voxel
+            // on the other side (decided before any state changes)
+            const uint32_t eaxes = ((edge >> (2 * kx)) & 1u) | (((edge >> (2 * ky)) & 1u) << 1) |
+                                   (((edge >> (2 * kz)) & 1u) << 2);
+            if ((eaxes & (uint32_t)L.periodic) != 0u) return false;
+        }
         if (((rels >> 1) & ~rels & 0x15u) != 0u) {  // CELL_OTHER slots
             if constexpr (EXCH) {
                 if (!exchange_faces<R, NP>(w, L, J, rels, wid, i, xhi, T, ndraw)) return false;
@@ -473,7 +485,6 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
             }
         }
         if (edge != 0u) {
-            const int32_t kz = 3 - kx - ky;
             if (((edge >> (2 * kx)) & 1u) != 0u) {
                 const double xo = origin(w.gi.x, w.oi.x);
                 w.gi.x = cx < 0 ? -w.gi.x : 2 * L.nx - w.gi.x;
@@ -547,7 +558,7 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
 // Exchange draws (0 < P < 1 between two non-null classes) exist only in
 // geometries with two non-null classes, which the host runs with EXCH set;
 // without it the draw is not compiled.
-template <typename R, int NP, bool EXCH>
+template <typename R, int NP, bool EXCH, bool PER>
 __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t wid,
                                       int64_t i, V3<R> v, uint32_t xhi, int32_t npg, Tan<R, NP> &T)
 {
@@ -593,6 +604,7 @@ __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const 
     R trav = (R)0;
     int nd = 0;
     bool live = true;
+    uint32_t wraps = 0u;  // periodic crossings: bit 2k (down) or 2k+1 (up) for axis k
     R df[NP > 0 ? NP : 1];
 #pragma unroll
     for (int p = 0; p < (NP > 0 ? NP : 1); p++) df[p] = (R)0;
@@ -639,7 +651,11 @@ __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const 
             const bool up = vh > (R)0;
             const int32_t nk = gh + (up ? 1 : -1);
             bool flip;
-            if (nk < 0 || nk >= Nh) {
+            const bool out = nk < 0 || nk >= Nh;
+            // a periodic axis continues in the voxel on the other side of the
+            // lattice, with the origin moved by one lattice length
+            const bool wrap = PER && out && ((L.periodic >> hit) & 1) != 0;
+            if (out && !wrap) {
                 // lattice-edge mirror, main.cu:249-265; origin unfold main.cu:302-310
                 for3([&](auto K) __attribute__((always_inline)) {
                     constexpr int k = decltype(K)::value;
@@ -654,7 +670,8 @@ __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const 
                 flip = true;
             } else {
                 const int32_t Sh = pick(L.sx, L.nz, 1, hit);
-                const int32_t lin_n = w.lin + (up ? Sh : -Sh);
+                const int32_t nkw = wrap ? (nk < 0 ? Nh - 1 : 0) : nk;
+                const int32_t lin_n = w.lin + (wrap ? (nkw - gh) * Sh : (up ? Sh : -Sh));
                 const int32_t cn = class_at(L, lin_n);  // label gather, main.cu:272
                 const int pidx = w.c * NC + cn;
                 const int cat = (int)((L.pcat >> (2 * pidx)) & 3u);
@@ -675,9 +692,10 @@ __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const 
                     for3([&](auto K) __attribute__((always_inline)) {
                         constexpr int k = decltype(K)::value;
                         const bool h = (k == hit);
-                        at<k>(w.g) = h ? nk : at<k>(w.g);
+                        at<k>(w.g) = h ? nkw : at<k>(w.g);
                         at<k>(w.o) = h ? (up ? (R)0 : (R)1) : at<k>(w.o);
                     });
+                    wraps |= wrap ? (1u << (2 * hit + (up ? 1 : 0))) : 0u;
                     w.lin = lin_n;
                     if (cn != w.c) {
                         flush(w);
@@ -699,6 +717,19 @@ __device__ __forceinline__ void march(Walker<R> &w, const JobLocal<R> &L, const 
                 });
             }
         }
+    }
+    if (PER && wraps != 0u) {
+        // the origin follows each periodic crossing by one lattice length
+        for3([&](auto K) __attribute__((always_inline)) {
+            constexpr int k = decltype(K)::value;
+            const uint32_t wk = (wraps >> (2 * k)) & 3u;
+            if (wk != 0u) {
+                const int32_t Nk = k == 0 ? L.nx : (k == 1 ? L.ny : L.nz);
+                const double xo = origin(at<k>(w.gi), at<k>(w.oi));
+                at<k>(w.gi) = at<k>(w.gi) + (wk == 2u ? -Nk : Nk);
+                at<k>(w.ps) = mirror_ps(at<k>(w.ps), npg, xo, origin(at<k>(w.gi), at<k>(w.oi)));
+            }
+        });
     }
     if (f > (R)0) {
         // finish inside the voxel, main.cu:237-243
@@ -875,7 +906,8 @@ __device__ __forceinline__ void adj_record(const double *cot, int nclass, int nb
 }
 
 // The persistent walk kernel.  SIG: bit 0 narrow-pulse b-table signal,
-// bit 1 PGSE signal.  NP > 0 instantiates the adjoint pass: the same walk
+// bit 1 PGSE signal, bit 2 exchange faces on the fast path, bit 3 periodic
+// lattice edges (the march wraps; without it every edge is a mirror).  NP > 0 instantiates the adjoint pass: the same walk
 // with tangents for D of classes 1..NP, contracted with the cotangents of the
 // record rows and PGSE signal into per-batch parameter gradients.
 //
@@ -983,8 +1015,8 @@ void walk_kernel(LaunchArgs a)
             const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1, tab, xhi);
             const int32_t lin0 = w.lin;
             const int32_t npg = i - i0;  // PGSE terms since the last flush
-            if (!fast_step<R, NP, (SIG & 4) != 0>(w, L, J, cell, v, wid, i, xhi, npg, T))
-                march<R, NP, (SIG & 4) != 0>(w, L, J, wid, i, v, xhi, npg, T);
+            if (!fast_step<R, NP, (SIG & 4) != 0, (SIG & 8) != 0>(w, L, J, cell, v, wid, i, xhi, npg, T))
+                march<R, NP, (SIG & 4) != 0, (SIG & 8) != 0>(w, L, J, wid, i, v, xhi, npg, T);
             if (w.lin != lin0) cell = L.cells[w.lin];  // only lanes that changed voxel read
             if (SIG & 2) {
                 // running sums of the displacement in voxels, (g + o) - (gi + oi),
@@ -1293,33 +1325,20 @@ __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, cons
 }
 
 // ------------------------------------------------------------------ launch
+template <typename R, int NP, int... S>
+static const void *walk_fn_sig(int sig, std::integer_sequence<int, S...>)
+{
+    static const void *const fns[] = {(const void *)walk_kernel<R, S, NP>...};
+    return fns[sig & 15];
+}
+
 template <int NP>
 static const void *walk_fn(int prec_f64, int sig)
 {
     // sig bit 0: narrow-pulse b-table, bit 1: PGSE table, bit 2: exchange
-    // faces on the fast path
-    if (prec_f64) {
-        switch (sig & 7) {
-        case 0: return (const void *)walk_kernel<double, 0, NP>;
-        case 1: return (const void *)walk_kernel<double, 1, NP>;
-        case 2: return (const void *)walk_kernel<double, 2, NP>;
-        case 3: return (const void *)walk_kernel<double, 3, NP>;
-        case 4: return (const void *)walk_kernel<double, 4, NP>;
-        case 5: return (const void *)walk_kernel<double, 5, NP>;
-        case 6: return (const void *)walk_kernel<double, 6, NP>;
-        default: return (const void *)walk_kernel<double, 7, NP>;
-        }
-    }
-    switch (sig & 7) {
-    case 0: return (const void *)walk_kernel<float, 0, NP>;
-    case 1: return (const void *)walk_kernel<float, 1, NP>;
-    case 2: return (const void *)walk_kernel<float, 2, NP>;
-    case 3: return (const void *)walk_kernel<float, 3, NP>;
-    case 4: return (const void *)walk_kernel<float, 4, NP>;
-    case 5: return (const void *)walk_kernel<float, 5, NP>;
-    case 6: return (const void *)walk_kernel<float, 6, NP>;
-    default: return (const void *)walk_kernel<float, 7, NP>;
-    }
+    // faces on the fast path, bit 3: periodic lattice edges
+    if (prec_f64) return walk_fn_sig<double, NP>(sig, std::make_integer_sequence<int, 16>{});
+    return walk_fn_sig<float, NP>(sig, std::make_integer_sequence<int, 16>{});
 }
 
 static const void *walk_fn_np(int prec_f64, int sig, int np)

@@ -29,9 +29,11 @@ class Geometry:
     kappa: permeability between non-null classes (um/ms): None keeps the
     reference rule P = min(1, sqrt(Dj/Di)) (main.cu:444-458), a scalar applies
     to every pair, a matrix per pair.  label_class optionally maps labels to
-    classes (uint16 lattices with many labels)."""
+    classes (uint16 lattices with many labels).  periodic names the axes
+    ("x", "y", "z" in any combination) whose lattice edges wrap around; the
+    others are the reference's mirrors (main.cu:246-265)."""
 
-    def __init__(self, labels, res, D, T2=None, kappa=None, label_class=None):
+    def __init__(self, labels, res, D, T2=None, kappa=None, label_class=None, periodic=""):
         labels = np.asarray(labels)
         if labels.ndim != 3:
             raise ValueError("labels must be a 3-D array")
@@ -58,6 +60,9 @@ class Geometry:
                 K[: kk.shape[0], : kk.shape[1]] = kk
         self.kappa = K
         self.label_class = None if label_class is None else np.ascontiguousarray(label_class, np.uint8)
+        if any(a not in "xyz" for a in periodic):
+            raise ValueError("periodic: axes among 'x', 'y', 'z'")
+        self.periodic = sum(1 << "xyz".index(a) for a in set(periodic))
 
     @property
     def nclass(self):
@@ -81,6 +86,7 @@ class Geometry:
             g.T2[i] = self.T2[i]
         for i in range(capi.NC_MAX * capi.NC_MAX):
             g.kappa[i] = float(self.kappa.flat[i])
+        g.periodic = self.periodic
         return g
 
     def derive(self, dt):

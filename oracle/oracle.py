@@ -32,6 +32,8 @@ class _Geom(C.Structure):
         ("D", C.c_double * NC_MAX),
         ("T2", C.c_double * NC_MAX),
         ("kappa", C.c_double * (NC_MAX * NC_MAX)),
+        ("periodic", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -192,7 +194,7 @@ def sincos(x):
 class Geometry:
     """Label lattice plus per-class physics (the fiber/diffusivity/T2 inputs)."""
 
-    def __init__(self, labels, res, D, T2=None, kappa=None, label_class=None):
+    def __init__(self, labels, res, D, T2=None, kappa=None, label_class=None, periodic=""):
         labels = np.ascontiguousarray(labels)
         if labels.dtype not in (np.uint8, np.uint16):
             labels = labels.astype(np.uint16 if labels.max() > 255 else np.uint8)
@@ -213,6 +215,7 @@ class Geometry:
                 K[: kk.shape[0], : kk.shape[1]] = kk
         self.kappa = K
         self.label_class = None if label_class is None else np.ascontiguousarray(label_class, np.uint8)
+        self.periodic = sum(1 << "xyz".index(a) for a in set(periodic))
 
     @property
     def nclass(self):
@@ -232,6 +235,7 @@ class Geometry:
             g.T2[i] = self.T2[i]
         for i in range(NC_MAX * NC_MAX):
             g.kappa[i] = self.kappa.flat[i]
+        g.periodic = self.periodic
         return g
 
 

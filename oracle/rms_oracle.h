@@ -40,6 +40,8 @@ typedef struct {
     double D[OR_NC_MAX];         /* um^2/ms */
     double T2[OR_NC_MAX];        /* ms, +inf = no relaxation */
     double kappa[OR_NC_MAX * OR_NC_MAX]; /* um/ms, < 0 = diffusivity-ratio rule */
+    int32_t periodic;            /* bit k: axis k wraps around instead of mirroring */
+    int32_t reserved;
 } or_geom;
 
 typedef struct {

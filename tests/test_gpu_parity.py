@@ -13,9 +13,9 @@ def _both(rms, orc, engine, jobs, prec, final_state=True, **kw):
     for jb in jobs:
         g = jb["geom"]
         gg = rms.Geometry(g["labels"], g["res"], g["D"], T2=g.get("T2"), kappa=g.get("kappa"),
-                          label_class=g.get("label_class"))
+                          label_class=g.get("label_class"), periodic=g.get("periodic", ""))
         og = orc.Geometry(g["labels"], g["res"], g["D"], T2=g.get("T2"), kappa=g.get("kappa"),
-                          label_class=g.get("label_class"))
+                          label_class=g.get("label_class"), periodic=g.get("periodic", ""))
         rest = {k: v for k, v in jb.items() if k != "geom"}
         gjobs.append(dict(geom=gg, **rest))
         ojobs.append(dict(geom=og, **rest))
@@ -139,6 +139,26 @@ def test_chunked_and_graph_equal_single_launch(rms, orc, engine, prec):
     _assert_same(gr, ref, final_state=False)
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_periodic_edges_bitexact(rms, orc, engine, prec):
+    """Periodic lattice edges: a crossing wraps to the voxel on the other
+    side (its class decides cross / reflect / draw) and moves the origin by
+    one lattice length, which the PGSE sums follow; x and z periodic, y a
+    mirror.  Single launch and chunks of 250 steps."""
+    print(f"case periodic {prec}", flush=True)
+    lab = random_blobs((10, 9, 11), 3, seed=5)
+    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0, 0.6], T2=[50.0, 80.0, 20.0], kappa=0.05, periodic="xz")
+    grad = pgse_table(fib_dirs(3), [0.5, 1.5], [0.03], 0.01)
+    jobs = [dict(geom=geom, nwalk=640, seed=52, grad=grad, bval=np.array([1.0]), bvec=fib_dirs(1))]
+    base, ref = _both(rms, orc, engine, jobs, prec, dt=2e-4, TN=700, timepoints=35)
+    _assert_same(base, ref)
+    assert base[0]["counters"][2] > 0  # mirrors on y
+    gi = base[0]["fin_gi"]
+    assert np.any((gi[0] < 0) | (gi[0] >= lab.shape[0]))  # origins moved by wraps in x
+    ch, _ = _both(rms, orc, engine, jobs, prec, dt=2e-4, TN=700, timepoints=35, chunk_steps=250)
+    _assert_same(ch, ref)
+
+
 def test_pgse_sums_flush_period_bitexact(rms, orc, engine):
     """The PGSE sums keep sum(g) in int32 between flushes at lobe boundaries
     and every 2^16 steps.  70000 steps cross the 2^16 flush; the z lattice
@@ -161,22 +181,23 @@ def _adj_jobs(rms, orc, jobs):
     for jb in jobs:
         g = jb["geom"]
         mk = lambda M: M.Geometry(g["labels"], g["res"], g["D"], T2=g.get("T2"), kappa=g.get("kappa"),  # noqa: E731
-                                  label_class=g.get("label_class"))
+                                  label_class=g.get("label_class"), periodic=g.get("periodic", ""))
         rest = {k: v for k, v in jb.items() if k != "geom"}
         gj.append(dict(geom=mk(rms), **rest))
         oj.append(dict(geom=mk(orc), **rest))
     return gj, oj
 
 
-@pytest.mark.parametrize("prec,kappa", [("f32", 0.08), ("f64", 0.08), ("f32", 0.0), ("f64", 0.0)])
-def test_adjoint_bitexact(rms, orc, engine, prec, kappa):
+@pytest.mark.parametrize("prec,kappa,periodic", [("f32", 0.08, ""), ("f64", 0.08, ""), ("f32", 0.0, ""),
+                                                 ("f64", 0.0, ""), ("f32", 0.08, "xyz")])
+def test_adjoint_bitexact(rms, orc, engine, prec, kappa, periodic):
     """Tangent pass (pathwise D, score-function kappa, analytic T2) against
     the oracle's adjoint on the same seeds: gradients and their replica
     partials are bit-identical.  kappa = 0: impermeable compartments, the D
     entries are defined; kappa > 0: exchange, D entries NaN on both sides."""
-    print(f"case adjoint {prec} kappa={kappa}", flush=True)
+    print(f"case adjoint {prec} kappa={kappa} periodic={periodic}", flush=True)
     lab = random_blobs((10, 9, 11), 3, seed=3)
-    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0, 0.6], T2=[50.0, 80.0, 20.0], kappa=kappa)
+    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0, 0.6], T2=[50.0, 80.0, 20.0], kappa=kappa, periodic=periodic)
     bval = np.array([0.5, 2.0])
     bvec = fib_dirs(2)
     grad = pgse_table(fib_dirs(3), [0.5, 1.5], [0.03], 0.01)

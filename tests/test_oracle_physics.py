@@ -272,3 +272,46 @@ def test_initial_positions_uniform_over_walkable_voxels(orc, prec):
     assert np.all((oi >= 0) & (oi < 1))
     for k in range(3):
         assert stats.kstest(oi[k], "uniform").pvalue > 1e-3
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_periodic_wall_series(orc, prec):
+    """A single label-0 plane at z = 0 with z periodic: the walkers live
+    between the plane and its periodic image, a gap a = (nz - 1) voxels, so
+    <dz^2>(t) follows the impermeable-plane series for that gap.  With the
+    reference's mirror edge the same lattice is a gap of 2 (nz - 1) voxels
+    (checked too).  x and y are periodic and free: D = D0 along them."""
+    nz = 12
+    lab = np.ones((6, 6, nz), np.uint8)
+    lab[:, :, 0] = 0
+    TN, T = 3000, 30
+    TD = orc.diff_times(DT, TN, T)
+    for periodic, a in (("xyz", (nz - 1) * 0.1), ("", 2 * (nz - 1) * 0.1)):
+        g = orc.Geometry(lab, 0.1, [0.0, 2.0], periodic=periodic)
+        r = orc.run([dict(geom=g, nwalk=8192, seed=11)], DT, TN, T, prec=prec)[0]
+        dz2 = r["dx2"][:, 5] / r["sig0"]
+        rep = r["rec_rep"]
+        se = _se(rep[:, :, 1 + 2 + 5] / rep[:, :, 0])
+        z = (dz2 - _planes_x2(a, 2.0, TD)) / np.maximum(se, 1e-12)
+        assert np.max(np.abs(z[5:])) < 4.5, (periodic, z)
+        D = _D_axes(r, TD)
+        sexy = _se(_rep_D_axes(r, TD, 2))
+        assert np.max(np.abs((D[-10:, :2] - 2.0) / sexy[-10:, :2])) < 4.5
+        assert np.all(r["NPar_count"][:, 0] == 0)
+        if periodic:
+            assert r["counters"][2] == 0  # no mirrors: every edge wraps
+
+
+def test_periodic_free_diffusion_unwrapped(orc):
+    """All-1 lattice, periodic in x, y, z: the displacement keeps growing past
+    the lattice size (the origin moves with each wrap), <x^2> = 2 D0 t."""
+    lab = np.ones((4, 4, 4), np.uint8)
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0], periodic="xyz")
+    TN, T = 3000, 30
+    r = orc.run([dict(geom=g, nwalk=4096, seed=12)], DT, TN, T, prec="f32")[0]
+    TD = orc.diff_times(DT, TN, T)
+    D = _D_axes(r, TD)
+    se = _se(_rep_D_axes(r, TD, 2))
+    assert np.max(np.abs((D[-5:] - 2.0) / se[-5:])) < 4.5
+    assert 2 * 2.0 * TD[-1] > (4 * 0.1) ** 2 * 4  # displacements well beyond the 0.4 um lattice
+    assert r["counters"][2] == 0

@@ -220,9 +220,12 @@ class ReadRMSPGSE(ReadRMS):
         self.NPar_count = _load(os.path.join(root, "NPar_count.txt"))
 
 
-def run_setup(target, pgse=False, seed=None, precision=None, timepoints=None, device=None, timeout=None):
+def run_setup(target, pgse=False, seed=None, precision=None, timepoints=None, device=None, timeout=None,
+              periodic=None):
     """Run rms_main (or rms_main_pgse) with cwd = target, as
-    part1_demo3_simulation.m:85-89 runs ./main_cuda.  Returns stdout."""
+    part1_demo3_simulation.m:85-89 runs ./main_cuda.  periodic: axes whose
+    lattice edges wrap around (e.g. "xy"; default: the reference's mirrors).
+    Returns stdout."""
     exe = os.path.join(BIN, "rms_main_pgse" if pgse else "rms_main")
     if not os.path.exists(exe):
         raise RuntimeError(f"{exe} is missing: build the package first")
@@ -235,6 +238,8 @@ def run_setup(target, pgse=False, seed=None, precision=None, timepoints=None, de
         cmd += ["--timepoints", str(int(timepoints))]
     if device is not None:
         cmd += ["--device", str(int(device))]
+    if periodic:
+        cmd += ["--periodic", str(periodic)]
     r = subprocess.run(cmd, cwd=target, capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError(f"{os.path.basename(exe)} failed ({r.returncode}): {r.stderr.strip()}")

@@ -108,3 +108,15 @@ def test_readrms_normalisation(rms, tmp_path):
     K, D = r.akc_mom([0, 0, 1])
     np.testing.assert_allclose(D[:, 0], r.dx2t[:, 5] / 2 / r.TD)
     np.testing.assert_allclose(K[:, 0], r.dx4t[:, 14] / r.dx2t[:, 5] ** 2 - 3)
+
+
+def test_cli_rejects_unknown_periodic_axis():
+    """--periodic takes axes among x, y, z; anything else is a usage error
+    (exit 2) before any input is read or a device is opened."""
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "monte-carlo-simulation-3d-rms_amd", "bin", "rms_main")
+    if not os.path.exists(exe):
+        pytest.skip("CLI not built")
+    r = subprocess.run([exe, "--periodic", "xq"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "--periodic AXES" in r.stderr

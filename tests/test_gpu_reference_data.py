@@ -62,13 +62,13 @@ def _setup(tmp_path, fileio, lab, res, pgse=False, nwalk=640, TN=2000):
     return d
 
 
-def _engine_jobs(rms, d, lab, pgse):
+def _engine_jobs(rms, d, lab, pgse, periodic=""):
     """The inputs exactly as the executables read them back from text."""
     D = np.loadtxt(os.path.join(d, "diffusivity.txt")).ravel()
     T2 = np.loadtxt(os.path.join(d, "T2.txt")).ravel()
     p = np.loadtxt(os.path.join(d, "simParamInput.txt")).ravel()
     res = p[4] if pgse else p[5]
-    g = rms.Geometry(lab, res, list(D), T2=list(T2))
+    g = rms.Geometry(lab, res, list(D), T2=list(T2), periodic=periodic)
     job = dict(geom=g, nwalk=int(p[2]), seed=77)
     if pgse:
         job["grad"] = np.loadtxt(os.path.join(d, "gradient_time.txt")).ravel().reshape(-1, 6)
@@ -108,14 +108,15 @@ def test_cli_narrow_pulse_end_to_end(rms, engine, tmp_path, lattice):
     np.testing.assert_allclose(r.TD, A.diff_times(dt, TN, 1000), rtol=1e-6)
 
 
-def test_cli_pgse_end_to_end(rms, engine, tmp_path):
-    print("case cli pgse", flush=True)
+@pytest.mark.parametrize("periodic", ["", "xyz"])
+def test_cli_pgse_end_to_end(rms, engine, tmp_path, periodic):
+    print(f"case cli pgse periodic={periodic}", flush=True)
     fileio = importlib.import_module(rms.__name__ + ".fileio")
     lab, res = _fixture("fiber_IV_000.npz")
     d = _setup(tmp_path, fileio, lab, res, pgse=True)
-    fileio.run_setup(d, pgse=True, seed=77, timeout=120)
+    fileio.run_setup(d, pgse=True, seed=77, timeout=120, periodic=periodic)
     r = fileio.ReadRMSPGSE(d)
-    job, dt, TN = _engine_jobs(rms, d, lab, pgse=True)
+    job, dt, TN = _engine_jobs(rms, d, lab, pgse=True, periodic=periodic)
     e = engine.run([job], dt, TN, 1000, precision="f64", replicas=1)[0]
     np.testing.assert_allclose(r.sig, np.abs(e["sig"]), rtol=1e-14)
     np.testing.assert_allclose(r.sig0, e["sig0"], rtol=1e-14)

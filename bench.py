@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--cpu-tn", type=int, default=0, help="CPU baseline steps (0 = same as --tn)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay the run as a captured hipGraph")
+    ap.add_argument("--periodic", default="", help="axes of the single-axon lattice whose edges wrap around "
+                    "(default: the reference's mirrors)")
     ap.add_argument("--forms", default="plain,batched,gradient",
                     help="comma list of forms to time beside the plain form (also: sweep)")
     ap.add_argument("--batched-walkers", type=int, default=10_000_000)
@@ -129,7 +131,7 @@ def main():
     grad = pgse_table_c2()
     TN = args.tn
     timepoints = min(1000, TN)
-    geom = rms.Geometry(lab, res, [0.0, D0], T2=[math.inf, math.inf])
+    geom = rms.Geometry(lab, res, [0.0, D0], T2=[math.inf, math.inf], periodic=args.periodic)
     s_vox = math.sqrt(6 * dt * D0) / res
     ops = ops_per_step(s_vox, pgse=True)
     job = dict(geom=geom, nwalk=args.walkers, seed=0x5EED0001, grad=grad)
@@ -176,6 +178,7 @@ def main():
             "workload": "single beaded undulating axon, 64x64x1000 uint8 lattice at 0.1 um, "
                         f"{args.walkers} walkers x {TN} steps, PGSE 360 rows, plain form",
             "walkers": args.walkers, "TN": TN, "dt_ms": dt, "D0": D0, "records": timepoints,
+            "lattice_edges": f"periodic {args.periodic}" if args.periodic else "mirror (reference)",
             "positions": args.precision + " in-voxel offsets + int32 voxel", "accumulation": "fp64",
             "graph": bool(args.graph), "grid": plan.timing["grid"], "block": plan.timing["block"],
         },
@@ -440,7 +443,7 @@ def cpu_baseline(lab, res, D0, dt, grad, args):
 
     nthreads = min(16, os.cpu_count() or 1)
     tn = args.cpu_tn or args.tn
-    g = orc.Geometry(lab, res, [0.0, D0])
+    g = orc.Geometry(lab, res, [0.0, D0], periodic=args.periodic)
     job = dict(geom=g, nwalk=args.cpu_walkers, seed=0x5EED0001, grad=grad)
     t0 = time.perf_counter()
     orc.run([job], dt, tn, min(1000, tn), prec="f64", replicas=32, nthreads=nthreads)

@@ -49,7 +49,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the walk kernel on this workload, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE passes scaled by a 1 GiB device copy (x2.0 reads, x1.0 writes); the summary with the
 # commands is profiles/walk_kernel_axon_pgse_f32.md.  Keyed by (precision, walkers, steps).
-MEASURED_TRAFFIC = {("f32", 1_000_000, 100_000): 2.014e10}
+MEASURED_TRAFFIC = {("f32", 1_000_000, 100_000): 2.007e10}
 
 
 def ops_per_step(s, pgse=True):

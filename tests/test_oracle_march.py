@@ -224,6 +224,44 @@ def test_fast_path_equals_march(orc):
     np.testing.assert_allclose(f["o"], [0.5 + step * 0.6, 0.5, 0.5 + step * 0.8], atol=1e-15)
 
 
+def test_fast_path_equals_march_periodic(orc):
+    """Periodic x and z, mirror y: the fast path hands every step that meets a
+    periodic lattice side to the march (which wraps the voxel index and moves
+    the origin by one lattice length) and otherwise equals the march: same end
+    voxel, counts and origin on random states, same position up to rounding."""
+    rng = np.random.default_rng(12)
+    lab = rng.integers(0, 3, (5, 6, 4)).astype(np.uint8)
+    lab[rng.random(lab.shape) < 0.5] = 1
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0, 2.0], T2=[np.inf] * 3, kappa=0.0, periodic="xz")
+    dt = 2e-4
+    applied = wrapped = 0
+    for _ in range(4000):
+        while True:
+            gv = rng.integers(0, [5, 6, 4])
+            cls = int(lab[tuple(gv)])
+            if cls != 0:
+                break
+        o = rng.random(3)
+        u1, u2 = rng.random(2)
+        sth = 2 * np.sqrt(u1 * (1 - u1))
+        v = np.array([sth * np.cos(2 * np.pi * u2), sth * np.sin(2 * np.pi * u2), 1 - 2 * u1])
+        gi, oi = rng.integers(-3, 9, 3), rng.random(3)
+        f = orc.fast_step_f64(g, dt, gv, o, v, cls, gi=gi, oi=oi)
+        m = orc.march_f64(g, dt, gv, o, v, cls, gi=gi, oi=oi)
+        assert m["gi"][0] - gi[0] in (-5, 0, 5) and m["gi"][2] - gi[2] in (-4, 0, 4)  # wraps, not mirrors
+        assert np.all((m["g"] >= 0) & (m["g"] < [5, 6, 4]))
+        wrapped += int(m["gi"][0] != gi[0] or m["gi"][2] != gi[2])
+        if f is None:
+            continue
+        applied += 1
+        np.testing.assert_array_equal(f["g"], m["g"])
+        np.testing.assert_allclose(f["o"], m["o"], atol=1e-12)
+        np.testing.assert_array_equal(f["cnt"][:3], m["cnt"][:3])
+        np.testing.assert_array_equal(f["gi"], m["gi"])
+        np.testing.assert_array_equal(f["oi"], m["oi"])
+    assert applied > 0.3 * 4000 and wrapped > 100
+
+
 def test_fast_path_three_faces_in_order(orc):
     """Steps that cross three faces next to random null voxels of one octant:
     the fast path meets them in the order of the face distances and agrees

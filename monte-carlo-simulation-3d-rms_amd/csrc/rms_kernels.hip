@@ -1001,7 +1001,9 @@ void walk_kernel(LaunchArgs a)
         if (SIG & 2) {
             while (kb <= nbound && J.bounds[kb] <= a.step0) kb++;
             bnext = __builtin_amdgcn_readfirstlane(kb <= nbound ? (int32_t)J.bounds[kb] : -1);
-            const int32_t lastb = (int32_t)J.bounds[kb - 1];
+            // bounds[0] = 0; a job without a PGSE table (nbound = 0) in a plan
+            // whose other jobs have one has no bounds array at all
+            const int32_t lastb = kb > 1 ? (int32_t)J.bounds[kb - 1] : 0;
             i0 = __builtin_amdgcn_readfirstlane(lastb > i0 ? lastb : i0);
         }
         uint32_t b0 = 0, b1 = 0;

@@ -123,6 +123,25 @@ def test_batched_jobs_bitexact_and_match_single(rms, orc, engine):
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_mixed_tables_in_one_plan_bitexact(rms, orc, engine, prec):
+    """One plan with a PGSE job, a job without tables and a narrow-pulse job:
+    the plan runs the PGSE kernel for all three, and a job without a PGSE
+    table has no lobe boundaries to read.  Single launch and chunks."""
+    print(f"case mixed tables {prec}", flush=True)
+    grad = pgse_table(fib_dirs(3), [0.5, 1.5], [0.03], 0.01)
+    g1 = dict(labels=cylinder(10, 5, 4.0), res=0.1, D=[0.0, 2.0], T2=[80.0, 80.0], periodic="z")
+    g2 = dict(labels=random_blobs((5, 3, 12), 3, seed=9), res=0.1, D=[0.0, 2.0, 0.7], T2=[60.0, 60.0, 80.0],
+              kappa=0.5)
+    g3 = dict(labels=random_blobs((3, 3, 8), 2, seed=10), res=0.1, D=[0.0, 1.2], T2=[50.0, 50.0])
+    jobs = [dict(geom=g1, nwalk=112, seed=1, grad=grad), dict(geom=g2, nwalk=16, seed=2),
+            dict(geom=g3, nwalk=100, seed=3, bval=np.array([1.0, 2.0]), bvec=fib_dirs(2))]
+    base, ref = _both(rms, orc, engine, jobs, prec, dt=2e-4, TN=242, timepoints=8)
+    _assert_same(base, ref)
+    ch, _ = _both(rms, orc, engine, jobs, prec, dt=2e-4, TN=242, timepoints=8, chunk_steps=60)
+    _assert_same(ch, ref)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_chunked_and_graph_equal_single_launch(rms, orc, engine, prec):
     print(f"case chunked/graph {prec}", flush=True)
     lab = random_blobs((8, 8, 8), 3, seed=4)

@@ -82,3 +82,35 @@ def test_random_inputs_adjoint_bitexact(rms, orc, engine):
         b = orc.run([dict(geom=og, **job)], 2e-4, TN, T, prec=prec, replicas=4, cot_rec=cot_rec, cot_sig=cot_sig)[0]
         np.testing.assert_array_equal(a["grad"], b["grad"], err_msg=desc)
         np.testing.assert_array_equal(a["grad_rep"], b["grad_rep"], err_msg=desc)
+
+
+def test_random_multi_job_plans_bitexact(rms, orc, engine):
+    """60 seeded random plans of 2-5 random jobs each (mixed classes, tables,
+    periodic axes and lattice sizes in one launch): every job bit-exact
+    against the oracle and its records equal to the job run alone."""
+    rng = np.random.default_rng(20261206)
+    for k in range(60):
+        nj = int(rng.integers(2, 6))
+        prec = "f32" if rng.random() < 0.6 else "f64"
+        TN = int(rng.integers(20, 400))
+        T = int(rng.integers(1, min(TN, 40) + 1))
+        chunk = int(rng.choice([0, int(rng.integers(7, 200))]))
+        gj, oj = [], []
+        for _ in range(nj):
+            lab, D, T2, kappa, periodic, _, _, _, job, _ = _case(rng)
+            gj.append(dict(geom=rms.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic), **job))
+            oj.append(dict(geom=orc.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic), **job))
+        desc = (f"plan {k}: {nj} jobs {prec} TN={TN} T={T} chunk={chunk} "
+                f"shapes={[j['geom'].labels.shape for j in gj]} tables={[sorted(j) for j in gj]}")
+        print(desc, flush=True)
+        a = engine.run(gj, 2e-4, TN, T, precision=prec, replicas=4, final_state=True, chunk_steps=chunk)
+        b = orc.run(oj, 2e-4, TN, T, prec=prec, replicas=4, final_state=True)
+        for j in range(nj):
+            for key in ("counters", "fin_g", "fin_gi", "fin_c", "fin_o", "fin_oi", "rec", "rec_rep"):
+                np.testing.assert_array_equal(np.asarray(a[j][key]), np.asarray(b[j][key]),
+                                              err_msg=f"job {j} {key}: {desc}")
+            if b[j]["sig"] is not None:
+                np.testing.assert_array_equal(a[j]["sig"], b[j]["sig"], err_msg=f"job {j}: {desc}")
+        j = int(rng.integers(0, nj))
+        alone = engine.run([gj[j]], 2e-4, TN, T, precision=prec, replicas=4)[0]
+        np.testing.assert_array_equal(alone["rec"], a[j]["rec"], err_msg=f"alone job {j}: {desc}")

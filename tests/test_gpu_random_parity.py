@@ -86,8 +86,9 @@ def test_random_inputs_adjoint_bitexact(rms, orc, engine):
 
 def test_random_multi_job_plans_bitexact(rms, orc, engine):
     """60 seeded random plans of 2-5 random jobs each (mixed classes, tables,
-    periodic axes and lattice sizes in one launch): every job bit-exact
-    against the oracle and its records equal to the job run alone."""
+    periodic axes and lattice sizes in one launch, random launch shapes and
+    chunks): every job bit-exact against the oracle and its records equal to
+    the job run alone."""
     rng = np.random.default_rng(20261206)
     for k in range(60):
         nj = int(rng.integers(2, 6))
@@ -95,15 +96,17 @@ def test_random_multi_job_plans_bitexact(rms, orc, engine):
         TN = int(rng.integers(20, 400))
         T = int(rng.integers(1, min(TN, 40) + 1))
         chunk = int(rng.choice([0, int(rng.integers(7, 200))]))
+        shape = dict(block_threads=int(rng.choice([0, 64, 128, 192, 256])),
+                     grid_blocks=int(rng.choice([0, int(rng.integers(1, 3000))])), xcd_split=int(rng.choice([-1, 0, 1])))
         gj, oj = [], []
         for _ in range(nj):
             lab, D, T2, kappa, periodic, _, _, _, job, _ = _case(rng)
             gj.append(dict(geom=rms.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic), **job))
             oj.append(dict(geom=orc.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic), **job))
-        desc = (f"plan {k}: {nj} jobs {prec} TN={TN} T={T} chunk={chunk} "
+        desc = (f"plan {k}: {nj} jobs {prec} TN={TN} T={T} chunk={chunk} launch={shape} "
                 f"shapes={[j['geom'].labels.shape for j in gj]} tables={[sorted(j) for j in gj]}")
         print(desc, flush=True)
-        a = engine.run(gj, 2e-4, TN, T, precision=prec, replicas=4, final_state=True, chunk_steps=chunk)
+        a = engine.run(gj, 2e-4, TN, T, precision=prec, replicas=4, final_state=True, chunk_steps=chunk, **shape)
         b = orc.run(oj, 2e-4, TN, T, prec=prec, replicas=4, final_state=True)
         for j in range(nj):
             for key in ("counters", "fin_g", "fin_gi", "fin_c", "fin_o", "fin_oi", "rec", "rec_rep"):

@@ -195,6 +195,23 @@ def test_pgse_sums_flush_period_bitexact(rms, orc, engine):
     _assert_same(ch, ref)
 
 
+def test_pgse_flush_periods_differ_between_jobs_bitexact(rms, orc, engine):
+    """A 3 x 3 x 40000 lattice flushes its int32 sums every 2^15 steps (2^16
+    would let sum(g) reach 2^31), a small cylinder every 2^16: both in one
+    plan over 70000 steps, single launch and chunks of 25000."""
+    print("case pgse flush periods per job f32", flush=True)
+    tube = np.ones((3, 3, 40000), np.uint8)
+    tube[1, 1, ::7] = 0
+    grad = pgse_table(fib_dirs(2), [1.0], [5.0], 2.0)
+    g1 = dict(labels=tube, res=0.1, D=[0.0, 2.0], T2=[80.0, 80.0], periodic="z")
+    g2 = dict(labels=cylinder(10, 6, 4.0), res=0.1, D=[0.0, 2.0], T2=[80.0, 80.0])
+    jobs = [dict(geom=g1, nwalk=64, seed=71, grad=grad), dict(geom=g2, nwalk=64, seed=72, grad=grad)]
+    base, ref = _both(rms, orc, engine, jobs, "f32", dt=2e-4, TN=70000, timepoints=7)
+    _assert_same(base, ref)
+    ch, _ = _both(rms, orc, engine, jobs, "f32", dt=2e-4, TN=70000, timepoints=7, chunk_steps=25000)
+    _assert_same(ch, ref)
+
+
 def _adj_jobs(rms, orc, jobs):
     gj, oj = [], []
     for jb in jobs:

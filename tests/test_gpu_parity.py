@@ -362,3 +362,20 @@ def test_autotuned_launch_gives_identical_results(rms, engine):
         np.testing.assert_array_equal(tuned[pos]["rec"], base[k]["rec"])
         np.testing.assert_array_equal(tuned[pos]["rec_rep"], base[k]["rec_rep"])
         np.testing.assert_array_equal(tuned[pos]["counters"], base[k]["counters"])
+
+
+def test_near_max_lattice_bitexact(rms, orc, engine):
+    """The largest lattice the host accepts is below 2^31 voxels (the
+    reference's int index): 1200 x 1200 x 1491 = 2.147e9 voxels (2 GB of
+    labels, 17 GB of cell words), label-0 planes every 100 slices, periodic
+    x.  Voxel indices near 2^31 - 1 in the walk, the cell build and the
+    march stay in range: bit-exact against the oracle."""
+    print("case near-max lattice 1200x1200x1491 f32", flush=True)
+    lab = np.ones((1200, 1200, 1491), np.uint8)
+    lab[:, :, ::100] = 0
+    assert lab.size < 2 ** 31 and lab.size > 2 ** 31 - 2 ** 21
+    geom = dict(labels=lab, res=0.1, D=[0.0, 2.0], T2=[80.0, 80.0], periodic="x")
+    jobs = [dict(geom=geom, nwalk=256, seed=2147, bval=np.array([1.0]), bvec=fib_dirs(1))]
+    gpu, ref = _both(rms, orc, engine, jobs, "f32", dt=2e-4, TN=300, timepoints=10)
+    _assert_same(gpu, ref)
+    assert int(np.max(gpu[0]["fin_g"][0])) > 1000 and int(np.max(gpu[0]["fin_g"][2])) > 1400

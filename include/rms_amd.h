@@ -79,6 +79,12 @@ typedef struct {
     const double *bvec;          /* [nbvec*3] */
     int32_t ngrad;               /* PGSE table (gradient_time.txt), 0 = none */
     const double *grad;          /* [ngrad*6] rows: Delta, delta, |g|, gx, gy, gz */
+    /* id of the job's first walker in the Philox counter (reference: one
+     * walker per thread index, main.cu:64-70).  0 for a whole run; a shard of
+     * a larger run passes its first global walker id, a multiple of
+     * 64 * replicas, so its walkers follow the same trajectories and its
+     * batches fall into the same replicas as in the unsharded run. */
+    int64_t walker_base;
 } rms_job;
 
 typedef struct {
@@ -103,7 +109,8 @@ typedef struct {
     double *rec_rep;             /* [replicas * timepoints * nval] */
     double *sig;                 /* [ngrad] PGSE signal (sig.txt, main_PGSE.cu:648-651) */
     double *sig_rep;             /* [replicas * ngrad] */
-    uint64_t counters[5];        /* crossings, reflections, edge mirrors, truncated steps, exchange draws */
+    /* crossings, reflections, edge mirrors, truncated steps, exchange draws */
+    uint64_t counters[5];
     /* optional final walker state, axis-major [3][nwalk] etc. */
     int32_t *fin_g;
     double *fin_o;
@@ -128,7 +135,10 @@ typedef struct {
  * Estimators: T2 analytic (weights only); kappa score function of the
  * exchange draws; D pathwise (faces fixed, step ~ sqrt(D)).  The pathwise D
  * derivative is exact only while no walker can change class: for a geometry
- * with exchange between non-null classes the D entries are NaN. */
+ * with exchange between non-null classes the D entries are NaN.  The kappa
+ * entry is NaN when a pair of non-null classes is impermeable (P < 1e-6)
+ * only because of its kappa: no draw happens there, so the score function
+ * has nothing to sum although P grows with kappa. */
 typedef struct {
     const double *cot_rec;       /* [timepoints * nval] or NULL */
     const double *cot_sig;       /* [ngrad] or NULL */

@@ -11,9 +11,11 @@ Modules:
   geometry  synthetic planes, cylinders, beaded/undulating axons, bundles
   autograd  torch.autograd.Function over the adjoint pass
   autotune  launch-shape and packing search
+  workloads the five benchmark workloads as job lists
   build     hipcc recipe
 """
 from . import _capi  # noqa: F401
+from . import workloads  # noqa: F401
 from .engine import Engine, Geometry, Plan, nval, split_records  # noqa: F401
 
 __all__ = ["Engine", "Geometry", "Plan", "nval", "split_records"]

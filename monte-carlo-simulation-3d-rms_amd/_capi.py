@@ -64,6 +64,7 @@ class Job(C.Structure):
         ("bvec", C.c_void_p),
         ("ngrad", C.c_int32),
         ("grad", C.c_void_p),
+        ("walker_base", C.c_int64),
     ]
 
 

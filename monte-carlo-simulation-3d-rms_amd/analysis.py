@@ -4,8 +4,8 @@ dki_shell / akc_shell / dki_lls restate readRMS.m:51-112 (signal-based
 tensor fits); akc_lls needs the DKI class the reference does not ship.
 
 readRMS.m:13-18 define the multiplicities of the 6 second-order and 15
-fourth-order moment components; readRMS.m:115-117 normalise by sig0;
-akc_mom (readRMS.m:168-175) gives D(t) = x2/(2t) and K(t) = x4/x2^2 - 3 along
+fourth-order moment components; readRMS.m:44-45 normalise by sig0;
+akc_mom (readRMS.m:96-103) gives D(t) = x2/(2t) and K(t) = x4/x2^2 - 3 along
 unit directions n.  part1_demo4_analysis.m:88-90 combines compartments by
 volume fraction.  Standard errors come from the batch-means replicas the
 engine returns (walker batches b = rho mod R).
@@ -39,13 +39,13 @@ def diff_times(dt, TN, timepoints):
 
 
 def moments(sig0, dx2, dx4):
-    """dx2t, dx4t of readRMS.m:115-117 (multiplicity-weighted, /sig0)."""
+    """dx2t, dx4t of readRMS.m:44-45 (multiplicity-weighted, /sig0)."""
     sig0 = np.asarray(sig0, float)[..., None]
     return np.asarray(dx2) * D_CNT / sig0, np.asarray(dx4) * W_CNT / sig0
 
 
 def akc_mom(sig0, dx2, dx4, TD, n):
-    """[K, D] along directions n (readRMS.m:168-175).  Returns arrays of shape
+    """[K, D] along directions n (readRMS.m:96-103).  Returns arrays of shape
     (timepoints, ndir)."""
     dx2t, dx4t = moments(sig0, dx2, dx4)
     x2 = dx2t @ ndir2(n).T
@@ -148,7 +148,7 @@ def dki_lls(sig, bvec, bval):
 
 
 def narrow_pulse_signal(sig0, sigRe):
-    """sig = |sigRe| / sig0 (readRMS.m:119-120)."""
+    """sig = |sigRe| / sig0 (readRMS.m:47-48)."""
     return np.abs(np.asarray(sigRe)) / np.asarray(sig0, float)[:, None]
 
 

@@ -11,11 +11,14 @@ incoming cotangents on the device.
 
 Estimators used by the adjoint pass:
   D      pathwise derivative of the displacement (faces fixed, step length
-         d s_c / d D_c = s_c / (2 D_c)) plus the score of the exchange draws
-         whose probability depends on D (P = min(1, sqrt(Dj/Di)) ...);
+         d s_c / d D_c = s_c / (2 D_c)); NaN when walkers can cross between
+         two non-null classes (a class change at a crossing is a jump the
+         pathwise derivative does not contain);
   T2     analytic: d/dT2_c exp(-sum t_c/T2_c) = (t_c / T2_c^2) exp(...);
   kappa  score function of the Bernoulli exchange draws,
-         d log P / d kappa = 1 / (kappa (1 + pi0)).
+         d log P / d kappa = 1 / (kappa (1 + pi0)); NaN when a pair of
+         non-null classes is impermeable only because of kappa (kappa = 0),
+         where no draw happens.
 Class 0 (the null compartment) has no gradient.  The inputs are host tensors
 (float64); the simulation itself always runs on the engine's device.
 """

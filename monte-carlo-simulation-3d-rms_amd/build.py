@@ -47,7 +47,8 @@ def build_lib(verbose=False):
     tmp = LIB + ".tmp"
     cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp,
            os.path.join(CSRC, "rms_kernels.hip"),
-           os.path.join(CSRC, "rms_api.cpp")]
+           os.path.join(CSRC, "rms_api.cpp"),
+           os.path.join(CSRC, "rms_host.cpp")]
     out = _run(cmd)
     os.replace(tmp, LIB)
     if verbose and out:

@@ -7,17 +7,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <map>
 #include <new>
 #include <string>
-#include <tuple>
 #include <vector>
 
 #include "../../include/rms_amd.h"
+#include "rms_host.h"
 #include "rms_internal.h"
 #include "rms_types.h"
 
@@ -41,17 +39,14 @@ struct rms_ctx {
 
 namespace {
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t n = 0;
-};
-
 struct JobHost {
     bool exchange;      // walkers can cross between different non-null classes
+    bool kappa_closed;  // a pair of non-null classes is impermeable because of its kappa
     int32_t geom_id;
     int64_t nwalk, nbatch, batch0, wbase;
     int32_t nbvec, ngrad, nval, nbound, ntriple;
-    std::vector<int32_t> gperm;
+    // offsets (doubles) of the job's totals and replica sums in the results block
+    int64_t rec_tot, rec_rep, sig_tot, sig_rep;
 };
 
 }  // namespace
@@ -70,7 +65,12 @@ struct rms_plan {
     ReduceDesc *d_red = nullptr;
     std::vector<ReduceDesc> red;   // host copy (device pointers)
     int64_t max_nq = 0;
-    unsigned long long *d_counters = nullptr;
+    unsigned long long *d_counters = nullptr;  // [njobs][NCOUNT], inside the results block
+    // results block: [totals of every reduction][counters][replica sums],
+    // copied back by one asynchronous copy into its pinned host mirror
+    double *d_res = nullptr;
+    double *h_res = nullptr;
+    int64_t res_head = 0, res_all = 0;  // bytes up to the replica sums / in total
     StateBuf st{};
     double *d_snap = nullptr;
     FinalOut fin{};
@@ -88,6 +88,8 @@ struct rms_plan {
     std::vector<ReduceDesc> red_adj;
     std::vector<double *> cot_rec, cot_sig;
     double *d_tsnap = nullptr;
+    double *d_adj_res = nullptr;  // [njobs][NPARAM] totals, then [njobs][R][NPARAM]
+    double *h_adj_res = nullptr;
 };
 
 static const char *kVersion = "rms_amd 0.1 (gfx950)";
@@ -147,6 +149,7 @@ static rms_status create_impl(const rms_device_opts *opts, rms_ctx **out)
     if (dev < 0 || dev >= ndev) return RMS_ERR_INVALID_ARG;
     rms_ctx *c = new rms_ctx();
     c->device = dev;
+    c->err.reserve(1024);
     if (hipSetDevice(dev) != hipSuccess) {
         delete c;
         return RMS_ERR_HIP;
@@ -185,119 +188,20 @@ extern "C" rms_status rms_destroy(rms_ctx *ctx)
 
 extern "C" const char *rms_last_error(const rms_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-// main.cu:429-458 (+ the kappa extension); identical double arithmetic to the
-// host restatement used for parity.
-static bool derive(const rms_geometry &g, double dt, double step[NC], double inv_step[NC], double T2s[NC],
-                   double P[NC * NC], int Pcat[NC * NC], std::string &err)
-{
-    if (g.nclass < 1 || g.nclass > NC) {
-        err = "nclass must be in 1..4";
-        return false;
-    }
-    for (int i = 0; i < NC; i++) {
-        step[i] = inv_step[i] = 0.0;
-        T2s[i] = INFINITY;
-    }
-    for (int i = 0; i < NC * NC; i++) {
-        P[i] = 0.0;
-        Pcat[i] = 0;
-    }
-    for (int i = 0; i < g.nclass; i++) {
-        T2s[i] = g.T2[i] / dt;
-        step[i] = std::sqrt(6.0 * dt * g.D[i]) / g.res;
-        inv_step[i] = (step[i] > 0.0) ? 1.0 / step[i] : 0.0;
-    }
-    for (int i = 0; i < g.nclass; i++) {
-        for (int j = 0; j < g.nclass; j++) {
-            double p;
-            if ((i == 0) || (j == 0)) {
-                p = 0.0;
-            } else if (i == j) {
-                p = 1.0;
-            } else {
-                p = std::fmin(1.0, std::sqrt(g.D[j] / g.D[i]));
-                double kap = g.kappa[i * NC + j];
-                if (kap >= 0.0) {
-                    double smin = std::fmin(step[i], step[j]);
-                    double pi0 = 4.0 * kap * dt / (g.res * smin);
-                    p = p * (pi0 / (1.0 + pi0));
-                }
-            }
-            P[i * NC + j] = p;
-            Pcat[i * NC + j] = (p > 0.999999) ? 1 : ((p < 0.000001) ? 0 : 2);
-        }
-    }
-    return true;
-}
-
 static rms_status derive_impl(const rms_geometry *geom, double dt, double *step, double *T2s, double *P)
 {
     if (!geom || !(dt > 0.0)) return RMS_ERR_INVALID_ARG;
-    double st[NC], ist[NC], t2[NC], pp[NC * NC];
-    int pc[NC * NC];
-    std::string err;
-    if (!derive(*geom, dt, st, ist, t2, pp, pc, err)) return RMS_ERR_INVALID_ARG;
-    if (step) std::memcpy(step, st, sizeof(st));
-    if (T2s) std::memcpy(T2s, t2, sizeof(t2));
-    if (P) std::memcpy(P, pp, sizeof(pp));
+    Derived d;
+    if (derive(*geom, dt, d)) return RMS_ERR_INVALID_ARG;
+    if (step) std::memcpy(step, d.step, sizeof(d.step));
+    if (T2s) std::memcpy(T2s, d.T2s, sizeof(d.T2s));
+    if (P) std::memcpy(P, d.P, sizeof(d.P));
     return RMS_OK;
 }
 
 extern "C" rms_status rms_derive(const rms_geometry *geom, double dt, double *step, double *T2s, double *P)
 {
     return guard(nullptr, [&] { return derive_impl(geom, dt, step, T2s, P); });
-}
-
-// Constants of the tangent (adjoint) pass, from the same quantities as
-// derive(): parameter p < 3 is D of class p+1, p = 3 is kappa.  For the
-// probabilistic pairs P = min(1, sqrt(Dj/Di)) * pi0/(1+pi0) with
-// pi0 = 4 kappa dt / (res min(s_i, s_j)); d log P and d log(1-P) = -P/(1-P) d log P.
-static void derive_tangent(const rms_geometry &g, double dt, const double step[NC], const double inv_step[NC],
-                           const double T2s[NC], const double P[NC * NC], const int Pcat[NC * NC], JobDesc &D)
-{
-    for (int c = 0; c < NC; c++) {
-        D.dstep_d[c] = D.dinv_d[c] = D.rT[c] = D.invT2[c] = 0.0;
-        D.dstep_f[c] = D.dinv_f[c] = 0.0f;
-    }
-    for (int i = 0; i < NC * NC * 4; i++) D.dlogP[i] = D.dlog1mP[i] = 0.0;
-    for (int c = 0; c < g.nclass; c++) {
-        const double ds = (g.D[c] > 0.0) ? step[c] / (2.0 * g.D[c]) : 0.0;
-        const double di = (g.D[c] > 0.0) ? -inv_step[c] / (2.0 * g.D[c]) : 0.0;
-        D.dstep_d[c] = ds;
-        D.dinv_d[c] = di;
-        D.dstep_f[c] = (float)ds;
-        D.dinv_f[c] = (float)di;
-        D.rT[c] = 1.0 / T2s[c];
-        D.invT2[c] = 1.0 / g.T2[c];
-    }
-    for (int i = 1; i < g.nclass; i++) {
-        for (int j = 1; j < g.nclass; j++) {
-            const int ij = i * NC + j;
-            if (Pcat[ij] != 2) continue;
-            double dl[3] = {0.0, 0.0, 0.0};
-            double dk = 0.0;
-            const double ratio = std::sqrt(g.D[j] / g.D[i]);
-            if (ratio < 1.0) {
-                dl[j - 1] = dl[j - 1] + 0.5 / g.D[j];
-                dl[i - 1] = dl[i - 1] - 0.5 / g.D[i];
-            }
-            const double kap = g.kappa[ij];
-            if (kap >= 0.0) {
-                const double smin = std::fmin(step[i], step[j]);
-                const double pi0 = 4.0 * kap * dt / (g.res * smin);
-                const int m = (step[i] <= step[j]) ? i : j;
-                dl[m - 1] = dl[m - 1] - 0.5 / (g.D[m] * (1.0 + pi0));
-                dk = (kap > 0.0) ? 1.0 / (kap * (1.0 + pi0)) : 0.0;
-            }
-            const double r = -P[ij] / (1.0 - P[ij]);
-            for (int q = 0; q < 3; q++) {
-                D.dlogP[ij * 4 + q] = dl[q];
-                D.dlog1mP[ij * 4 + q] = r * dl[q];
-            }
-            D.dlogP[ij * 4 + 3] = dk;
-            D.dlog1mP[ij * 4 + 3] = r * dk;
-        }
-    }
 }
 
 static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id)
@@ -409,65 +313,6 @@ static rms_status pupload(rms_plan *p, T **ptr, const T *src, size_t n)
     return RMS_OK;
 }
 
-// The PGSE sums keep sum(g) of the steps since the last flush in int32; with
-// voxel indices below max(nx, ny, nz) a flush every 2^k steps, k the largest
-// value <= 16 with 2^k max(nx, ny, nz) < 2^31, keeps it in range.
-static int32_t pgse_flush_log2(int32_t nx, int32_t ny, int32_t nz)
-{
-    const int64_t m = std::max<int64_t>(1, std::max<int64_t>(nx, std::max<int64_t>(ny, nz)));
-    int32_t k = 16;
-    while (k > 0 && (m << k) > 2147483647LL) k--;
-    return k;
-}
-
-// Threshold on the high half xhi of an FP32 step's first exchange draw: the
-// draw's uniform is at least u01_f32(xhi << 16) = (256 xhi + 1) 2^-24, so
-// xhi >= the smallest xhi with that bound >= P reflects whatever the low half.
-// -1: the face reflects without a draw (P = 0); 65536: never decided by xhi
-// alone (P = 1 crosses, and with more than two non-null classes the relation
-// word does not name the other class).
-static int32_t exchange_threshold(int32_t nclass, int cat, float P)
-{
-    if (nclass != 3 || cat == 1) return 65536;
-    if (cat == 0) return -1;
-    int32_t lo = 0, hi = 65536;  // smallest xhi in [0, 65536) with the bound >= P, else 65536
-    while (lo < hi) {
-        const int32_t mid = (lo + hi) / 2;
-        const float umin = (2.0f * (float)(mid * 128) + 1.0f) * 5.9604644775390625e-08f;
-        if (umin >= P)
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    return lo;
-}
-
-static int64_t first_false_lt(double dt, int64_t TN, double thr)
-{
-    int64_t lo = 0, hi = TN;
-    while (lo < hi) {
-        int64_t mid = lo + (hi - lo) / 2;
-        if ((double)(mid + 1) * dt < thr)
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
-static int64_t first_true_ge(double dt, int64_t TN, double thr)
-{
-    int64_t lo = 0, hi = TN;
-    while (lo < hi) {
-        int64_t mid = lo + (hi - lo) / 2;
-        if ((double)(mid + 1) * dt >= thr)
-            hi = mid;
-        else
-            lo = mid + 1;
-    }
-    return lo;
-}
-
 static void plan_free(rms_plan *p)
 {
     if (!p) return;
@@ -477,6 +322,8 @@ static void plan_free(rms_plan *p)
     for (auto &e : p->ev)
         if (e) (void)hipEventDestroy(e);
     for (void *q : p->allocs) (void)hipFree(q);
+    if (p->h_res) (void)hipHostFree(p->h_res);
+    if (p->h_adj_res) (void)hipHostFree(p->h_adj_res);
     delete p;
 }
 
@@ -493,6 +340,19 @@ struct PlanHolder {
     rms_plan *p = nullptr;
     ~PlanHolder() { plan_free(p); }
 };
+
+// Stand-ins for the tables a kernel variant reads from every job of a plan,
+// for the jobs that have none of their own: an empty PGSE table (bounds
+// {0}, no triple, sentinel first gradient 0) and an empty b-table.
+struct Sentinels {
+    int64_t bounds[2] = {0, 0};
+    int32_t triple[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int32_t gperm[2] = {0, 0};
+    double grad[6] = {0, 0, 0, 0, 0, 0};
+    double qscale[2] = {0, 0};
+    double bvec[3] = {0, 0, 0};
+    double psig[2] = {0, 0};
+};
 }  // namespace
 
 static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_run_params *prm,
@@ -501,14 +361,7 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
     if (!ctx || !jobs || njobs < 1 || !prm || !out) return fail(ctx, RMS_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     const rms_run_params &P = *prm;
-    if (!(P.dt > 0.0)) return fail(ctx, RMS_ERR_INVALID_ARG, "dt must be positive");
-    if (P.timepoints < 1) return fail(ctx, RMS_ERR_INVALID_ARG, "timepoints must be >= 1");
-    if (P.TN < P.timepoints)
-        return fail(ctx, RMS_ERR_INVALID_ARG,
-                    "TN must be >= timepoints (the reference divides by TN/timepoints, main.cu:68,333)");
-    if (P.TN > 2147483647LL) return fail(ctx, RMS_ERR_INVALID_ARG, "TN must be < 2^31");
-    if (P.precision != RMS_PREC_F32 && P.precision != RMS_PREC_F64)
-        return fail(ctx, RMS_ERR_INVALID_ARG, "precision must be RMS_PREC_F32 or RMS_PREC_F64");
+    if (const char *e = check_params(P)) return fail(ctx, RMS_ERR_INVALID_ARG, e);
     const int R = P.replicas < 1 ? 1 : P.replicas;
     const int64_t T = P.timepoints;
     const int64_t Tstep = P.TN / P.timepoints;
@@ -532,76 +385,27 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
     int maxbound = 0;
     p->hjobs.resize(njobs);
     p->jobs.resize(njobs);
-    std::vector<std::vector<double>> qscale(njobs);
     for (int j = 0; j < njobs; j++) {
         const rms_job &J = jobs[j];
         if (J.geom_id < 0 || J.geom_id >= (int32_t)ctx->geoms.size()) BAD("job " + std::to_string(j) + ": bad geom_id");
-        if (J.nwalk < 1) BAD("job " + std::to_string(j) + ": nwalk must be >= 1");
-        if (J.nbvec < 0 || (J.nbvec > 0 && (!J.bval || !J.bvec))) BAD("job: bad b-table");
-        if (J.ngrad < 0 || (J.ngrad > 0 && !J.grad)) BAD("job: bad gradient table");
-        if (J.nbvec > 0) sig |= 1;
-        if (J.ngrad > 0) sig |= 2;
+        if (J.nbvec > 0) sig |= SIG_NARROW;
+        if (J.ngrad > 0) sig |= SIG_PGSE;
     }
-    p->sig = sig;
     for (int j = 0; j < njobs; j++) {
         const rms_job &J = jobs[j];
         const GeomDev &G = ctx->geoms[J.geom_id];
         JobDesc &D = p->hjobs[j];
-        std::memset(&D, 0, sizeof(D));
-        double step[NC], inv_step[NC], T2s[NC], Pp[NC * NC];
-        int Pcat[NC * NC];
-        std::string err;
-        if (!derive(G.meta, P.dt, step, inv_step, T2s, Pp, Pcat, err)) BAD(err);
-        for (int c = 1; c < G.meta.nclass; c++) {
-            if (!(G.meta.D[c] > 0.0)) BAD("non-null classes need a positive diffusivity");
-            if (!(step[c] < 1.0))
-                BAD("step size sqrt(6*D*dt) must be below one voxel (main.cu:210-214 assumes it)");
-            if (!(G.meta.T2[c] > 0.0)) BAD("T2 must be positive");
-        }
+        JobHost &H = p->jobs[j];
+        int bits = 0;
+        bool exch = false;
+        if (const char *e = fill_job(G.meta, J, P, D, bits, exch)) BAD("job " + std::to_string(j) + ": " + e);
+        Derived d;
+        (void)derive(G.meta, P.dt, d);
+        sig |= bits;
         D.cells = G.d_cells;
         D.cls = G.d_cls;
-        D.nx = G.meta.nx;
-        D.ny = G.meta.ny;
-        D.nz = G.meta.nz;
-        D.pg_log2 = pgse_flush_log2(G.meta.nx, G.meta.ny, G.meta.nz);
-        D.periodic = G.meta.periodic;
-        D.nclass = G.meta.nclass;
-        D.nbvec = J.nbvec;
-        D.ngrad = J.ngrad;
-        D.nwalk = J.nwalk;
-        D.key0 = (uint32_t)J.seed;
-        D.key1 = (uint32_t)(J.seed >> 32);
-        D.res = G.meta.res;
-        uint32_t pcat = 0;
-        for (int i = 0; i < NC * NC; i++) pcat |= (uint32_t)(Pcat[i] & 3) << (2 * i);
-        D.pcat = pcat;
-        for (int c = 0; c < NC; c++) {
-            D.step_f[c] = (float)step[c];
-            D.inv_step_f[c] = (float)inv_step[c];
-            D.step_d[c] = step[c];
-            D.inv_step_d[c] = inv_step[c];
-            D.T2s[c] = T2s[c];
-        }
-        for (int i = 0; i < NC * NC; i++) {
-            D.P_f[i] = (float)Pp[i];
-            D.P_d[i] = Pp[i];
-        }
-        D.xh12 = exchange_threshold(G.meta.nclass, Pcat[1 * NC + 2], D.P_f[1 * NC + 2]);
-        D.xh21 = exchange_threshold(G.meta.nclass, Pcat[2 * NC + 1], D.P_f[2 * NC + 1]);
-        derive_tangent(G.meta, P.dt, step, inv_step, T2s, Pp, Pcat, D);
-        JobHost &H = p->jobs[j];
-        H.exchange = false;
-        for (int a = 1; a < G.meta.nclass; a++)
-            for (int b = 1; b < G.meta.nclass; b++)
-                if (a != b && Pcat[a * NC + b] != 0) H.exchange = true;
-        // the walk kernel with faces between the two non-null classes on the
-        // fast path (reflect at P = 0, draw at 0 < P < 1); it is also the only
-        // variant whose march compiles exchange draws (category 2)
-        bool draws = false;
-        for (int i = 0; i < NC * NC; i++) draws = draws || Pcat[i] == 2;
-        if (G.meta.nclass == 3 || draws) p->sig |= 4;
-        // the walk kernel whose march wraps periodic lattice edges
-        if (G.meta.periodic != 0) p->sig |= 8;
+        H.exchange = exch;
+        H.kappa_closed = kappa_closes_pair(G.meta, d);
         H.geom_id = J.geom_id;
         H.nwalk = J.nwalk;
         H.nbatch = (J.nwalk + WAVE - 1) / WAVE;
@@ -609,136 +413,141 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         H.wbase = nbatch * WAVE;
         H.nbvec = J.nbvec;
         H.ngrad = J.ngrad;
-        H.nval = 1 + G.meta.nclass + 21 + ((sig & 1) ? J.nbvec : 0);
+        H.nval = 1 + G.meta.nclass + 21 + ((sig & SIG_NARROW) ? J.nbvec : 0);
         H.nbound = 0;
         H.ntriple = 0;
         D.batch0 = H.batch0;
         D.wbase = H.wbase;
         nbatch += H.nbatch;
-        // narrow-pulse q scale table, sqrt(bval/TD) (main.cu:375)
-        if (J.nbvec > 0) {
-            for (int b = 0; b < J.nbvec; b++)
-                if (!(J.bval[b] >= 0.0)) BAD("b-values must be >= 0");
-            qscale[j].resize((size_t)T * J.nbvec);
-            for (int64_t r = 0; r < T; r++) {
-                double TD = (double)(r * Tstep + 1) * P.dt;  // main.cu:486
-                for (int b = 0; b < J.nbvec; b++) qscale[j][(size_t)r * J.nbvec + b] = std::sqrt(J.bval[b] / TD);
-            }
-        }
     }
+    p->sig = sig;
     p->nbatch = nbatch;
     p->ws = nbatch * WAVE;
+
+    // sentinel tables for the jobs without their own
+    Sentinels sen;
+    int64_t *s_bounds;
+    int32_t *s_triple, *s_gperm;
+    double *s_grad, *s_qscale, *s_bvec, *s_psig;
+    TRY(pupload(p, &s_bounds, sen.bounds, 2));
+    TRY(pupload(p, &s_triple, sen.triple, 8));
+    TRY(pupload(p, &s_gperm, sen.gperm, 2));
+    TRY(pupload(p, &s_grad, sen.grad, 6));
+    TRY(pupload(p, &s_qscale, sen.qscale, 2));
+    TRY(pupload(p, &s_bvec, sen.bvec, 3));
+    TRY(pupload(p, &s_psig, sen.psig, 2));
 
     // PGSE tables (window sums), per job
     for (int j = 0; j < njobs; j++) {
         const rms_job &J = jobs[j];
         JobDesc &D = p->hjobs[j];
         JobHost &H = p->jobs[j];
+        D.bounds = s_bounds;
+        D.triple = s_triple;
+        D.gperm = s_gperm;
+        D.grad = s_grad;
         if (J.ngrad <= 0) continue;
-        std::vector<std::array<int64_t, 3>> abe(J.ngrad);
-        std::vector<int64_t> bl;
-        bl.push_back(P.TN);
-        for (int g = 0; g < J.ngrad; g++) {
-            const double *gr = J.grad + 6 * g;
-            int64_t a = first_false_lt(P.dt, P.TN, gr[1]);
-            int64_t b = first_true_ge(P.dt, P.TN, gr[0]);
-            int64_t e = first_false_lt(P.dt, P.TN, gr[0] + gr[1]);
-            if (b < a) b = a;
-            if (e < b) e = b;
-            abe[g] = {a, b, e};
-            bl.push_back(a);
-            bl.push_back(b);
-            bl.push_back(e);
-        }
-        std::sort(bl.begin(), bl.end());
-        bl.erase(std::unique(bl.begin(), bl.end()), bl.end());
-        std::vector<int64_t> bounds(1, 0);
-        for (int64_t v : bl)
-            if (v != 0) bounds.push_back(v);
-        if ((int)bounds.size() - 1 > MAX_BOUNDS) BAD("too many distinct PGSE lobe boundaries (max 64)");
-        H.nbound = (int)bounds.size() - 1;
-        maxbound = std::max(maxbound, H.nbound);
-        auto kidx = [&](int64_t v) {
-            return (int32_t)(std::lower_bound(bounds.begin(), bounds.end(), v) - bounds.begin());
-        };
-        std::vector<std::tuple<int32_t, int32_t, int32_t, int32_t>> order;
-        for (int g = 0; g < J.ngrad; g++) order.emplace_back(kidx(abe[g][0]), kidx(abe[g][1]), kidx(abe[g][2]), g);
-        std::stable_sort(order.begin(), order.end());
+        std::vector<int32_t> perm(J.ngrad), trip(4 * ((size_t)J.ngrad + 1));
+        std::vector<uint64_t> keys(J.ngrad);
+        std::vector<int64_t> bounds(MAX_BOUNDS + 1);
+        int32_t nb = 0, nt = 0;
+        if (const char *e = pgse_tables(J.grad, J.ngrad, P.dt, P.TN, perm.data(), trip.data(), J.ngrad,
+                                        bounds.data(), nb, nt, keys.data()))
+            BAD("job " + std::to_string(j) + ": " + e);
         std::vector<double> gs((size_t)J.ngrad * 6);
-        std::vector<int32_t> perm(J.ngrad), trip;
-        for (int g = 0; g < J.ngrad; g++) {
-            int src = std::get<3>(order[g]);
-            perm[g] = src;
-            std::memcpy(&gs[(size_t)g * 6], J.grad + 6 * src, 6 * sizeof(double));
-            bool newt = g == 0 || std::get<0>(order[g]) != std::get<0>(order[g - 1]) ||
-                        std::get<1>(order[g]) != std::get<1>(order[g - 1]) ||
-                        std::get<2>(order[g]) != std::get<2>(order[g - 1]);
-            if (newt) {
-                trip.push_back(std::get<0>(order[g]));
-                trip.push_back(std::get<1>(order[g]));
-                trip.push_back(std::get<2>(order[g]));
-                trip.push_back(g);
-            }
-        }
-        H.ntriple = (int)trip.size() / 4;
-        trip.push_back(0);
-        trip.push_back(0);
-        trip.push_back(0);
-        trip.push_back(J.ngrad);  // sentinel
-        H.gperm = perm;
-        D.ntriple = H.ntriple;
-        D.nbound = H.nbound;
+        for (int g = 0; g < J.ngrad; g++) std::memcpy(&gs[(size_t)g * 6], J.grad + 6 * (size_t)perm[g], 6 * sizeof(double));
+        H.nbound = nb;
+        H.ntriple = nt;
+        D.nbound = nb;
+        D.ntriple = nt;
+        maxbound = std::max(maxbound, nb);
         double *dg;
         int32_t *dp, *dt3;
         int64_t *db;
         TRY(pupload(p, &dg, gs.data(), gs.size()));
         TRY(pupload(p, &dp, perm.data(), perm.size()));
-        TRY(pupload(p, &dt3, trip.data(), trip.size()));
-        TRY(pupload(p, &db, bounds.data(), bounds.size()));
+        TRY(pupload(p, &dt3, trip.data(), 4 * ((size_t)nt + 1)));
+        TRY(pupload(p, &db, bounds.data(), (size_t)nb + 1));
         D.grad = dg;
         D.gperm = dp;
         D.triple = dt3;
         D.bounds = db;
     }
+    // narrow-pulse q scale table, sqrt(bval/TD) (main.cu:375), and b-vectors
     for (int j = 0; j < njobs; j++) {
+        const rms_job &J = jobs[j];
         JobDesc &D = p->hjobs[j];
-        if (jobs[j].nbvec > 0) {
+        D.qscale = s_qscale;
+        D.bvec = s_bvec;
+        if (J.nbvec > 0) {
+            std::vector<double> qs((size_t)T * J.nbvec);
+            for (int64_t r = 0; r < T; r++) {
+                const double TD = (double)(r * Tstep + 1) * P.dt;  // main.cu:486
+                for (int b = 0; b < J.nbvec; b++) qs[(size_t)r * J.nbvec + b] = std::sqrt(J.bval[b] / TD);
+            }
             double *dq, *dbv;
-            TRY(pupload(p, &dq, qscale[j].data(), qscale[j].size()));
-            TRY(pupload(p, &dbv, jobs[j].bvec, (size_t)jobs[j].nbvec * 3));
+            TRY(pupload(p, &dq, qs.data(), qs.size()));
+            TRY(pupload(p, &dbv, J.bvec, (size_t)J.nbvec * 3));
             D.qscale = dq;
             D.bvec = dbv;
         }
     }
-    // outputs: per-batch partials, replicas, totals
-    TRY(palloc(p, &p->d_counters, sizeof(unsigned long long) * 5 * njobs));
+    // results block: totals of every reduction, the counters, then the
+    // replica sums (one copy brings the first part back, or all of it)
+    int64_t nres = 0;
+    for (int j = 0; j < njobs; j++) {
+        JobHost &H = p->jobs[j];
+        H.rec_tot = nres;
+        nres += T * H.nval;
+        H.sig_tot = nres;
+        nres += H.ngrad;
+    }
+    const int64_t cnt_off = nres;  // in doubles; NCOUNT 64-bit counters per job
+    nres += (int64_t)NCOUNT * njobs;
+    p->res_head = nres * 8;
+    for (int j = 0; j < njobs; j++) {
+        JobHost &H = p->jobs[j];
+        H.rec_rep = nres;
+        nres += (int64_t)R * T * H.nval;
+        H.sig_rep = nres;
+        nres += (int64_t)R * H.ngrad;
+    }
+    p->res_all = nres * 8;
+    TRY(palloc(p, &p->d_res, (size_t)p->res_all));
+    {
+        void *h = nullptr;
+        if (hipHostMalloc(&h, (size_t)p->res_all, hipHostMallocDefault) != hipSuccess)
+            TRY(fail(ctx, RMS_ERR_OUT_OF_MEMORY, "hipHostMalloc of the results mirror failed"));
+        p->h_res = (double *)h;
+    }
+    p->d_counters = (unsigned long long *)(p->d_res + cnt_off);
     for (int j = 0; j < njobs; j++) {
         JobDesc &D = p->hjobs[j];
         JobHost &H = p->jobs[j];
-        D.counters = p->d_counters + 5 * j;
-        double *part, *rep, *tot;
-        int64_t nq = T * H.nval;
+        D.counters = p->d_counters + NCOUNT * j;
+        double *part;
+        const int64_t nq = T * H.nval;
         TRY(palloc(p, &part, sizeof(double) * H.nbatch * nq));
-        TRY(palloc(p, &rep, sizeof(double) * R * nq));
-        TRY(palloc(p, &tot, sizeof(double) * nq));
         D.part = part;
-        p->red.push_back(ReduceDesc{part, rep, tot, H.nbatch, nq});
+        p->red.push_back(ReduceDesc{part, p->d_res + H.rec_rep, p->d_res + H.rec_tot, H.nbatch, nq});
         p->max_nq = std::max(p->max_nq, nq);
+        D.psig = s_psig;
         if (H.ngrad > 0) {
-            double *ps, *sr, *stt;
+            double *ps;
             TRY(palloc(p, &ps, sizeof(double) * H.nbatch * H.ngrad));
-            TRY(palloc(p, &sr, sizeof(double) * R * H.ngrad));
-            TRY(palloc(p, &stt, sizeof(double) * H.ngrad));
             D.psig = ps;
-            p->red.push_back(ReduceDesc{ps, sr, stt, H.nbatch, H.ngrad});
+            p->red.push_back(ReduceDesc{ps, p->d_res + H.sig_rep, p->d_res + H.sig_tot, H.nbatch, H.ngrad});
             p->max_nq = std::max(p->max_nq, (int64_t)H.ngrad);
         }
     }
     p->maxbound = maxbound;
-    if (sig & 2) {
+    if (sig & SIG_PGSE) {
         TRY(palloc(p, &p->d_snap, sizeof(double) * (size_t)(maxbound + 1) * 3 * p->ws));
         for (int j = 0; j < njobs; j++) p->hjobs[j].snap = p->d_snap;
     }
+    // every table the chosen variant reads exists for every job
+    for (int j = 0; j < njobs; j++)
+        if (const char *e = check_desc(p->hjobs[j], sig)) TRY(fail(ctx, RMS_ERR_STATE, "job " + std::to_string(j) + ": " + e));
     // chunked launches keep walker state in HBM between launches
     const bool chunked = P.chunk_steps > 0 && P.chunk_steps < P.TN;
     p->st.ws = p->ws;
@@ -756,7 +565,7 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         }
         TRY(palloc(p, &p->st.tsum, sizeof(double) * NC * ws));
         TRY(palloc(p, &p->st.cnt, sizeof(uint32_t) * 5 * ws));
-        if (sig & 2) {
+        if (sig & SIG_PGSE) {
             TRY(palloc(p, &p->st.ps, sizeof(double) * 3 * ws));
             TRY(palloc(p, &p->st.sg, sizeof(int32_t) * 3 * ws));
         }
@@ -827,7 +636,7 @@ static rms_status enqueue(rms_plan *p, bool with_events)
 {
     rms_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
-    HIPCHK(ctx, launch_fill_zero(p->d_counters, sizeof(unsigned long long) * 5 * (int64_t)p->jobs.size(), s));
+    HIPCHK(ctx, launch_fill_zero(p->d_counters, sizeof(unsigned long long) * NCOUNT * (int64_t)p->jobs.size(), s));
     if (with_events) HIPCHK(ctx, hipEventRecord(p->ev[0], s));
     const int64_t TN = p->prm.TN;
     const int64_t ch = (p->prm.chunk_steps > 0 && p->prm.chunk_steps < TN) ? p->prm.chunk_steps : TN;
@@ -842,15 +651,28 @@ static rms_status enqueue(rms_plan *p, bool with_events)
     return RMS_OK;
 }
 
+static rms_status init_failures(rms_ctx *ctx, const unsigned long long *cnt, int nj)
+{
+    for (int j = 0; j < nj; j++)
+        if (cnt[NCOUNT * j + 5] != 0)
+            return fail(ctx, RMS_ERR_STATE,
+                        "job " + std::to_string(j) + ": " + std::to_string(cnt[NCOUNT * j + 5]) +
+                            " walkers found no non-null start voxel in 2^26 trials (walkable fraction too small)");
+    return RMS_OK;
+}
+
 static rms_status plan_execute_impl(rms_plan *p, rms_job_out *outs, rms_timing *timing)
 {
     rms_ctx *ctx = p->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const int nj = (int)p->jobs.size();
     // final-state outputs requested?
-    bool want_fin = false;
+    bool want_fin = false, want_rep = false;
     if (outs)
-        for (int j = 0; j < nj; j++) want_fin = want_fin || outs[j].fin_g;
+        for (int j = 0; j < nj; j++) {
+            want_fin = want_fin || outs[j].fin_g;
+            want_rep = want_rep || outs[j].rec_rep || outs[j].sig_rep;
+        }
     if (want_fin && !p->fin_alloc) {
         const int64_t ws = p->ws;
         rms_status st;
@@ -890,6 +712,10 @@ static rms_status plan_execute_impl(rms_plan *p, rms_job_out *outs, rms_timing *
         rms_status st = enqueue(p, true);
         if (st != RMS_OK) return st;
     }
+    // one copy brings back every total and counter (and the replica sums
+    // when any job asks for them)
+    const int64_t nbytes = want_rep ? p->res_all : p->res_head;
+    HIPCHK(ctx, hipMemcpyAsync(p->h_res, p->d_res, (size_t)nbytes, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
     if (timing) {
         std::memset(timing, 0, sizeof(*timing));
@@ -911,24 +737,22 @@ static rms_status plan_execute_impl(rms_plan *p, rms_job_out *outs, rms_timing *
         const int64_t ch = (p->prm.chunk_steps > 0 && p->prm.chunk_steps < TN) ? p->prm.chunk_steps : TN;
         timing->launches = (int32_t)((TN + ch - 1) / ch);
     }
+    const unsigned long long *cnt = (const unsigned long long *)(p->h_res) + (p->d_counters - (unsigned long long *)p->d_res);
+    if (rms_status st = init_failures(ctx, cnt, nj)) return st;
     if (!outs) return RMS_OK;
     const int R = p->prm.replicas;
-    std::vector<unsigned long long> cnt(5 * nj);
-    HIPCHK(ctx, hipMemcpy(cnt.data(), p->d_counters, sizeof(unsigned long long) * 5 * nj, hipMemcpyDeviceToHost));
-    size_t ri = 0;
+    const int64_t T = p->prm.timepoints;
     for (int j = 0; j < nj; j++) {
         const JobHost &H = p->jobs[j];
         rms_job_out &O = outs[j];
-        const ReduceDesc &rd = p->red[ri++];
-        if (O.rec) HIPCHK(ctx, hipMemcpy(O.rec, rd.tot, sizeof(double) * rd.nq, hipMemcpyDeviceToHost));
-        if (O.rec_rep) HIPCHK(ctx, hipMemcpy(O.rec_rep, rd.rep, sizeof(double) * rd.nq * R, hipMemcpyDeviceToHost));
+        const int64_t nq = T * H.nval;
+        if (O.rec) std::memcpy(O.rec, p->h_res + H.rec_tot, sizeof(double) * nq);
+        if (O.rec_rep) std::memcpy(O.rec_rep, p->h_res + H.rec_rep, sizeof(double) * nq * R);
         if (H.ngrad > 0) {
-            const ReduceDesc &rs = p->red[ri++];
-            if (O.sig) HIPCHK(ctx, hipMemcpy(O.sig, rs.tot, sizeof(double) * rs.nq, hipMemcpyDeviceToHost));
-            if (O.sig_rep)
-                HIPCHK(ctx, hipMemcpy(O.sig_rep, rs.rep, sizeof(double) * rs.nq * R, hipMemcpyDeviceToHost));
+            if (O.sig) std::memcpy(O.sig, p->h_res + H.sig_tot, sizeof(double) * H.ngrad);
+            if (O.sig_rep) std::memcpy(O.sig_rep, p->h_res + H.sig_rep, sizeof(double) * H.ngrad * R);
         }
-        for (int k = 0; k < 5; k++) O.counters[k] = cnt[5 * j + k];
+        for (int k = 0; k < 5; k++) O.counters[k] = cnt[NCOUNT * j + k];
         if (O.fin_g && p->fin_alloc) {
             const int64_t ws = p->ws, n = H.nwalk, b = H.wbase;
             for (int k = 0; k < 3; k++) {
@@ -977,17 +801,25 @@ static rms_status adjoint_setup(rms_plan *p)
     p->cot_sig.assign(nj, nullptr);
     p->red_adj.clear();  // a failed earlier setup may have left entries
     rms_status st;
+    const size_t nres = (size_t)nj * NPARAM * (1 + R);
+    if ((st = palloc(p, &p->d_adj_res, sizeof(double) * nres)) != RMS_OK) return st;
+    if (!p->h_adj_res) {
+        void *h = nullptr;
+        if (hipHostMalloc(&h, sizeof(double) * nres, hipHostMallocDefault) != hipSuccess)
+            return fail(ctx, RMS_ERR_OUT_OF_MEMORY, "hipHostMalloc of the gradient mirror failed");
+        p->h_adj_res = (double *)h;
+    }
     for (int j = 0; j < nj; j++) {
         const JobHost &H = p->jobs[j];
-        double *pg, *rep, *tot;
+        double *pg;
         if ((st = palloc(p, &p->cot_rec[j], sizeof(double) * T * H.nval)) != RMS_OK) return st;
         if (H.ngrad > 0 && (st = palloc(p, &p->cot_sig[j], sizeof(double) * H.ngrad)) != RMS_OK) return st;
         if ((st = palloc(p, &pg, sizeof(double) * H.nbatch * NPARAM)) != RMS_OK) return st;
-        if ((st = palloc(p, &rep, sizeof(double) * R * NPARAM)) != RMS_OK) return st;
-        if ((st = palloc(p, &tot, sizeof(double) * NPARAM)) != RMS_OK) return st;
+        double *tot = p->d_adj_res + (size_t)j * NPARAM;
+        double *rep = p->d_adj_res + (size_t)nj * NPARAM + (size_t)j * R * NPARAM;
         p->red_adj.push_back(ReduceDesc{pg, rep, tot, H.nbatch, NPARAM});
     }
-    if (p->sig & 2) {
+    if (p->sig & SIG_PGSE) {
         const size_t n = (size_t)(p->maxbound + 1) * p->adj_np * 3 * p->ws;
         if ((st = palloc(p, &p->d_tsnap, sizeof(double) * n)) != RMS_OK) return st;
     }
@@ -1049,6 +881,11 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, rms_tim
     HIPCHK(ctx, hipEventRecord(p->ev[1], s));
     HIPCHK(ctx, launch_reduce(p->d_red_adj, (int32_t)p->red_adj.size(), NPARAM, p->prm.replicas, s));
     HIPCHK(ctx, hipEventRecord(p->ev[2], s));
+    const int R = p->prm.replicas;
+    bool want_rep = false;
+    for (int j = 0; j < nj; j++) want_rep = want_rep || adj[j].grad_rep;
+    const size_t nbytes = sizeof(double) * (size_t)nj * NPARAM * (want_rep ? 1 + R : 1);
+    HIPCHK(ctx, hipMemcpyAsync(p->h_adj_res, p->d_adj_res, nbytes, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
     if (timing) {
         std::memset(timing, 0, sizeof(*timing));
@@ -1063,20 +900,25 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, rms_tim
         timing->block = p->block;
         timing->launches = 1;
     }
-    const int R = p->prm.replicas;
     for (int j = 0; j < nj; j++) {
-        const ReduceDesc &rd = p->red_adj[j];
-        HIPCHK(ctx, hipMemcpy(adj[j].grad, rd.tot, sizeof(double) * NPARAM, hipMemcpyDeviceToHost));
+        std::memcpy(adj[j].grad, p->h_adj_res + (size_t)j * NPARAM, sizeof(double) * NPARAM);
         if (adj[j].grad_rep)
-            HIPCHK(ctx, hipMemcpy(adj[j].grad_rep, rd.rep, sizeof(double) * NPARAM * R, hipMemcpyDeviceToHost));
+            std::memcpy(adj[j].grad_rep, p->h_adj_res + (size_t)nj * NPARAM + (size_t)j * R * NPARAM,
+                        sizeof(double) * NPARAM * R);
+        auto nan_entry = [&](int c) {
+            adj[j].grad[c] = NAN;
+            if (adj[j].grad_rep)
+                for (int r = 0; r < R; r++) adj[j].grad_rep[(size_t)r * NPARAM + c] = NAN;
+        };
         if (p->jobs[j].exchange) {
             // a crossing between non-null classes changes the class (and the
             // step length) discontinuously in D: no pathwise D derivative
-            for (int c = 1; c < NC; c++) {
-                adj[j].grad[c] = NAN;
-                if (adj[j].grad_rep)
-                    for (int r = 0; r < R; r++) adj[j].grad_rep[(size_t)r * NPARAM + c] = NAN;
-            }
+            for (int c = 1; c < NC; c++) nan_entry(c);
+        }
+        if (p->jobs[j].kappa_closed) {
+            // a pair closed by its kappa draws nothing, so the score function
+            // sees no draw although P grows with kappa: no estimate
+            nan_entry(8);
         }
     }
     return RMS_OK;

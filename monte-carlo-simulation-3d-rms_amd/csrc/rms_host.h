@@ -1,0 +1,73 @@
+// rms_host.h -- host-side construction and checking of the walk's job
+// descriptors: the derived parameters of hpc_code/lib/main.cu:429-458, the
+// PGSE lobe tables of main_PGSE.cu:341-345, and the checks a descriptor must
+// pass before a kernel variant may read it.
+//
+// Plain C++ (no HIP calls) and free of heap allocation, so the same code
+// builds every plan, the allocation-free entry point, and a CPU test harness.
+// Functions return nullptr on success and a static message otherwise.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/rms_amd.h"
+#include "rms_types.h"
+
+namespace rms {
+
+// Kernel variant bits (the SIG template argument of walk_kernel).
+enum : int { SIG_NARROW = 1, SIG_PGSE = 2, SIG_EXCH = 4, SIG_PERIODIC = 8 };
+
+// Steps must stay below one voxel by this margin: the straight move ends at
+// e = fma(s, v, o) with o in [0, 1], and e must round below 2 in FP32 and FP64.
+constexpr double STEP_MAX = 1.0 - 0x1p-20;
+
+struct Derived {
+    double step[NC], inv_step[NC], T2s[NC], P[NC * NC];
+    int Pcat[NC * NC];
+};
+
+// main.cu:429-458 (+ the kappa extension).
+const char *derive(const rms_geometry &g, double dt, Derived &d);
+
+// Tangent constants of the adjoint pass into D.
+void derive_tangent(const rms_geometry &g, double dt, const Derived &d, JobDesc &D);
+
+// True when walkers can move between two different non-null classes.
+bool has_exchange(const rms_geometry &g, const Derived &d);
+
+// True when some pair of non-null classes is impermeable (P < 1e-6) only
+// because of its kappa (the reference rule would let walkers cross).
+bool kappa_closes_pair(const rms_geometry &g, const Derived &d);
+
+int32_t exchange_threshold(int32_t nclass, int cat, float P);
+int32_t pgse_flush_log2(int32_t nx, int32_t ny, int32_t nz);
+
+// Validation of the run parameters shared by every entry point.
+const char *check_params(const rms_run_params &P);
+
+// Per-job scalars of a descriptor: lattice size, physics, seed, walker ids.
+// Device table pointers are left null (the caller points them at its
+// buffers or at sentinels).  sig_bits receives the variant bits the job
+// needs (SIG_EXCH, SIG_PERIODIC; the table bits come from the job's tables).
+const char *fill_job(const rms_geometry &g, const rms_job &J, const rms_run_params &P, JobDesc &D, int &sig_bits,
+                     bool &exchange);
+
+// PGSE lobe tables (window-sum form), allocation free.
+//   grad [ngrad*6] rows (Delta, delta, |g|, gx, gy, gz)
+//   perm [ngrad]: perm[k] = original index of the k-th sorted gradient
+//   triple [4*(ntriple+1)]: (ka, kb, ke, first sorted gradient) per distinct
+//     lobe triple, then a sentinel (0, 0, 0, ngrad); capacity max_triple+1
+//   bounds [nbound+1]: bounds[0] = 0, then the distinct nonzero boundaries;
+//     capacity MAX_BOUNDS+1
+//   keys [ngrad] work space
+// Lobe 1 of gradient g is steps [0, bounds[ka]), lobe 2 [bounds[kb], bounds[ke]),
+// from the reference's double comparisons (i+1)*dt < delta and
+// Delta <= (i+1)*dt < Delta+delta (main_PGSE.cu:341-345).
+const char *pgse_tables(const double *grad, int32_t ngrad, double dt, int64_t TN, int32_t *perm, int32_t *triple,
+                        int32_t max_triple, int64_t *bounds, int32_t &nbound, int32_t &ntriple, uint64_t *keys);
+
+// Every table that walk_kernel<., sig, .> can read for this job is present
+// (the job's own or a sentinel) and consistent with its sizes.
+const char *check_desc(const JobDesc &D, int sig);
+
+}  // namespace rms

@@ -242,9 +242,12 @@ __device__ __forceinline__ double mirror_ps(double ps, int32_t npg, double xo, d
 }
 
 // Rejection-sampled start voxel, main.cu:117-132 (uniforms over (0,1)).
+// Returns false when INIT_TRIALS trials found no non-null voxel; the walker
+// then sits in voxel (0, 0, 0) and the host reports the run as failed.
 template <typename R>
-__device__ __forceinline__ void init_walker(Walker<R> &w, const JobLocal<R> &L, uint64_t wid)
+__device__ __forceinline__ bool init_walker(Walker<R> &w, const JobLocal<R> &L, uint64_t wid)
 {
+    bool found = false;
     const double nx = (double)L.nx, ny = (double)L.ny, nz = (double)L.nz;
     w.g = {0, 0, 0};
     w.o = {(R)0.5, (R)0.5, (R)0.5};
@@ -269,12 +272,13 @@ __device__ __forceinline__ void init_walker(Walker<R> &w, const JobLocal<R> &L, 
             w.o = {(R)(x0 - (double)g0), (R)(x1 - (double)g1), (R)(x2 - (double)g2)};
             w.c = c;
             w.lin = lin;
+            found = true;
             break;
         }
     }
-    // lattices without a non-null voxel are rejected on the host
     w.gi = w.g;
     w.oi = w.o;
+    return found;
 }
 
 // Exchange draw j (0..3) of step i.  f32 layout: draw 0 is u01_f32 of the
@@ -946,8 +950,8 @@ void walk_kernel(LaunchArgs a)
         const JobDesc &J = a.jobs[jid];
         const JobLocal<R> L = job_local<R>(J);
         const int64_t lb = b - J.batch0;                   // batch within the job
-        const uint64_t wid = (uint64_t)(lb * WAVE + lane);  // walker id within the job
-        const bool active = (int64_t)wid < J.nwalk;
+        const bool active = lb * WAVE + lane < J.nwalk;
+        const uint64_t wid = (uint64_t)(J.wid0 + lb * WAVE + lane);  // Philox walker id
         const int64_t slot = J.wbase + lb * WAVE + lane;    // state slot
         const int nbvec = (SIG & 1) ? J.nbvec : 0;
         const int nval = 1 + L.nclass + 21 + nbvec;
@@ -974,7 +978,11 @@ void walk_kernel(LaunchArgs a)
         w.ps = {0.0, 0.0, 0.0};
         w.sg = {0, 0, 0};
         if (a.step0 == 0) {
-            init_walker<R>(w, L, wid);
+            const bool started = init_walker<R>(w, L, wid);
+            if constexpr (NP == 0) {
+                const uint32_t nfail = wave_sum_u32((active && !started) ? 1u : 0u);
+                if (lane == 0 && nfail != 0u) atomicAdd(J.counters + 5, (unsigned long long)nfail);
+            }
             w.sc = sel4(w.c, L.st0, L.st1, L.st2, L.st3);
             w.isc = sel4(w.c, L.is0, L.is1, L.is2, L.is3);
         } else {

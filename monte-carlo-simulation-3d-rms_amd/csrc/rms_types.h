@@ -13,6 +13,7 @@ constexpr int NITE = 4;        // face iterations per step (main.cu:36)
 constexpr int WAVE = 64;       // walkers per batch
 constexpr int MAX_BOUNDS = 64; // distinct PGSE lobe boundaries
 constexpr uint32_t INIT_TRIALS = 1u << 26;
+constexpr int NCOUNT = 6;      // device counters per job (index 5: failed starts)
 
 // Cell word of a voxel (one uint64 per voxel, index (x*ny + y)*nz + z): the
 // relation of each voxel of its 3x3x3 neighbourhood to the voxel's class, two
@@ -73,12 +74,13 @@ struct JobDesc {
     // outputs
     double *part;              // [nbatch_job][timepoints][nval]
     double *psig;              // [nbatch_job][ngrad]
-    unsigned long long *counters; // [5]
+    unsigned long long *counters; // [NCOUNT]: the 5 of rms_job_out, then walkers whose start failed
     // tangent outputs / inputs
     const double *cot_rec;     // [timepoints][nval] cotangent of the record sums
     const double *cot_sig;     // [ngrad]
     double *pgrad;             // [nbatch_job][NPARAM]
     int64_t wbase;             // first walker slot in the state arrays
+    int64_t wid0;              // Philox walker id of the job's first walker
 };
 
 // Walker state saved between chunked launches (structure of arrays).

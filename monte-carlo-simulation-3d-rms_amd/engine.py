@@ -207,6 +207,7 @@ class Plan:
             cj[k].bvec = _ptr(bvec)
             cj[k].ngrad = ng
             cj[k].grad = _ptr(grad)
+            cj[k].walker_base = int(jb.get("walker_base", 0))
             self.jobs.append(dict(nwalk=int(jb["nwalk"]), nclass=geom.nclass, nbvec=nb, ngrad=ng))
         self.any_nb = any(j["nbvec"] for j in self.jobs)
         self.params = capi.RunParams(

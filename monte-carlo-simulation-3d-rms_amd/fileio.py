@@ -189,7 +189,7 @@ class ReadRMS:
         self.NPar_count = _load(os.path.join(root, "NPar_count.txt"))
 
     def akc_mom(self, n):
-        """[K, D] along unit directions n (readRMS.m:168-175)."""
+        """[K, D] along unit directions n (readRMS.m:96-103)."""
         x2 = self.dx2t @ A.ndir2(n).T
         x4 = self.dx4t @ A.ndir4(n).T
         D = x2 / 2.0 / self.TD[:, None]

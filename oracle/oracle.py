@@ -58,6 +58,7 @@ class _Job(C.Structure):
         ("bvec", C.c_void_p),
         ("ngrad", C.c_int32),
         ("grad", C.c_void_p),
+        ("walker_base", C.c_int64),
     ]
 
 
@@ -346,6 +347,7 @@ def run(jobs, dt, TN, timepoints, prec="f32", replicas=32, pgse_bruteforce=False
         cj[k].bvec = _ptr(bvec)
         cj[k].ngrad = ng
         cj[k].grad = _ptr(grad)
+        cj[k].walker_base = int(jb.get("walker_base", 0))
         nv = nval(geom.nclass, nb)
         R = max(1, replicas)
         r = {

@@ -405,6 +405,19 @@ typedef struct {
 /* d log P_ij and d log(1 - P_ij) with respect to D_1..D_3 and kappa for the
  * probabilistic pairs (Pcat = 2): P = min(1, sqrt(Dj/Di)) * pi0/(1+pi0),
  * pi0 = 4 kappa dt / (res min(s_i, s_j)). */
+/* 1 when a pair of non-null classes is impermeable (P < 1e-6) only because
+ * of its kappa: the reference rule min(1, sqrt(Dj/Di)) would let walkers
+ * cross. */
+static int or_kappa_closes_pair(const or_geom *g, const or_derived *d)
+{
+    for (int a = 1; a < d->nclass; a++)
+        for (int b = 1; b < d->nclass; b++) {
+            if (a == b || g->kappa[a * OR_NC_MAX + b] < 0.0 || d->Pcat[a * OR_NC_MAX + b] != 0) continue;
+            if (g->D[a] > 0.0 && g->D[b] > 0.0) return 1;
+        }
+    return 0;
+}
+
 /* 1 when walkers can move between two different non-null classes. */
 static int or_has_exchange(const or_derived *d)
 {
@@ -653,9 +666,12 @@ static int validate_job(const or_job *J, const or_params *p, const or_derived *d
     if (G->label_bytes != 1 && G->label_bytes != 2) return fail("label_bytes must be 1 or 2");
     if (p->timepoints < 1 || p->TN < p->timepoints) return fail("TN must be >= timepoints");
     if (p->TN > 2147483647LL) return fail("TN too large");
+    int64_t R = p->replicas < 1 ? 1 : p->replicas;
+    if (J->walker_base < 0 || J->walker_base % (OR_BATCH * R) != 0)
+        return fail("walker_base must be a non-negative multiple of 64 * replicas");
     for (int c = 1; c < G->nclass; c++) {
         if (!(G->D[c] > 0.0)) return fail("non-null classes need D > 0");
-        if (!(d->step[c] < 1.0)) return fail("step size must be below one voxel");
+        if (!(d->step[c] <= 1.0 - 0x1p-20)) return fail("step size must be below one voxel by at least 2^-20");
         if (!(G->T2[c] > 0.0)) return fail("T2 must be positive");
     }
     return 1;
@@ -876,6 +892,12 @@ static int or_run_impl(const or_job *jobs, int32_t njobs, const or_params *p, or
                     if (adj[jb].grad_rep)
                         for (int r = 0; r < R; r++) adj[jb].grad_rep[(size_t)r * 16 + c] = NAN;
                 }
+            }
+            /* a pair closed by its kappa draws nothing: no kappa estimate */
+            if (or_kappa_closes_pair(J->geom, &d)) {
+                adj[jb].grad[8] = NAN;
+                if (adj[jb].grad_rep)
+                    for (int r = 0; r < R; r++) adj[jb].grad_rep[(size_t)r * 16 + 8] = NAN;
             }
             free(grep);
             free(pgrad);

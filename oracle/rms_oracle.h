@@ -62,6 +62,7 @@ typedef struct {
     const double *bvec;          /* 3*nbvec */
     int32_t ngrad;               /* PGSE table (0 = none) */
     const double *grad;          /* 6*ngrad rows: Delta, delta, |g|, gx, gy, gz */
+    int64_t walker_base;         /* Philox id of the first walker (multiple of 64*replicas) */
 } or_job;
 
 typedef struct {
@@ -129,7 +130,9 @@ int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs);
 /* Adjoint pass: derivatives of sum(cot_rec*rec) + sum(cot_sig*sig) with
  * respect to D[c] (grad[c], c = 1..3), T2[c] (grad[4+c]) and kappa (grad[8]).
  * D entries are NaN when walkers can cross between non-null classes (the
- * pathwise derivative misses the jump of a class change). */
+ * pathwise derivative misses the jump of a class change); the kappa entry is
+ * NaN when a pair of non-null classes is impermeable only because of its
+ * kappa (no draw happens there, so the score function sees nothing). */
 typedef struct {
     const double *cot_rec;  /* [timepoints*nval] or NULL */
     const double *cot_sig;  /* [ngrad] or NULL */

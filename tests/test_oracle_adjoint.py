@@ -187,6 +187,37 @@ def test_kappa_score_gradient_matches_finite_difference(orc):
     assert abs(adj - fd) < 4 * np.hypot(adj_se, fd_se) + 0.03 * abs(fd), (adj, adj_se, fd, fd_se)
 
 
+def test_kappa_gradient_is_nan_where_kappa_closes_the_membrane(orc):
+    """At kappa = 0 (and for a kappa so small that P < 1e-6) the membrane
+    reflects without a draw, so the score function has nothing to sum; yet the
+    expectation moves with kappa (a forward difference from 0 is clearly
+    nonzero), so a zero would be wrong: the kappa entry is NaN there."""
+    w, nper = 5, 3
+    z = np.arange(2 * w * nper)
+    cls = np.where(((z + w // 2) // w) % 2 == 0, 1, 2).astype(np.uint8)
+    lab = np.ascontiguousarray(np.broadcast_to(cls, (4, 4, z.size)))
+    D, T2 = [0.0, 2.0, 1.2], [np.inf, 80.0, 40.0]
+    TN, T, R, N = 800, 8, 32, 1 << 12
+    cot = np.zeros((T, orc.nval(3, 0)))
+    cot[:, 1 + 3 + 5] = 50.0  # sum of dz^2 (um^2): walkers leave their slab only through the membranes
+    for k0 in (0.0, 1e-9):
+        r0 = orc.run(_job(orc, lab, D, T2, k0, nwalk=N, seed=11), DT, TN, T, prec="f64", replicas=R,
+                     cot_rec=[cot])[0]
+        assert np.isnan(r0["grad"][8]) and np.all(np.isnan(r0["grad_rep"][:, 8]))
+        assert np.all(np.isfinite(r0["grad"][4:8]))
+    # finite for a membrane that draws
+    r1 = orc.run(_job(orc, lab, D, T2, 0.5, nwalk=N, seed=11), DT, TN, T, prec="f64", replicas=R, cot_rec=[cot])[0]
+    assert np.isfinite(r1["grad"][8])
+
+    def L(k):
+        r = orc.run(_job(orc, lab, D, T2, k, nwalk=N, seed=11), DT, TN, T, prec="f64", replicas=R)[0]
+        return (cot[None] * r["rec_rep"]).reshape(R, -1).sum(1)
+
+    h = 0.1
+    fd, fd_se = _batch_stats((L(h) - L(0.0)) / h)
+    assert abs(fd) > 5 * fd_se, (fd, fd_se)
+
+
 def test_akc_cotangent_matches_numeric_derivative(rms):
     """analysis.akc_cotangent (the chain rule from D(t), K(t) to the record
     sums) against central differences of analysis.akc_mom."""

@@ -315,3 +315,18 @@ def test_periodic_free_diffusion_unwrapped(orc):
     assert np.max(np.abs((D[-5:] - 2.0) / se[-5:])) < 4.5
     assert 2 * 2.0 * TD[-1] > (4 * 0.1) ** 2 * 4  # displacements well beyond the 0.4 um lattice
     assert r["counters"][2] == 0
+
+
+def test_start_failure_is_an_error(orc):
+    """A walker's start is rejection sampling with a cap of 2^26 trials
+    (main.cu:117-132 has no cap and would loop for ever).  One walkable voxel
+    in 1e8 gives a failure probability exp(-2^26 / 1e8) ~ 0.51 per walker:
+    seed 2 exhausts the trials and the run is refused instead of reporting a
+    walker parked in voxel (0, 0, 0); seed 1 finds the voxel."""
+    lab = np.zeros((500, 500, 400), np.uint8)
+    lab[123, 321, 77] = 1
+    g = orc.Geometry(lab, 0.1, [0.0, 2.0])
+    r = orc.run([dict(geom=g, nwalk=1, seed=1)], 2e-4, 10, 10, prec="f32", replicas=1, final_state=True)[0]
+    assert r["fin_g"].ravel().tolist() == [123, 321, 77]
+    with pytest.raises(RuntimeError, match="no non-null voxel"):
+        orc.run([dict(geom=g, nwalk=1, seed=2)], 2e-4, 10, 10, prec="f32", replicas=1)

@@ -178,6 +178,26 @@ rms_status rms_run(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_r
  * reference has no gradients).  Recomputes every trajectory from its seed. */
 rms_status rms_plan_adjoint(rms_plan *plan, const rms_job_adj *adj, rms_timing *timing);
 
+/* ---- allocation-free run ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same walk and reductions as rms_run for one job,
+ * with every device buffer beyond the inputs (the geometry added earlier by
+ * rms_add_geometry, the job's tables) and the outputs (host buffers, as for
+ * rms_run) carved from one static device array of RMS_SCRATCH_BYTES per GPU,
+ * which the GPU holds until the process ends.  No hipMalloc, hipFree or
+ * other allocating routine is called, on the device or the host.
+ * Limits: one job; one launch (chunk_steps 0 or >= TN, use_graph 0); no
+ * final-state outputs; ngrad <= 2000 with at most 512 distinct lobe triples;
+ * a run whose buffers exceed RMS_SCRATCH_BYTES is refused with
+ * RMS_ERR_INVALID_ARG before anything is launched.
+ * Every call on one GPU uses that GPU's array and leaves its contents there;
+ * calls that overlap on one GPU (two threads, or two contexts on the same
+ * device) race. */
+#define RMS_SCRATCH_BYTES (256ull << 20)
+rms_status rms_run_noalloc(rms_ctx *ctx, const rms_job *job, const rms_run_params *params, rms_job_out *out,
+                           rms_timing *timing);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,9 @@ STATUS = {
 EXPORTS = (
     "rms_create", "rms_destroy", "rms_last_error", "rms_version", "rms_add_geometry", "rms_derive",
     "rms_nval", "rms_plan_create", "rms_plan_execute", "rms_plan_destroy", "rms_run", "rms_plan_adjoint",
+    "rms_run_noalloc",
 )
+SCRATCH_BYTES = 256 << 20  # RMS_SCRATCH_BYTES
 
 
 class RmsError(RuntimeError):
@@ -149,6 +151,8 @@ def lib():
     L.rms_run.argtypes = [C.c_void_p, C.POINTER(Job), C.c_int32, C.POINTER(RunParams), C.POINTER(JobOut),
                           C.POINTER(Timing)]
     L.rms_plan_adjoint.argtypes = [C.c_void_p, C.POINTER(JobAdj), C.POINTER(Timing)]
+    L.rms_run_noalloc.argtypes = [C.c_void_p, C.POINTER(Job), C.POINTER(RunParams), C.POINTER(JobOut),
+                                  C.POINTER(Timing)]
     for name in EXPORTS:
         if name not in ("rms_version", "rms_last_error", "rms_nval"):
             getattr(L, name).restype = C.c_int

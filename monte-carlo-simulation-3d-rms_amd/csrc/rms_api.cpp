@@ -35,6 +35,10 @@ struct rms_ctx {
     int ncu = 0;
     std::string err;
     std::vector<GeomDev> geoms;
+    // this device's copy of the scratch array of rms_run_noalloc, and the
+    // events that time its runs (both obtained once, here at creation)
+    unsigned char *scratch = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
 };
 
 namespace {
@@ -164,6 +168,18 @@ static rms_status create_impl(const rms_device_opts *opts, rms_ctx **out)
         delete c;
         return RMS_ERR_HIP;
     }
+    // device address of this device's scratch array (device `dev` is current)
+    void *sp = nullptr;
+    bool ok = scratch_address(&sp) == hipSuccess && sp != nullptr;
+    c->scratch = (unsigned char *)sp;
+    for (auto &e : c->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    if (!ok) {
+        for (auto &e : c->ev)
+            if (e) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return RMS_ERR_HIP;
+    }
     *out = c;
     return RMS_OK;
 }
@@ -181,6 +197,8 @@ extern "C" rms_status rms_destroy(rms_ctx *ctx)
         if (g.d_cells) (void)hipFree(g.d_cells);
         if (g.d_cls) (void)hipFree(g.d_cls);
     }
+    for (auto &e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return RMS_OK;
@@ -928,4 +946,192 @@ extern "C" rms_status rms_plan_adjoint(rms_plan *p, const rms_job_adj *adj, rms_
 {
     if (!p || !adj) return RMS_ERR_INVALID_ARG;
     return guard(p->ctx, [&] { return plan_adjoint_impl(p, adj, timing); });
+}
+
+// ---------------------------------------------------------- allocation free
+// rms_run for one job with every device buffer in the scratch array.  The
+// host side uses only the stack and the context's preallocated message
+// buffer: no hipMalloc / hipFree, no heap.
+namespace {
+struct NoAllocFail {
+    rms_ctx *ctx;
+    rms_status operator()(rms_status s, const char *msg) const
+    {
+        ctx->err.assign(msg);  // fits the capacity reserved at creation
+        return s;
+    }
+    rms_status hip(const char *what, hipError_t e) const
+    {
+        ctx->err.assign(what);
+        ctx->err.append(": ");
+        ctx->err.append(hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? RMS_ERR_OUT_OF_MEMORY : RMS_ERR_HIP;
+    }
+};
+}  // namespace
+
+static rms_status run_noalloc_impl(rms_ctx *ctx, const rms_job *job, const rms_run_params *prm, rms_job_out *out,
+                                   rms_timing *timing)
+{
+    const NoAllocFail bad{ctx};
+#define NA_HIP(call)                                   \
+    do {                                               \
+        const hipError_t e_ = (call);                  \
+        if (e_ != hipSuccess) return bad.hip(#call, e_); \
+    } while (0)
+    if (!job || !prm || !out) return bad(RMS_ERR_INVALID_ARG, "null argument");
+    if (!ctx->scratch) return bad(RMS_ERR_STATE, "no scratch array on this device");
+    const rms_run_params &P = *prm;
+    if (const char *e = check_params(P)) return bad(RMS_ERR_INVALID_ARG, e);
+    if (P.use_graph) return bad(RMS_ERR_INVALID_ARG, "rms_run_noalloc: use_graph must be 0");
+    if (P.chunk_steps > 0 && P.chunk_steps < P.TN) return bad(RMS_ERR_INVALID_ARG, "rms_run_noalloc: one launch only (chunk_steps 0)");
+    if (out->fin_g || out->fin_o || out->fin_gi || out->fin_oi || out->fin_c || out->fin_cnt)
+        return bad(RMS_ERR_INVALID_ARG, "rms_run_noalloc: no final-state outputs");
+    if (job->geom_id < 0 || job->geom_id >= (int32_t)ctx->geoms.size()) return bad(RMS_ERR_INVALID_ARG, "bad geom_id");
+    if (job->ngrad > NOALLOC_MAX_GRAD) return bad(RMS_ERR_INVALID_ARG, "rms_run_noalloc: ngrad must be <= 2000");
+    const int R = P.replicas < 1 ? 1 : P.replicas;
+    const int32_t T = P.timepoints;
+    const int64_t Tstep = P.TN / P.timepoints;
+    const GeomDev &G = ctx->geoms[job->geom_id];
+    JobDesc D;
+    int sig = 0;
+    bool exch = false;
+    if (const char *e = fill_job(G.meta, *job, P, D, sig, exch)) return bad(RMS_ERR_INVALID_ARG, e);
+    if (job->nbvec > 0) sig |= SIG_NARROW;
+    if (job->ngrad > 0) sig |= SIG_PGSE;
+    // PGSE lobe tables on the stack
+    int32_t perm[NOALLOC_MAX_GRAD], trip[4 * (NOALLOC_MAX_TRIPLE + 1)];
+    uint64_t keys[NOALLOC_MAX_GRAD];
+    int64_t bounds[MAX_BOUNDS + 1];
+    int32_t nbound = 0, ntriple = 0;
+    if (job->ngrad > 0) {
+        if (const char *e = pgse_tables(job->grad, job->ngrad, P.dt, P.TN, perm, trip, NOALLOC_MAX_TRIPLE, bounds,
+                                        nbound, ntriple, keys))
+            return bad(RMS_ERR_INVALID_ARG, e);
+    }
+    const int64_t nbatch = (job->nwalk + WAVE - 1) / WAVE;
+    const int32_t nval = 1 + G.meta.nclass + 21 + job->nbvec;
+    ScratchLayout L;
+    if (const char *e = scratch_layout(nbatch, T, nval, R, job->ngrad, nbound, ntriple, job->nbvec, SCRATCH_BYTES, L))
+        return bad(RMS_ERR_INVALID_ARG, e);
+    unsigned char *S = ctx->scratch;
+    auto dp = [&](int64_t off) { return (void *)(S + off); };
+    D.cells = G.d_cells;
+    D.cls = G.d_cls;
+    D.batch0 = 0;
+    D.wbase = 0;
+    D.part = (double *)dp(L.part);
+    D.psig = (double *)dp(L.psig);
+    D.snap = (double *)dp(L.snap);
+    D.counters = (unsigned long long *)dp(L.counters);
+    D.grad = (const double *)dp(L.grad);
+    D.gperm = (const int32_t *)dp(L.perm);
+    D.triple = (const int32_t *)dp(L.triple);
+    D.bounds = (const int64_t *)dp(L.bounds);
+    D.qscale = (const double *)dp(L.qscale);
+    D.bvec = (const double *)dp(L.bvec);
+    D.nbound = nbound;
+    D.ntriple = ntriple;
+    if (const char *e = check_desc(D, sig)) return bad(RMS_ERR_STATE, e);
+    ReduceDesc red[2];
+    int nred = 1;
+    red[0] = ReduceDesc{D.part, (double *)dp(L.rep), (double *)dp(L.tot), nbatch, (int64_t)T * nval};
+    int64_t max_nq = (int64_t)T * nval;
+    if (job->ngrad > 0) {
+        red[1] = ReduceDesc{D.psig, (double *)dp(L.srep), (double *)dp(L.stot), nbatch, job->ngrad};
+        nred = 2;
+        max_nq = std::max<int64_t>(max_nq, job->ngrad);
+    }
+    // launch shape: persistent grid at the resident limit
+    const int block = P.block_threads > 0 ? P.block_threads : 256;
+    if (block % WAVE != 0 || block > 256) return bad(RMS_ERR_INVALID_ARG, "block_threads must be a multiple of 64 and <= 256");
+    const bool f64 = P.precision == RMS_PREC_F64;
+    int bpc = 0;
+    if (walk_occupancy(f64, sig, block, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+    const int64_t need = (nbatch + block / WAVE - 1) / (block / WAVE);
+    int64_t grid = P.grid_blocks > 0 ? P.grid_blocks : std::min(need, (int64_t)bpc * ctx->ncu);
+    if (grid < 1) grid = 1;
+    const int xcd = (P.xcd_split > 0 && grid >= 8) ? 1 : 0;
+    if (xcd) grid = grid / 8 * 8;
+
+    NA_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    // tables and descriptors in (the stack buffers stay live until the
+    // synchronisation below), sentinels and counters cleared, then the walk
+    NA_HIP(hipMemcpyAsync(dp(L.job), &D, sizeof(D), hipMemcpyHostToDevice, s));
+    NA_HIP(hipMemcpyAsync(dp(L.red), red, sizeof(ReduceDesc) * nred, hipMemcpyHostToDevice, s));
+    NA_HIP(launch_fill_zero(dp(L.batch_job), (nbatch * 4 + 7) / 8 * 8, s));  // one job: every batch is job 0
+    NA_HIP(launch_fill_zero(dp(L.counters), 8 * NCOUNT, s));
+    if (job->ngrad > 0) {
+        NA_HIP(hipMemcpyAsync(dp(L.grad_raw), job->grad, sizeof(double) * 6 * job->ngrad, hipMemcpyHostToDevice, s));
+        NA_HIP(hipMemcpyAsync(dp(L.perm), perm, sizeof(int32_t) * job->ngrad, hipMemcpyHostToDevice, s));
+        NA_HIP(hipMemcpyAsync(dp(L.triple), trip, sizeof(int32_t) * 4 * (ntriple + 1), hipMemcpyHostToDevice, s));
+        NA_HIP(hipMemcpyAsync(dp(L.bounds), bounds, sizeof(int64_t) * (nbound + 1), hipMemcpyHostToDevice, s));
+        NA_HIP(launch_gather_rows((const double *)dp(L.grad_raw), (const int32_t *)dp(L.perm), job->ngrad, 6,
+                                  (double *)dp(L.grad), s));
+    } else {
+        NA_HIP(launch_fill_zero(dp(L.bounds), 16, s));
+        NA_HIP(launch_fill_zero(dp(L.triple), 32, s));
+        NA_HIP(launch_fill_zero(dp(L.perm), 8, s));
+        NA_HIP(launch_fill_zero(dp(L.grad), 48, s));
+    }
+    if (job->nbvec > 0) {
+        NA_HIP(hipMemcpyAsync(dp(L.bval), job->bval, sizeof(double) * job->nbvec, hipMemcpyHostToDevice, s));
+        NA_HIP(hipMemcpyAsync(dp(L.bvec), job->bvec, sizeof(double) * 3 * job->nbvec, hipMemcpyHostToDevice, s));
+        NA_HIP(launch_qscale((const double *)dp(L.bval), job->nbvec, T, Tstep, P.dt, (double *)dp(L.qscale), s));
+    } else {
+        NA_HIP(launch_fill_zero(dp(L.qscale), 16, s));
+        NA_HIP(launch_fill_zero(dp(L.bvec), 24, s));
+    }
+    LaunchArgs a{};
+    a.jobs = (const JobDesc *)dp(L.job);
+    a.batch_job = (const int32_t *)dp(L.batch_job);
+    a.nbatch = nbatch;
+    a.TN = P.TN;
+    a.step0 = 0;
+    a.step1 = P.TN;
+    a.Tstep = Tstep;
+    a.timepoints = T;
+    a.dt = P.dt;
+    a.st.ws = nbatch * WAVE;
+    a.xcd_split = xcd;
+    NA_HIP(hipEventRecord(ctx->ev[0], s));
+    NA_HIP(launch_walk(a, f64, sig, (int)grid, block, s));
+    NA_HIP(hipEventRecord(ctx->ev[1], s));
+    NA_HIP(launch_reduce((const ReduceDesc *)dp(L.red), nred, max_nq, R, s));
+    NA_HIP(hipEventRecord(ctx->ev[2], s));
+    unsigned long long cnt[NCOUNT];
+    NA_HIP(hipMemcpyAsync(cnt, dp(L.counters), sizeof(cnt), hipMemcpyDeviceToHost, s));
+    if (out->rec) NA_HIP(hipMemcpyAsync(out->rec, dp(L.tot), sizeof(double) * T * nval, hipMemcpyDeviceToHost, s));
+    if (out->rec_rep)
+        NA_HIP(hipMemcpyAsync(out->rec_rep, dp(L.rep), sizeof(double) * R * T * nval, hipMemcpyDeviceToHost, s));
+    if (job->ngrad > 0 && out->sig)
+        NA_HIP(hipMemcpyAsync(out->sig, dp(L.stot), sizeof(double) * job->ngrad, hipMemcpyDeviceToHost, s));
+    if (job->ngrad > 0 && out->sig_rep)
+        NA_HIP(hipMemcpyAsync(out->sig_rep, dp(L.srep), sizeof(double) * R * job->ngrad, hipMemcpyDeviceToHost, s));
+    NA_HIP(hipStreamSynchronize(s));
+    if (cnt[5] != 0) return bad(RMS_ERR_STATE, "walkers found no non-null start voxel in 2^26 trials (walkable fraction too small)");
+    for (int k = 0; k < 5; k++) out->counters[k] = cnt[k];
+    if (timing) {
+        std::memset(timing, 0, sizeof(*timing));
+        float ms = 0.f;
+        NA_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[2]));
+        timing->total_ms = ms;
+        NA_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        timing->walk_ms = ms;
+        NA_HIP(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+        timing->reduce_ms = ms;
+        timing->grid = (int32_t)grid;
+        timing->block = block;
+        timing->launches = 1;
+    }
+    return RMS_OK;
+#undef NA_HIP
+}
+
+extern "C" rms_status rms_run_noalloc(rms_ctx *ctx, const rms_job *job, const rms_run_params *params,
+                                      rms_job_out *out, rms_timing *timing)
+{
+    if (!ctx) return RMS_ERR_INVALID_ARG;
+    return guard(ctx, [&] { return run_noalloc_impl(ctx, job, params, out, timing); });
 }

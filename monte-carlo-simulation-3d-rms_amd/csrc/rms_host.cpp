@@ -374,4 +374,49 @@ const char *check_desc(const JobDesc &D, int sig)
     return nullptr;
 }
 
+const char *scratch_layout(int64_t nbatch, int32_t timepoints, int32_t nval, int32_t replicas, int32_t ngrad,
+                           int32_t nbound, int32_t ntriple, int32_t nbvec, int64_t cap, ScratchLayout &L)
+{
+    const char *too_big = "run needs more device memory than the RMS_SCRATCH_BYTES scratch array holds "
+                          "(fewer walkers, records or replicas, or rms_run)";
+    if (nbatch < 1 || timepoints < 1 || nval < 1 || replicas < 1 || ngrad < 0 || nbvec < 0) return "bad sizes";
+    int64_t off = 0;
+    bool over = false;
+    // sizes as doubles first, so no product can overflow before the check
+    auto seg = [&](double elems, double elem_bytes, double min_bytes) -> int64_t {
+        double b = elems * elem_bytes;
+        if (b < min_bytes) b = min_bytes;
+        const int64_t o = off;
+        if (over || (double)off + b + 256.0 > (double)cap) {
+            over = true;
+            return 0;
+        }
+        off += ((int64_t)b + 255) / 256 * 256;
+        return o;
+    };
+    const double nb = (double)nbatch, T = (double)timepoints, nv = (double)nval, R = (double)replicas;
+    const double ng = (double)ngrad, ws = nb * WAVE;
+    L.job = seg(1, (double)sizeof(JobDesc), 0);
+    L.red = seg(2, (double)sizeof(ReduceDesc), 0);
+    L.batch_job = seg(nb, 4, 8);
+    L.counters = seg(NCOUNT, 8, 0);
+    L.part = seg(nb * T, nv * 8, 0);
+    L.rep = seg(R * T, nv * 8, 0);
+    L.tot = seg(T, nv * 8, 0);
+    L.psig = seg(nb * ng, 8, 16);
+    L.srep = seg(R * ng, 8, 16);
+    L.stot = seg(ng, 8, 16);
+    L.snap = seg(ngrad > 0 ? (double)(nbound + 1) * 3 * ws : 1, 8, 16);
+    L.grad_raw = seg(6 * ng, 8, 48);
+    L.grad = seg(6 * ng, 8, 48);
+    L.perm = seg(ng, 4, 8);
+    L.triple = seg(4.0 * (ntriple + 1), 4, 32);
+    L.bounds = seg((double)nbound + 1, 8, 16);
+    L.bval = seg((double)nbvec, 8, 16);
+    L.qscale = seg(T * nbvec, 8, 16);
+    L.bvec = seg(3.0 * nbvec, 8, 24);
+    L.end = off;
+    return over ? too_big : nullptr;
+}
+
 }  // namespace rms

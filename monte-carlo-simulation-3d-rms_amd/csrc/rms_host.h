@@ -21,6 +21,12 @@ enum : int { SIG_NARROW = 1, SIG_PGSE = 2, SIG_EXCH = 4, SIG_PERIODIC = 8 };
 // e = fma(s, v, o) with o in [0, 1], and e must round below 2 in FP32 and FP64.
 constexpr double STEP_MAX = 1.0 - 0x1p-20;
 
+// Allocation-free runs: capacity of the per-device scratch array and the
+// table sizes whose host work space lives on the stack.
+constexpr int64_t SCRATCH_BYTES = (int64_t)RMS_SCRATCH_BYTES;
+constexpr int32_t NOALLOC_MAX_GRAD = 2000;   // reference Ngrad_max (main_PGSE.cu:37)
+constexpr int32_t NOALLOC_MAX_TRIPLE = 512;  // distinct PGSE lobe triples
+
 struct Derived {
     double step[NC], inv_step[NC], T2s[NC], P[NC * NC];
     int Pcat[NC * NC];
@@ -69,5 +75,15 @@ const char *pgse_tables(const double *grad, int32_t ngrad, double dt, int64_t TN
 // Every table that walk_kernel<., sig, .> can read for this job is present
 // (the job's own or a sentinel) and consistent with its sizes.
 const char *check_desc(const JobDesc &D, int sig);
+
+// Byte offsets of one allocation-free run's buffers in the scratch array,
+// each 256-byte aligned.  A table the job does not have keeps a small
+// segment that holds its zero sentinel.
+struct ScratchLayout {
+    int64_t job, red, batch_job, counters, part, rep, tot, psig, srep, stot, snap, grad_raw, grad, perm, triple,
+        bounds, bval, qscale, bvec, end;
+};
+const char *scratch_layout(int64_t nbatch, int32_t timepoints, int32_t nval, int32_t replicas, int32_t ngrad,
+                           int32_t nbound, int32_t ntriple, int32_t nbvec, int64_t cap, ScratchLayout &L);
 
 }  // namespace rms

@@ -7,15 +7,6 @@
 
 namespace rms {
 
-// One ordered reduction: part[nb][nq] -> rep[R][nq] -> tot[nq].
-struct ReduceDesc {
-    const double *part;
-    double *rep;
-    double *tot;
-    int64_t nb;
-    int64_t nq;
-};
-
 hipError_t launch_walk(const LaunchArgs &a, int prec_f64, int sig, int grid, int block, hipStream_t s);
 hipError_t walk_occupancy(int prec_f64, int sig, int block, int *blocks_per_cu);
 hipError_t launch_reduce(const ReduceDesc *rd_dev, int32_t nred, int64_t max_nq, int32_t R, hipStream_t s);
@@ -28,5 +19,15 @@ hipError_t launch_build_cells(const void *labels, int32_t label_bytes, const uin
 // with the record/signal cotangents per walker.
 hipError_t launch_adjoint(const LaunchArgs &a, int prec_f64, int sig, int np, int grid, int block, hipStream_t s);
 hipError_t adjoint_occupancy(int prec_f64, int sig, int np, int block, int *blocks_per_cu);
+
+// Allocation-free runs: device address of this GPU's scratch array (the
+// current device's copy), and the two table kernels that fill it.
+hipError_t scratch_address(void **ptr);
+// dst[k][c] = src[perm[k]][c] for k < nrow, c < width
+hipError_t launch_gather_rows(const double *src, const int32_t *perm, int32_t nrow, int32_t width, double *dst,
+                              hipStream_t s);
+// out[r][j] = sqrt(bval[j] / ((r Tstep + 1) dt)), the q scale of main.cu:375
+hipError_t launch_qscale(const double *bval, int32_t nbvec, int32_t T, int64_t Tstep, double dt, double *out,
+                         hipStream_t s);
 
 }  // namespace rms

@@ -26,6 +26,7 @@
 
 #include <utility>
 
+#include "../../include/rms_amd.h"
 #include "rms_device.h"
 #include "rms_internal.h"
 #include "rms_turn_table.h"
@@ -1334,6 +1335,35 @@ __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, cons
     }
 }
 
+// Scratch of the allocation-free entry point (rms_run_noalloc): every device
+// buffer of such a run beyond its inputs and outputs.  One copy per GPU, held
+// for the life of the process.  In host code `scratch` is only a handle:
+// device addresses come from scratch_address(), once per device.
+static __device__ __align__(256) unsigned char scratch[RMS_SCRATCH_BYTES];
+
+hipError_t scratch_address(void **ptr) { return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)); }
+
+// Rows of the PGSE table in lobe-triple order: dst[k] = src[perm[k]].
+__global__ void gather_rows_kernel(const double *src, const int32_t *perm, int32_t nrow, int32_t width, double *dst)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nrow * width) return;
+    const int32_t k = (int32_t)(t / width), c = (int32_t)(t - (int64_t)k * width);
+    dst[t] = src[(int64_t)perm[k] * width + c];
+}
+
+// q scale of the narrow-pulse signal, sqrt(bval / TD) with TD = (r Tstep + 1) dt
+// (main.cu:375, 486); IEEE division and square root, as on the host.
+__global__ void qscale_kernel(const double *bval, int32_t nbvec, int32_t T, int64_t Tstep, double dt, double *out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)T * nbvec) return;
+    const int64_t r = t / nbvec;
+    const int32_t j = (int32_t)(t - r * nbvec);
+    const double TD = (double)(r * Tstep + 1) * dt;
+    out[t] = __dsqrt_rn(__ddiv_rn(bval[j], TD));
+}
+
 // ------------------------------------------------------------------ launch
 template <typename R, int NP, int... S>
 static const void *walk_fn_sig(int sig, std::integer_sequence<int, S...>)
@@ -1409,6 +1439,26 @@ hipError_t launch_build_cells(const void *labels, int32_t label_bytes, const uin
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(build_cells_kernel, dim3((unsigned)grid), dim3(256), 0, s, labels, label_bytes, lclass, nx, ny,
                        nz, cells, cls);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_rows(const double *src, const int32_t *perm, int32_t nrow, int32_t width, double *dst,
+                              hipStream_t s)
+{
+    const int64_t n = (int64_t)nrow * width;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, perm, nrow, width,
+                       dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_qscale(const double *bval, int32_t nbvec, int32_t T, int64_t Tstep, double dt, double *out,
+                         hipStream_t s)
+{
+    const int64_t n = (int64_t)T * nbvec;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(qscale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, bval, nbvec, T, Tstep, dt,
+                       out);
     return hipGetLastError();
 }
 

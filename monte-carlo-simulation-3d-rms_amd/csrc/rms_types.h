@@ -126,4 +126,13 @@ struct LaunchArgs {
     int32_t xcd_split;         // 1: partition batches per XCD
 };
 
+// One ordered reduction: part[nb][nq] -> rep[R][nq] -> tot[nq].
+struct ReduceDesc {
+    const double *part;
+    double *rep;
+    double *tot;
+    int64_t nb;
+    int64_t nq;
+};
+
 }  // namespace rms

@@ -167,6 +167,39 @@ class Engine:
         return Plan(self, jobs, dt, TN, timepoints, precision, replicas, chunk_steps, use_graph, block_threads,
                     grid_blocks, xcd_split)
 
+    def run_noalloc(self, job, dt, TN, timepoints=1000, precision="f32", replicas=32, block_threads=0, grid_blocks=0,
+                    xcd_split=0):
+        """One job through rms_run_noalloc: the walk of `run` with every device
+        buffer in the library's static scratch array (no device allocation;
+        the geometry is uploaded once, on first use, like `run` does)."""
+        geom = job["geom"]
+        gid = self.add_geometry(geom)
+        bval, bvec, grad = job.get("bval"), job.get("bvec"), job.get("grad")
+        bval = None if bval is None else np.ascontiguousarray(bval, np.float64).ravel()
+        bvec = None if bvec is None else np.ascontiguousarray(bvec, np.float64).reshape(-1, 3)
+        grad = None if grad is None else np.ascontiguousarray(grad, np.float64).reshape(-1, 6)
+        nb = 0 if bval is None else len(bval)
+        ng = 0 if grad is None else grad.shape[0]
+        cj = capi.Job(geom_id=gid, nwalk=int(job["nwalk"]), seed=int(job.get("seed", 1)) & 0xFFFFFFFFFFFFFFFF,
+                      nbvec=nb, bval=_ptr(bval), bvec=_ptr(bvec), ngrad=ng, grad=_ptr(grad),
+                      walker_base=int(job.get("walker_base", 0)))
+        R = max(1, int(replicas))
+        prm = capi.RunParams(dt=float(dt), TN=int(TN), timepoints=int(timepoints), precision=capi.PREC[precision],
+                             replicas=R, chunk_steps=0, use_graph=0, block_threads=int(block_threads),
+                             grid_blocks=int(grid_blocks), xcd_split=int(xcd_split))
+        nv = nval(geom.nclass, nb)
+        r = {"rec": np.zeros((int(timepoints), nv)), "rec_rep": np.zeros((R, int(timepoints), nv)),
+             "sig": np.zeros(ng) if ng else None, "sig_rep": np.zeros((R, ng)) if ng else None}
+        co = capi.JobOut()
+        co.rec, co.rec_rep = _ptr(r["rec"]), _ptr(r["rec_rep"])
+        co.sig, co.sig_rep = _ptr(r["sig"]), _ptr(r["sig_rep"])
+        tm = capi.Timing()
+        self._check(capi.lib().rms_run_noalloc(self._h, C.byref(cj), C.byref(prm), C.byref(co), C.byref(tm)),
+                    "rms_run_noalloc")
+        r["counters"] = np.array(co.counters[:], np.uint64)
+        r["timing"] = {f: getattr(tm, f) for f, _ in capi.Timing._fields_}
+        return split_records(r, geom.nclass, nb)
+
     def run(self, jobs, dt, TN, timepoints=1000, precision="f32", replicas=32, final_state=False, **kw):
         p = self.plan(jobs, dt, TN, timepoints, precision, replicas, **kw)
         try:

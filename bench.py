@@ -1,33 +1,43 @@
 """Throughput benchmark: walker-steps/s of the RMS random walk on one MI355X.
 
-Workload (one "step" of this benchmark = one full simulation run): a single
-synthetic beaded, undulating axon (r0 = 0.8 um, caliber CV 0.2, undulation
-0.5 um) voxelised at 0.1 um in a 64x64x1000 uint8 lattice; 1e6 intra-axonal
-walkers, D0 = 2 um^2/ms, dt = 2e-4 ms, 1e5 steps; PGSE table of 30
-directions x b in {0.5,1,2,3} ms/um^2 x Delta in {10,20,40} ms at delta = 5 ms
-(360 rows); FP32 in-voxel offsets with FP64 moment accumulation; moments
-recorded every 100 steps (1000 records, as the reference's main.cu).
+Workload of the headline number (one "step" of this benchmark = one full
+simulation run): a single synthetic beaded, undulating axon (r0 = 0.8 um,
+caliber CV 0.2, undulation 0.5 um) voxelised at 0.1 um in a 64x64x1000 uint8
+periodic lattice; 1e6 intra-axonal walkers, D0 = 2 um^2/ms, dt = 2e-4 ms, 1e5
+steps; PGSE table of 30 directions x b in {0.5,1,2,3} ms/um^2 x Delta in
+{10,20,40} ms at delta = 5 ms (360 rows); FP32 in-voxel offsets with FP64
+moment accumulation; moments recorded every 100 steps (1000 records, as the
+reference's main.cu).
 
 Prints one JSON line with the throughput, the kernel's position against the
-VALU roofline and the CPU baseline (the oracle, FP64 reference semantics, on
-a bounded walker subset of the same workload, timed on this host).
+VALU roofline and the CPU baseline (the oracle, FP64 reference semantics,
+built for this host and run on all its cores, on bounded samples of every
+workload).
 
-Beside the plain form the line carries ("forms") the two other forms of the
-metric, each with the standard errors of D(t_end), K(t_end) and S from 32
-batch-means replicas:
-  batched   256 axons with extra-axonal shells (gamma radii mean 0.6 um,
-            CV 0.4, lengths 20-200 um, uneven uint16 lattices), kappa drawn in
-            [0, 0.02] um/ms, T2 80/60 ms, D 2.0/1.5 um^2/ms, 1e7 walkers in
-            total, 5e4 steps, the PGSE table above, one persistent launch with
-            the autotuned launch shape and geometry order;
-  gradient  the axon above with extra-axonal space (label 2 outside), kappa
-            0.01 um/ms, 1e6 walkers, 5e4 steps, 20 records, narrow-pulse
-            b-table of 64 directions x b in {0.5,1,2,3} ms/um^2: forward pass
-            plus the adjoint pass of S(b, g, t) and D(t), K(t) along z.
-On request (--forms sweep, not in the default run) the parameter sweep: 4096
-axons with extra-axonal space in 70x70x1000 uint16 lattices (caliber CV,
-undulation and kappa drawn per geometry, ~20e9 voxels, ~180 GB of cell words
-and classes in HBM), 4e7 walkers, 2e4 steps, batched forward plus adjoint.
+Beside the headline the line carries ("forms") every workload of the metric,
+each with the standard error of D(t_end) (and K, S) from 32 batch-means
+replicas and the cost per geometry of reaching a 1 % standard error of
+D(t_end) on this GPU (walkers needed x steps / rate), the figure that puts the
+forms side by side at matched Monte Carlo error:
+  plain         the headline run (periodic lattice edges)
+  plain_mirror  the same with the reference's mirror edges (main.cu:246-265)
+  planes        impermeable planes 5 um apart, 32x32x52 uint8, x/y periodic,
+                1e4 walkers x 1e4 steps, records every 100 steps
+  batched       256 axons with extra-axonal shells (gamma radii mean 0.6 um,
+                CV 0.4, lengths 20-200 um, uneven uint16 lattices), kappa in
+                [0, 0.02] um/ms, T2 80/60 ms, D 2.0/1.5 um^2/ms, 1e7 walkers,
+                5e4 steps, the PGSE table, one persistent launch with the
+                autotuned shape and order, replayed as a captured hipGraph
+  gradient      the axon with extra-axonal space, kappa 0.01 um/ms, 1e6
+                walkers, 5e4 steps, 20 records, narrow-pulse 64 directions x b
+                in {0.5,1,2,3}: forward pass plus the adjoint pass of
+                sum_t [D_t(z) + K_t(z) + sum_j S_j(t)]
+  sweep         the parameter sweep (caliber CV, undulation and kappa drawn per
+                geometry, 70x70x1000 uint16 lattices, 2e4 steps, 20 records,
+                batched forward plus adjoint) on its first --sweep-geoms of the
+                4096 geometries (default 512) with the full run's 9766 walkers
+                per geometry; --sweep-geoms 4096 runs all of it (~40 GB of
+                labels, a few minutes of setup)
 """
 from __future__ import annotations
 
@@ -35,6 +45,7 @@ import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -46,10 +57,10 @@ sys.path.insert(0, ROOT)
 # VALU peak of MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz (non-packed lane ops/s)
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the walk kernel on this workload, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE passes scaled by a 1 GiB device copy (x2.0 reads, x1.0 writes); the summary with the
-# commands is profiles/walk_kernel_axon_pgse_f32.md.  Keyed by (precision, walkers, steps).
-MEASURED_TRAFFIC = {("f32", 1_000_000, 100_000): 2.007e10}
+# HBM bytes per launch of the walk kernel on the headline workload, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE passes scaled by a 1 GiB device copy (x2.0 reads, x1.0 writes); the summaries with the
+# commands are in profiles/.  Keyed by (precision, walkers, steps, periodic axes).
+MEASURED_TRAFFIC = {("f32", 1_000_000, 100_000, ""): 2.007e10, ("f32", 1_000_000, 100_000, "xyz"): 2.127e10}
 
 
 def ops_per_step(s, pgse=True):
@@ -59,22 +70,55 @@ def ops_per_step(s, pgse=True):
     return 136 + 28 * (1 + 1.5 * s) + 9 * s + (7 if pgse else 0)
 
 
-def pgse_table_c2():
-    """30 directions x 4 b-values x 3 Delta at delta = 5 ms (rows Delta,
-    delta, |g|, gx, gy, gz; b = g^2 delta^2 (Delta - delta/3))."""
-    n = 30
-    i = np.arange(n) + 0.5
-    phi = np.arccos(1 - 2 * i / n)
-    th = np.pi * (1 + 5 ** 0.5) * i
-    dirs = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
-    rows = []
-    delta = 5.0
-    for Delta in (10.0, 20.0, 40.0):
-        for b in (0.5, 1.0, 2.0, 3.0):
-            g = math.sqrt(b / (delta ** 2 * (Delta - delta / 3)))
-            for d in dirs:
-                rows.append([Delta, delta, g, *d])
-    return np.array(rows)
+def _jackknife(fn, rec_rep, sig_rep=None):
+    """Delete-one jackknife over the batch-means replicas of fn(rec, sig)."""
+    R = rec_rep.shape[0]
+    rt = rec_rep.sum(0)
+    st = None if sig_rep is None else sig_rep.sum(0)
+    est = np.asarray(fn(rt, st))
+    loo = np.stack([np.asarray(fn(rt - rec_rep[r], None if st is None else st - sig_rep[r])) for r in range(R)])
+    se = np.sqrt((R - 1) / R * np.sum((loo - loo.mean(0)) ** 2, axis=0))
+    return est, se
+
+
+def _stat_summary(A, r, dt, TN, T, nclass, ngrad, axis=(0.0, 0.0, 1.0)):
+    """D and K along `axis` at the last record and the first PGSE signals
+    S = sig / sig0(t=0), with jackknife standard errors."""
+    TD = A.diff_times(dt, TN, T)
+
+    def fn(rec, sig):
+        s = A.split(rec, nclass)
+        K, D = A.akc_mom(s["sig0"], s["dx2"], s["dx4"], TD, list(axis))
+        out = [D[-1, 0], K[-1, 0]]
+        if sig is not None:
+            out += list(sig[:3] / rec[0, 0])
+        return np.array(out)
+
+    est, se = _jackknife(fn, r["rec_rep"], r.get("sig_rep") if ngrad else None)
+    out = {"D_end": float(est[0]), "D_end_se": float(se[0]), "K_end": float(est[1]), "K_end_se": float(se[1])}
+    if ngrad:
+        out["S_first"] = [float(x) for x in est[2:]]
+        out["S_first_se"] = [float(x) for x in se[2:]]
+    return out
+
+
+def _se_cost(rate, walkers_per_geom, TN, rel_se, target=0.01):
+    """Seconds of this GPU per geometry to reach a relative standard error
+    `target` of D(t_end): walkers needed N (rel_se / target)^2, times TN
+    steps, at the form's walker-steps/s."""
+    return float(walkers_per_geom * (rel_se / target) ** 2 * TN / rate)
+
+
+def _timed_runs(plan, n, want_last=True):
+    import torch
+    ks = []
+    t0 = time.perf_counter()
+    r = None
+    for k in range(n):
+        r = plan.execute(want_replicas=(want_last and k == n - 1))
+        ks.append(plan.timing["walk_ms"])
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t0, float(np.mean(ks))
 
 
 def main():
@@ -85,20 +129,19 @@ def main():
     ap.add_argument("--walkers", type=int, default=1_000_000)
     ap.add_argument("--tn", type=int, default=100_000)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--cpu-walkers", type=int, default=10_000)
-    ap.add_argument("--cpu-tn", type=int, default=0, help="CPU baseline steps (0 = same as --tn)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--graph", action="store_true", help="replay the run as a captured hipGraph")
-    ap.add_argument("--periodic", default="", help="axes of the single-axon lattice whose edges wrap around "
-                    "(default: the reference's mirrors)")
-    ap.add_argument("--forms", default="plain,batched,gradient",
-                    help="comma list of forms to time beside the plain form (also: sweep)")
+    ap.add_argument("--cpu-configs", default="c1,c2,c3,c4,c5")
+    ap.add_argument("--graph", action="store_true", help="replay the headline run as a captured hipGraph")
+    ap.add_argument("--periodic", default="xyz", help="axes of the single-axon lattice whose edges wrap around "
+                    "(the workload's periodic lattice); '' gives the reference's mirrors")
+    ap.add_argument("--forms", default="plain_mirror,planes,batched,gradient,sweep",
+                    help="comma list of forms to time beside the headline")
     ap.add_argument("--batched-walkers", type=int, default=10_000_000)
     ap.add_argument("--batched-tn", type=int, default=50_000)
     ap.add_argument("--grad-walkers", type=int, default=1_000_000)
     ap.add_argument("--grad-tn", type=int, default=50_000)
-    ap.add_argument("--sweep-geoms", type=int, default=4096)
-    ap.add_argument("--sweep-walkers", type=int, default=40_000_000)
+    ap.add_argument("--sweep-geoms", type=int, default=512)
+    ap.add_argument("--sweep-walkers-per-geom", type=int, default=40_000_000 // 4096)
     ap.add_argument("--sweep-tn", type=int, default=20_000)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration) and exit")
@@ -123,21 +166,20 @@ def main():
     import __graft_entry__ as ge
 
     rms = ge.load_package()
-    from rms_amd import geometry as G
+    from rms_amd import analysis as A
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
-    dt, D0, res = 2e-4, 2.0, 0.1
-    lab = G.axon(n=64, nz=1000, res=res, r0=0.8, cv=0.2, undulation=0.5, seed=1235)
-    grad = pgse_table_c2()
+    W = rms.workloads
+    dt, D0, res = W.DT, W.C2["D0"], W.RES
+    lab = W.c2_labels()
+    grad = W.pgse_table_c2()
     TN = args.tn
     timepoints = min(1000, TN)
-    geom = rms.Geometry(lab, res, [0.0, D0], T2=[math.inf, math.inf], periodic=args.periodic)
     s_vox = math.sqrt(6 * dt * D0) / res
     ops = ops_per_step(s_vox, pgse=True)
-    job = dict(geom=geom, nwalk=args.walkers, seed=0x5EED0001, grad=grad)
     eng = rms.Engine(0)
-    plan = eng.plan([job], dt, TN, timepoints=timepoints, precision=args.precision, replicas=32,
-                    use_graph=args.graph)
+    jobs = W.c2_jobs(rms.Geometry, args.walkers, args.periodic, lab)
+    plan = eng.plan(jobs, dt, TN, timepoints=timepoints, precision=args.precision, replicas=32, use_graph=args.graph)
     for _ in range(args.warmup):
         plan.execute(want_replicas=False)
     torch.cuda.synchronize()
@@ -153,14 +195,14 @@ def main():
     kern_ms = float(np.mean(walk_ms))
     kern_rate = steps_total / (kern_ms / 1e3)
     achieved = kern_rate * ops / 1e12
-    traffic = MEASURED_TRAFFIC.get((args.precision, args.walkers, TN))
+    traffic = MEASURED_TRAFFIC.get((args.precision, args.walkers, TN, "".join(sorted(args.periodic))))
     res0 = r[0]
     counters = res0["counters"].tolist()
-    # physics summary of the last run (SE from 32 batch-means replicas)
-    from rms_amd import analysis as A
     TD = A.diff_times(dt, TN, timepoints)
     K, D = A.akc_mom(res0["sig0"], res0["dx2"], res0["dx4"], TD, [0, 0, 1])
     S = res0["sig"] / res0["rec"][-1, 0]
+    edges = f"periodic {args.periodic}" if args.periodic else "mirror (reference)"
+    plain_stats = _stat_summary(A, res0, dt, TN, timepoints, 2, grad.shape[0])
     line = {
         "metric": "walker-steps/s on 1 MI355X (plain vs batched vs +gradient) at fixed MC std-error",
         "value": value,
@@ -178,7 +220,7 @@ def main():
             "workload": "single beaded undulating axon, 64x64x1000 uint8 lattice at 0.1 um, "
                         f"{args.walkers} walkers x {TN} steps, PGSE 360 rows, plain form",
             "walkers": args.walkers, "TN": TN, "dt_ms": dt, "D0": D0, "records": timepoints,
-            "lattice_edges": f"periodic {args.periodic}" if args.periodic else "mirror (reference)",
+            "lattice_edges": edges,
             "positions": args.precision + " in-voxel offsets + int32 voxel", "accumulation": "fp64",
             "graph": bool(args.graph), "grid": plan.timing["grid"], "block": plan.timing["block"],
         },
@@ -196,69 +238,82 @@ def main():
                     "S_first_rows": [float(x) for x in S[:3]], "counters": counters},
     }
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
-    line["forms"] = {"plain": dict(walker_steps_per_s=value, kernel_ms=kern_ms,
-                                   **_stat_summary(A, res0, dt, TN, timepoints, 2, grad.shape[0]))}
+    rel = plain_stats["D_end_se"] / plain_stats["D_end"]
+    line["forms"] = {"plain": dict(walker_steps_per_s=value, kernel_ms=kern_ms, geometries=1,
+                                   walkers_per_geometry=args.walkers, TN=TN, lattice_edges=edges,
+                                   roofline_frac=achieved / VALU_PEAK_TOPS, ops_per_walker_step=ops,
+                                   D_end_rel_se=rel,
+                                   seconds_per_geometry_at_1pct_se=_se_cost(value, args.walkers, TN, rel),
+                                   **plain_stats)}
     plan.close()
-    if "batched" in forms:
-        line["forms"]["batched"] = form_batched(rms, eng, args, dt, grad)
-    if "gradient" in forms:
-        line["forms"]["gradient"] = form_gradient(rms, eng, args, dt)
-    if "sweep" in forms:
-        line["forms"]["sweep"] = form_sweep(rms, eng, args, dt)
+    print(f"bench: plain {value:.4g} walker-steps/s", file=sys.stderr, flush=True)
+    form_fns = {"plain_mirror": form_plain_mirror, "planes": form_planes, "batched": form_batched,
+                "gradient": form_gradient, "sweep": form_sweep}
+    for f in forms:
+        if f in form_fns:
+            line["forms"][f] = form_fns[f](rms, eng, args, A)
+            print(f"bench: {f} done", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(lab, res, D0, dt, grad, args)
+        line["cpu_baseline"] = cpu_baseline(args, line)
     print(json.dumps(line), flush=True)
     eng.close()
 
 
-def _jackknife(fn, rec_rep, sig_rep=None):
-    """Delete-one jackknife over the batch-means replicas of fn(rec, sig)."""
-    R = rec_rep.shape[0]
-    rt = rec_rep.sum(0)
-    st = None if sig_rep is None else sig_rep.sum(0)
-    est = np.asarray(fn(rt, st))
-    loo = np.stack([np.asarray(fn(rt - rec_rep[r], None if st is None else st - sig_rep[r])) for r in range(R)])
-    se = np.sqrt((R - 1) / R * np.sum((loo - loo.mean(0)) ** 2, axis=0))
-    return est, se
+def form_plain_mirror(rms, eng, args, A):
+    """The headline workload with the reference's mirror edges."""
+    W = rms.workloads
+    dt, TN, T = W.DT, args.tn, min(1000, args.tn)
+    plan = eng.plan(W.c2_jobs(rms.Geometry, args.walkers, ""), dt, TN, timepoints=T, precision=args.precision,
+                    replicas=32)
+    plan.execute(want_replicas=False)
+    n = max(2, min(args.steps, 3))
+    r, el, kms = _timed_runs(plan, n)
+    plan.close()
+    rate = float(args.walkers) * TN * n / el
+    st = _stat_summary(A, r[0], dt, TN, T, 2, 360)
+    ops = ops_per_step(math.sqrt(6 * dt * W.C2["D0"]) / W.RES)
+    krate = float(args.walkers) * TN / (kms / 1e3)
+    rel = st["D_end_se"] / st["D_end"]
+    return dict(walker_steps_per_s=rate, kernel_ms=kms, kernel_walker_steps_per_s=krate, geometries=1,
+                walkers_per_geometry=args.walkers, TN=TN, lattice_edges="mirror (reference)",
+                roofline_frac=krate * ops / 1e12 / VALU_PEAK_TOPS, D_end_rel_se=rel,
+                seconds_per_geometry_at_1pct_se=_se_cost(rate, args.walkers, TN, rel), **st)
 
 
-def _stat_summary(A, r, dt, TN, T, nclass, ngrad):
-    """D and K along z at the last record and the first PGSE signals S =
-    sig / sig0(t=0), with jackknife standard errors."""
-    TD = A.diff_times(dt, TN, T)
+def form_planes(rms, eng, args, A):
+    """Impermeable planes at full size (1e4 walkers x 1e4 steps, FP32 offsets):
+    a latency-bound launch of 157 wavefronts; D_perp, D_par and K_perp at the
+    last record."""
+    W = rms.workloads
+    c = W.C1
+    plan = eng.plan(W.c1_jobs(rms.Geometry), W.DT, c["TN"], timepoints=c["timepoints"], precision=args.precision,
+                    replicas=32)
+    plan.execute(want_replicas=False)
+    n = max(3, args.steps)
+    r, el, kms = _timed_runs(plan, n)
+    plan.close()
+    ws = float(c["nwalk"]) * c["TN"]
+    st_perp = _stat_summary(A, r[0], W.DT, c["TN"], c["timepoints"], 2, 0, axis=(0.0, 0.0, 1.0))
+    st_par = _stat_summary(A, r[0], W.DT, c["TN"], c["timepoints"], 2, 0, axis=(1.0, 0.0, 0.0))
+    rate = ws * n / el
+    rel = st_perp["D_end_se"] / st_perp["D_end"]
+    return dict(walker_steps_per_s=rate, kernel_ms=kms, kernel_walker_steps_per_s=ws / (kms / 1e3), geometries=1,
+                walkers_per_geometry=c["nwalk"], TN=c["TN"], records=c["timepoints"],
+                lattice="32x32x52 uint8, label-0 planes at z = 0, 51 (gap 5.0 um), x/y periodic",
+                D_perp_end=st_perp["D_end"], D_perp_end_se=st_perp["D_end_se"], K_perp_end=st_perp["K_end"],
+                K_perp_end_se=st_perp["K_end_se"], D_par_end=st_par["D_end"], D_par_end_se=st_par["D_end_se"],
+                D_end_rel_se=rel, seconds_per_geometry_at_1pct_se=_se_cost(rate, c["nwalk"], c["TN"], rel))
 
-    def fn(rec, sig):
-        s = A.split(rec, nclass)
-        K, D = A.akc_mom(s["sig0"], s["dx2"], s["dx4"], TD, [0.0, 0.0, 1.0])
-        out = [D[-1, 0], K[-1, 0]]
-        if sig is not None:
-            out += list(sig[:3] / rec[0, 0])
-        return np.array(out)
 
-    est, se = _jackknife(fn, r["rec_rep"], r.get("sig_rep") if ngrad else None)
-    out = {"D_end": float(est[0]), "D_end_se": float(se[0]), "K_end": float(est[1]), "K_end_se": float(se[1])}
-    if ngrad:
-        out["S_first"] = [float(x) for x in est[2:]]
-        out["S_first_se"] = [float(x) for x in se[2:]]
-    return out
-
-
-def form_batched(rms, eng, args, dt, grad):
-    """Bundle of 256 axons in one persistent launch (autotuned shape/order)."""
-    import torch
-    from rms_amd import analysis as A
+def form_batched(rms, eng, args, A):
+    """Bundle of 256 axons in one persistent launch (autotuned shape and
+    order), replayed as a captured hipGraph."""
     from rms_amd import autotune
-    from rms_amd import geometry as G
-
-    labs = G.bundle(256, mean_r=0.6, cv_r=0.4, len_lo=20.0, len_hi=200.0, res=0.1, seed=1236)
-    rng = np.random.default_rng(2024)
-    kap = rng.uniform(0.0, 0.02, len(labs))
-    walkable = np.array([np.count_nonzero(l) for l in labs], float)
-    nw = np.maximum(64, np.round(args.batched_walkers * walkable / walkable.sum())).astype(np.int64)
-    jobs = [dict(geom=rms.Geometry(l, 0.1, [0.0, 2.0, 1.5], T2=[math.inf, 80.0, 60.0], kappa=float(k)),
-                 nwalk=int(w), seed=0x5EED0002 | (i << 32), grad=grad) for i, (l, k, w) in enumerate(zip(labs, kap, nw))]
-    TN = args.batched_tn
+    W = rms.workloads
+    dt, TN = W.DT, args.batched_tn
     T = min(1000, TN)
+    jobs = W.c3_jobs(rms.Geometry, args.batched_walkers)
+    nw = np.array([j["nwalk"] for j in jobs])
     t0 = time.perf_counter()
     best = autotune.tune(eng, jobs, dt, timepoints=10, precision=args.precision, probe_steps=200, reps=1,
                          use_cache=False)
@@ -266,62 +321,45 @@ def form_batched(rms, eng, args, dt, grad):
     ordered = [jobs[k] for k in best["order"]]
     plan = eng.plan(ordered, dt, TN, timepoints=T, precision=args.precision, replicas=32,
                     block_threads=best["block_threads"], grid_blocks=best["grid_blocks"],
-                    xcd_split=best["xcd_split"])
+                    xcd_split=best["xcd_split"], use_graph=True)
     plan.execute(want_replicas=False)
-    torch.cuda.synchronize()
     n = max(1, min(args.steps, 2))
-    ks = []
-    t0 = time.perf_counter()
-    for k in range(n):
-        r = plan.execute(want_replicas=(k == n - 1))
-        ks.append(plan.timing["walk_ms"])
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
+    r, el, kms = _timed_runs(plan, n)
     plan.close()
     total = float(nw.sum()) * TN
-    stats = [_stat_summary(A, rr, dt, TN, T, 3, grad.shape[0]) for rr in r]
+    stats = [_stat_summary(A, rr, dt, TN, T, 3, 360) for rr in r]
     rel = np.array([s["D_end_se"] / s["D_end"] for s in stats])
+    s = np.array([math.sqrt(6 * dt * 2.0) / W.RES, math.sqrt(6 * dt * 1.5) / W.RES])
+    ops = ops_per_step(float(s.mean()))  # the two compartments' steps, equal weight
+    rate = total * n / el
     return {
-        "walker_steps_per_s": total * n / el, "kernel_ms": float(np.mean(ks)),
-        "kernel_walker_steps_per_s": total / (np.mean(ks) / 1e3),
-        "geometries": len(jobs), "walkers": int(nw.sum()), "TN": TN, "records": T,
-        "voxels": int(sum(l.size for l in labs)), "label_bytes": 2,
+        "walker_steps_per_s": rate, "kernel_ms": kms,
+        "kernel_walker_steps_per_s": total / (kms / 1e3), "graph": True,
+        "roofline_frac_estimate": total / (kms / 1e3) * ops / 1e12 / VALU_PEAK_TOPS,
+        "geometries": len(jobs), "walkers": int(nw.sum()), "walkers_per_geometry_median": float(np.median(nw)),
+        "TN": TN, "records": T, "voxels": int(sum(j["geom"].labels.size for j in jobs)), "label_bytes": 2,
         "autotune": {k: best[k] for k in ("block_threads", "grid_blocks", "xcd_split", "pack")},
         "autotune_s": tune_s,
-        "geometry0": stats[best["order"].index(0)] if 0 in best["order"] else stats[0],
+        "geometry0": stats[best["order"].index(0)],
         "D_end_rel_se_median": float(np.median(rel)),
+        "seconds_per_geometry_at_1pct_se": float(np.median([_se_cost(rate, w, TN, e) for w, e in zip(nw, rel)])),
     }
 
 
-def form_gradient(rms, eng, args, dt):
+def form_gradient(rms, eng, args, A):
     """Forward plus adjoint pass on the axon with extra-axonal space."""
     import torch
-    from rms_amd import analysis as A
-    from rms_amd import geometry as G
-
-    lab = G.axon(n=64, nz=1000, res=0.1, r0=0.8, cv=0.2, undulation=0.5, seed=1235, ecs=True)
-    n = 64
-    i = np.arange(n) + 0.5
-    phi = np.arccos(1 - 2 * i / n)
-    th = np.pi * (1 + 5 ** 0.5) * i
-    dirs = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
-    bvals = np.repeat([0.5, 1.0, 2.0, 3.0], n)
-    bvecs = np.tile(dirs, (4, 1))
-    nb = bvals.size
-    geom = rms.Geometry(lab, 0.1, [0.0, 2.0, 1.5], T2=[math.inf, 80.0, 60.0], kappa=0.01)
-    TN, T, W = args.grad_tn, 20, args.grad_walkers
+    W = rms.workloads
+    c = W.C4
+    dt, TN, T, Wn = W.DT, args.grad_tn, c["timepoints"], args.grad_walkers
+    jobs = W.c4_jobs(rms.Geometry, Wn)
+    nb = int(jobs[0]["bval"].size)
     TD = A.diff_times(dt, TN, T)
-    plan = eng.plan([dict(geom=geom, nwalk=W, seed=0x5EED0003, bval=bvals, bvec=bvecs)], dt, TN, timepoints=T,
-                    precision=args.precision, replicas=32)
-    i_re = 1 + 3 + 21
+    plan = eng.plan(jobs, dt, TN, timepoints=T, precision=args.precision, replicas=32)
 
     def once(want):
         out = plan.execute(want_replicas=want)[0]
-        rec = out["rec"]
-        # objective: sum_t [D_t(z) + K_t(z) + sum_j S_j(t)], S = sigRe / sig0
-        cot = A.akc_cotangent(rec, 3, TD, [0.0, 0.0, 1.0])
-        cot[:, i_re:i_re + nb] += (1.0 / rec[:, 0])[:, None]
-        cot[:, 0] += -rec[:, i_re:i_re + nb].sum(1) / rec[:, 0] ** 2
+        cot = W.c4_cotangent(A, out["rec"], 3, TD, nb)
         adj = plan.adjoint([cot], None, want_replicas=want)[0]
         return out, adj
 
@@ -341,77 +379,58 @@ def form_gradient(rms, eng, args, dt):
     se = np.sqrt(32) * np.std(adj["grad_rep"], axis=0, ddof=1)
     names = {"D_intra": 1, "D_extra": 2, "T2_intra": 5, "T2_extra": 6, "kappa": 8}
     fin = lambda x: None if not np.isfinite(x) else float(x)  # noqa: E731
-    TDs = A.diff_times(dt, TN, T)
 
     def fS(rec, sig):
         s = A.split(rec, 3, nb)
-        K, D = A.akc_mom(s["sig0"], s["dx2"], s["dx4"], TDs, [0.0, 0.0, 1.0])
+        K, D = A.akc_mom(s["sig0"], s["dx2"], s["dx4"], TD, [0.0, 0.0, 1.0])
         S = A.narrow_pulse_signal(s["sig0"], s["sigRe"])
-        return np.array([D[-1, 0], K[-1, 0], S[-1, n], S[-1, 2 * n]])
+        return np.array([D[-1, 0], K[-1, 0], S[-1, c["ndir"]], S[-1, 2 * c["ndir"]]])
 
     est, ses = _jackknife(fS, out["rec_rep"])
+    rate = float(Wn) * TN * nrun / el
+    s = np.array([math.sqrt(6 * dt * 2.0) / W.RES, math.sqrt(6 * dt * 1.5) / W.RES])
+    ops_adj = 2 * ops_per_step(float(s.mean()), pgse=False) + 20 * 2
+    kern = float(Wn) * TN / ((np.mean(fwd_ms) + np.mean(adj_ms)) / 1e3)
+    rel = float(ses[0] / est[0])
     return {
-        "walker_steps_per_s": float(W) * TN * nrun / el,
+        "walker_steps_per_s": rate, "kernel_walker_steps_per_s": kern,
         "forward_kernel_ms": float(np.mean(fwd_ms)), "adjoint_kernel_ms": float(np.mean(adj_ms)),
-        "walkers": W, "TN": TN, "records": T, "bvecs": int(nb), "kappa": 0.01,
+        "roofline_frac_estimate": kern * ops_adj / 1e12 / VALU_PEAK_TOPS,
+        "geometries": 1, "walkers_per_geometry": Wn, "TN": TN, "records": T, "bvecs": nb, "kappa": c["kappa"],
         "objective": "sum_t [D_t(z) + K_t(z) + sum_j S_j(t)], narrow-pulse S = sigRe/sig0",
         "gradient": {k: fin(gr[v]) for k, v in names.items()},
         "gradient_se": {k: fin(se[v]) for k, v in names.items()},
-        "note": "D entries are undefined (NaN) with membrane exchange: a class change at a crossing is a jump "
-                "the pathwise derivative does not contain",
         "D_end": float(est[0]), "D_end_se": float(ses[0]), "K_end": float(est[1]), "K_end_se": float(ses[1]),
         "S_end_b1_first": float(est[2]), "S_end_b1_first_se": float(ses[2]),
         "S_end_b2_first": float(est[3]), "S_end_b2_first_se": float(ses[3]),
+        "D_end_rel_se": rel, "seconds_per_geometry_at_1pct_se": _se_cost(rate, Wn, TN, rel),
     }
 
 
-def form_sweep(rms, eng, args, dt):
-    """Parameter sweep: many axons with extra-axonal space, caliber CV in
-    [0, 0.4], undulation amplitude in [0, 1] um and kappa in [0, 0.03] um/ms
-    drawn per geometry, 70x70x1000 uint16 lattices, all in one batched forward
-    launch plus its adjoint (D(t), K(t) along z and narrow-pulse S per
-    geometry, 20 records, 16 directions x b in {1, 2} ms/um^2)."""
+def form_sweep(rms, eng, args, A):
+    """Parameter sweep: the first --sweep-geoms of the 4096 geometries, each
+    with the full run's walkers per geometry, in one batched forward launch
+    plus its adjoint."""
     import torch
-    from rms_amd import analysis as A
-    from rms_amd import geometry as G
-
-    ng = args.sweep_geoms
-    rng = np.random.default_rng(1238)
-    cv, und, kap = rng.uniform(0.0, 0.4, ng), rng.uniform(0.0, 1.0, ng), rng.uniform(0.0, 0.03, ng)
-    n = 16
-    i = np.arange(n) + 0.5
-    phi = np.arccos(1 - 2 * i / n)
-    th = np.pi * (1 + 5 ** 0.5) * i
-    dirs = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
-    bvals, bvecs = np.repeat([1.0, 2.0], n), np.tile(dirs, (2, 1))
-    nb = bvals.size
-    nw = max(64, args.sweep_walkers // ng)
-    TN, T = args.sweep_tn, 20
-    jobs, voxels = [], 0
+    W = rms.workloads
+    c = W.C5
+    ng, nwg, TN, T = args.sweep_geoms, args.sweep_walkers_per_geom, args.sweep_tn, c["timepoints"]
     t0 = time.perf_counter()
-    for k in range(ng):
-        lab = G.axon(n=70, nz=1000, res=0.1, r0=0.8, cv=float(cv[k]), undulation=float(und[k]), seed=1238 + k,
-                     ecs=True, dtype=np.uint16)
-        geom = rms.Geometry(lab, 0.1, [0.0, 2.0, 1.5], T2=[math.inf, 80.0, 60.0], kappa=float(kap[k]))
-        eng.add_geometry(geom)
-        voxels += lab.size
-        jobs.append(dict(geom=geom, nwalk=nw, seed=0x5EED0004 | (k << 32), bval=bvals, bvec=bvecs))
-        if (k + 1) % 256 == 0:
-            print(f"sweep: {k + 1}/{ng} geometries uploaded ({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
-                  flush=True)
+    jobs = []
+    for k0 in range(0, ng, 64):
+        for jb in W.c5_jobs(rms.Geometry, range(k0, min(ng, k0 + 64)), nwg):
+            eng.add_geometry(jb["geom"])
+            jb["geom"].labels = jb["geom"].labels[:0, :0, :0]  # the device holds the lattice now
+            jobs.append(jb)
+        print(f"sweep: {len(jobs)}/{ng} geometries uploaded ({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
+              flush=True)
     setup_s = time.perf_counter() - t0
-    TD = A.diff_times(dt, TN, T)
-    i_re = 1 + 3 + 21
-    plan = eng.plan(jobs, dt, TN, timepoints=T, precision=args.precision, replicas=32)
+    TD = A.diff_times(W.DT, TN, T)
+    nb = int(jobs[0]["bval"].size)
+    plan = eng.plan(jobs, W.DT, TN, timepoints=T, precision=args.precision, replicas=32)
     t0 = time.perf_counter()
-    outs = plan.execute(want_replicas=False)
-    cots = []
-    for o in outs:
-        rec = o["rec"]
-        cot = A.akc_cotangent(rec, 3, TD, [0.0, 0.0, 1.0])
-        cot[:, i_re:i_re + nb] += (1.0 / rec[:, 0])[:, None]
-        cot[:, 0] += -rec[:, i_re:i_re + nb].sum(1) / rec[:, 0] ** 2
-        cots.append(cot)
+    outs = plan.execute(want_replicas=True)
+    cots = [W.c4_cotangent(A, o["rec"], 3, TD, nb) for o in outs]
     adj = plan.adjoint(cots, None)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
@@ -419,38 +438,77 @@ def form_sweep(rms, eng, args, dt):
     plan.close()
     dk = np.array([a["grad"][8] for a in adj])
     dT2 = np.array([a["grad"][5] for a in adj])
-    Dend = []
-    for o in outs:
-        s = A.split(o["rec"], 3, nb)
-        K, D = A.akc_mom(s["sig0"], s["dx2"], s["dx4"], TD, [0.0, 0.0, 1.0])
-        Dend.append(D[-1, 0])
-    total = float(nw) * ng * TN
+    stats = [_stat_summary(A, o, W.DT, TN, T, 3, 0) for o in outs]
+    Dend = np.array([s["D_end"] for s in stats])
+    rel = np.array([s["D_end_se"] / s["D_end"] for s in stats])
+    total = float(nwg) * ng * TN
+    rate = total / el
     return {
-        "walker_steps_per_s": total / el, "forward_walker_steps_per_s": total / (fwd_ms / 1e3),
+        "walker_steps_per_s": rate, "forward_walker_steps_per_s": total / (fwd_ms / 1e3),
         "forward_kernel_ms": float(fwd_ms), "adjoint_kernel_ms": float(adj_ms),
-        "geometries": ng, "walkers": int(nw * ng), "TN": TN, "records": T, "bvecs": int(nb),
-        "voxels": int(voxels), "label_bytes": 2, "lattice_bytes_in_hbm": int(voxels * 9),
-        "setup_s": setup_s,
+        "geometries": ng, "of_geometries": c["ngeom"], "walkers_per_geometry": nwg, "walkers": int(nwg * ng),
+        "TN": TN, "records": T, "bvecs": nb, "voxels": ng * 70 * 70 * 1000, "label_bytes": 2,
+        "lattice_bytes_in_hbm": ng * 70 * 70 * 1000 * 9, "setup_s": setup_s,
         "D_end_median": float(np.median(Dend)), "D_end_range": [float(np.min(Dend)), float(np.max(Dend))],
-        "dobj_dkappa_median": float(np.median(dk)), "dobj_dT2_intra_median": float(np.median(dT2)),
+        "dobj_dkappa_median": float(np.nanmedian(dk)), "dobj_dT2_intra_median": float(np.median(dT2)),
+        "D_end_rel_se_median": float(np.median(rel)),
+        "seconds_per_geometry_at_1pct_se": _se_cost(rate, nwg, TN, float(np.median(rel))),
     }
 
 
-def cpu_baseline(lab, res, D0, dt, grad, args):
-    """The oracle (FP64 reference semantics, OpenMP over 64-walker batches) on
-    a walker subset of the same workload, timed on this host."""
-    from oracle import oracle as orc
+def _cpu_threads():
+    """CPUs this process may use: its affinity mask, capped by a cgroup quota
+    and by OMP_NUM_THREADS when those are set."""
+    n = len(os.sched_getaffinity(0))
+    caps = {"affinity": n}
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            caps["cgroup_quota"] = float(q) / float(per)
+            n = min(n, max(1, int(float(q) / float(per))))
+    except (OSError, ValueError):
+        pass
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        caps["OMP_NUM_THREADS"] = int(env)
+        n = min(n, int(env))
+    return n, caps
 
-    nthreads = min(16, os.cpu_count() or 1)
-    tn = args.cpu_tn or args.tn
-    g = orc.Geometry(lab, res, [0.0, D0], periodic=args.periodic)
-    job = dict(geom=g, nwalk=args.cpu_walkers, seed=0x5EED0001, grad=grad)
+
+def cpu_baseline(args, line):
+    """The oracle (FP64 reference semantics, OpenMP over every batch of every
+    geometry), built with -march=native for this host and run in a child
+    process with OMP_PROC_BIND=close and OMP_PLACES=cores on every CPU this
+    process may use, on bounded samples of every workload (oracle/cpu_baseline.py)."""
+    nth, caps = _cpu_threads()
+    native = os.path.join(ROOT, "oracle", "_native", "liboracle_rms.so")
+    r = subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"), "native"], capture_output=True,
+                       text=True, timeout=300)
+    lib, flags = (native, "-O3 -march=native -fopenmp -ffp-contract=off") if r.returncode == 0 else \
+        (os.path.join(ROOT, "oracle", "liboracle_rms.so"), "-O3 -march=x86-64-v3 -fopenmp -ffp-contract=off "
+         "(native build failed)")
+    env = dict(os.environ, OMP_PROC_BIND="close", OMP_PLACES="cores")
     t0 = time.perf_counter()
-    orc.run([job], dt, tn, min(1000, tn), prec="f64", replicas=32, nthreads=nthreads)
-    el = time.perf_counter() - t0
-    return {"value": args.cpu_walkers * tn / el, "unit": "walker-steps/s", "cores": nthreads, "kind": "port",
-            "sample": f"{args.cpu_walkers} walkers x {tn} steps of the same axon/PGSE workload, FP64, "
-                      f"{el:.1f} s"}
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), "--threads", str(nth),
+                        "--lib", lib, "--c2-periodic", args.periodic, "--configs", args.cpu_configs],
+                       capture_output=True, text=True, timeout=900, env=env)
+    if p.returncode != 0:
+        return {"value": None, "unit": "walker-steps/s", "cores": nth, "kind": "port",
+                "sample": "CPU baseline failed: " + p.stderr[-400:]}
+    d = json.loads(p.stdout.strip().splitlines()[-1])
+    pc = d["per_config"]
+    gpu = {"c1": line["forms"].get("planes"), "c2": line["forms"]["plain"], "c3": line["forms"].get("batched"),
+           "c4": line["forms"].get("gradient"), "c5": line["forms"].get("sweep")}
+    ratios = {k: (gpu[k]["walker_steps_per_s"] / v["value"]) for k, v in pc.items() if gpu.get(k)}
+    c2 = pc.get("c2")
+    return {
+        "value": None if c2 is None else c2["value"], "unit": "walker-steps/s", "cores": nth, "kind": "port",
+        "sample": None if c2 is None else c2["sample"] + f", FP64, {c2['seconds']:.1f} s",
+        "cpu_model": d["cpu_model"], "threads_limits": caps, "build_flags": flags,
+        "omp": {"OMP_PROC_BIND": d["omp_proc_bind"], "OMP_PLACES": d["omp_places"]},
+        "per_config": pc, "gpu_over_cpu": ratios, "target_gpu_over_cpu_c2": 100,
+        "seconds_total": time.perf_counter() - t0,
+    }
 
 
 if __name__ == "__main__":

@@ -699,215 +699,275 @@ static int cmp_triple(const void *a, const void *b)
     return ia < ib ? -1 : (ia > ib);
 }
 
+/* Per-job state of a run: derived parameters, tables and partial sums. */
+typedef struct {
+    or_derived d;
+    or_tderiv td;
+    or_jobaux aux;
+    double *TD;
+    int64_t nbatch, b0; /* batches of the job and its first work item */
+    double *part, *psig, *pgrad;
+} or_jobrun;
+
+static void or_jobrun_free(or_jobrun *jr)
+{
+    free(jr->aux.gorder);
+    free(jr->aux.gk);
+    free(jr->aux.qscale);
+    free(jr->TD);
+    free(jr->part);
+    free(jr->psig);
+    free(jr->pgrad);
+}
+
+/* Tables of one job: q scales, PGSE boundaries and lobe-triple order. */
+static int or_job_setup(const or_job *J, const or_params *p, const or_adj *adj, int jb, or_jobrun *jr)
+{
+    if (!or_derive(J->geom, p->dt, &jr->d)) return 0;
+    if (!validate_job(J, p, &jr->d)) return 0;
+    or_jobaux *aux = &jr->aux;
+    if (adj) {
+        or_derive_tangent(J->geom, p->dt, &jr->d, &jr->td);
+        aux->td = &jr->td;
+        aux->cot_rec = adj[jb].cot_rec;
+        aux->cot_sig = adj[jb].cot_sig;
+    }
+    aux->nval = or_nval(J->geom->nclass, J->nbvec);
+    aux->Tstep = p->TN / p->timepoints;
+    {
+        /* sum(g) between flushes stays below 2^31: the largest k <= 16 with
+         * 2^k max(nx, ny, nz) < 2^31 */
+        int64_t m = J->geom->nx;
+        if (J->geom->ny > m) m = J->geom->ny;
+        if (J->geom->nz > m) m = J->geom->nz;
+        if (m < 1) m = 1;
+        int k = 16;
+        while (k > 0 && (m << k) > 2147483647LL) k--;
+        aux->pgmask = ((int64_t)1 << k) - 1;
+    }
+    int T = p->timepoints;
+    jr->TD = (double *)malloc(sizeof(double) * T);
+    or_diff_times(p->dt, p->TN, T, jr->TD);
+    int nbq = J->nbvec > 0 ? J->nbvec : 1;
+    aux->qscale = (double *)calloc((size_t)T * nbq, sizeof(double));
+    for (int r = 0; r < T; r++)
+        for (int j = 0; j < J->nbvec; j++) aux->qscale[(size_t)r * nbq + j] = sqrt(J->bval[j] / jr->TD[r]);
+    /* PGSE boundaries */
+    if (J->ngrad > 0) {
+        int64_t *abe = (int64_t *)malloc(sizeof(int64_t) * 3 * J->ngrad);
+        or_pgse_lobes(p->dt, p->TN, J->ngrad, J->grad, abe);
+        int64_t *tmp = (int64_t *)malloc(sizeof(int64_t) * (3 * (size_t)J->ngrad + 1));
+        int n = 0;
+        tmp[n++] = p->TN;
+        for (int j = 0; j < J->ngrad; j++)
+            for (int q = 0; q < 3; q++) tmp[n++] = abe[3 * j + q];
+        /* sort unique, drop 0 */
+        for (int a = 1; a < n; a++) {
+            int64_t v = tmp[a];
+            int b = a - 1;
+            while (b >= 0 && tmp[b] > v) {
+                tmp[b + 1] = tmp[b];
+                b--;
+            }
+            tmp[b + 1] = v;
+        }
+        aux->bounds[0] = 0;
+        int nb = 0;
+        for (int a = 0; a < n; a++) {
+            if (tmp[a] == 0) continue;
+            if (nb > 0 && aux->bounds[nb] == tmp[a]) continue;
+            if (nb >= OR_MAX_BOUNDS) {
+                free(abe);
+                free(tmp);
+                return fail("too many distinct PGSE lobe boundaries");
+            }
+            aux->bounds[++nb] = tmp[a];
+        }
+        free(tmp);
+        aux->nbound = nb;
+        aux->gk = (int32_t *)malloc(sizeof(int32_t) * 3 * J->ngrad);
+        for (int j = 0; j < J->ngrad; j++)
+            for (int q = 0; q < 3; q++) {
+                int64_t v = abe[3 * j + q];
+                int k = 0;
+                while (aux->bounds[k] != v) k++;
+                aux->gk[3 * j + q] = k;
+            }
+        free(abe);
+        aux->gorder = (int32_t *)malloc(sizeof(int32_t) * J->ngrad);
+        for (int j = 0; j < J->ngrad; j++) aux->gorder[j] = j;
+        g_sort_aux = aux;
+        qsort(aux->gorder, J->ngrad, sizeof(int32_t), cmp_triple);
+    }
+    jr->nbatch = (J->nwalk + OR_BATCH - 1) / OR_BATCH;
+    jr->pgrad = adj ? (double *)calloc((size_t)jr->nbatch * 16, sizeof(double)) : NULL;
+    jr->part = (double *)calloc((size_t)jr->nbatch * T * aux->nval, sizeof(double));
+    jr->psig = J->ngrad > 0 ? (double *)calloc((size_t)jr->nbatch * J->ngrad, sizeof(double)) : NULL;
+    return 1;
+}
+
+/* Replica sums (batch order) and totals (replica order) of one job. */
+static void or_job_reduce(const or_job *J, const or_params *p, const or_adj *adj, int jb, or_jobrun *jr,
+                          or_out *O, int R)
+{
+    const int T = p->timepoints;
+    const int64_t nbatch = jr->nbatch;
+    size_t rv = (size_t)T * jr->aux.nval;
+    double *rep = (double *)calloc((size_t)R * rv, sizeof(double));
+    for (int64_t b = 0; b < nbatch; b++) {
+        double *dst = rep + (size_t)(b % R) * rv;
+        const double *src = jr->part + (size_t)b * rv;
+        for (size_t q = 0; q < rv; q++) dst[q] = dst[q] + src[q];
+    }
+    for (size_t q = 0; q < rv; q++) {
+        double s = 0.0;
+        for (int r = 0; r < R; r++) s = s + rep[(size_t)r * rv + q];
+        O->rec[q] = s;
+    }
+    if (O->rec_rep) memcpy(O->rec_rep, rep, sizeof(double) * R * rv);
+    free(rep);
+    if (J->ngrad > 0 && O->sig) {
+        double *srep = (double *)calloc((size_t)R * J->ngrad, sizeof(double));
+        for (int64_t b = 0; b < nbatch; b++)
+            for (int j = 0; j < J->ngrad; j++)
+                srep[(size_t)(b % R) * J->ngrad + j] =
+                    srep[(size_t)(b % R) * J->ngrad + j] + jr->psig[(size_t)b * J->ngrad + j];
+        for (int j = 0; j < J->ngrad; j++) {
+            double s = 0.0;
+            for (int r = 0; r < R; r++) s = s + srep[(size_t)r * J->ngrad + j];
+            O->sig[j] = s;
+        }
+        if (O->sig_rep) memcpy(O->sig_rep, srep, sizeof(double) * R * J->ngrad);
+        free(srep);
+    }
+    if (adj) {
+        double *grep = (double *)calloc((size_t)R * 16, sizeof(double));
+        for (int64_t b = 0; b < nbatch; b++)
+            for (int q = 0; q < 16; q++)
+                grep[(size_t)(b % R) * 16 + q] = grep[(size_t)(b % R) * 16 + q] + jr->pgrad[(size_t)b * 16 + q];
+        for (int q = 0; q < 16; q++) {
+            double s = 0.0;
+            for (int r = 0; r < R; r++) s = s + grep[(size_t)r * 16 + q];
+            adj[jb].grad[q] = s;
+        }
+        if (adj[jb].grad_rep) memcpy(adj[jb].grad_rep, grep, sizeof(double) * R * 16);
+        /* With exchange between non-null compartments a crossing changes
+         * the class (and the step length) discontinuously in D: the
+         * pathwise D derivative misses that jump, so it is reported as NaN. */
+        if (or_has_exchange(&jr->d)) {
+            for (int c = 1; c < OR_NC_MAX; c++) {
+                adj[jb].grad[c] = NAN;
+                if (adj[jb].grad_rep)
+                    for (int r = 0; r < R; r++) adj[jb].grad_rep[(size_t)r * 16 + c] = NAN;
+            }
+        }
+        /* a pair closed by its kappa draws nothing: no kappa estimate */
+        if (or_kappa_closes_pair(J->geom, &jr->d)) {
+            adj[jb].grad[8] = NAN;
+            if (adj[jb].grad_rep)
+                for (int r = 0; r < R; r++) adj[jb].grad_rep[(size_t)r * 16 + 8] = NAN;
+        }
+        free(grep);
+    }
+}
+
+/* All jobs of a run: tables first, then every (job, batch) pair spread over
+ * the OpenMP threads (each batch is computed on its own, so the thread
+ * count and schedule do not change any bit), then the ordered reductions. */
 static int or_run_impl(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs, const or_adj *adj)
 {
     g_err[0] = 0;
     int R = p->replicas < 1 ? 1 : p->replicas;
-    for (int jb = 0; jb < njobs; jb++) {
-        const or_job *J = &jobs[jb];
-        or_out *O = &outs[jb];
-        or_derived d;
-        if (!or_derive(J->geom, p->dt, &d)) return 0;
-        if (!validate_job(J, p, &d)) return 0;
-        or_jobaux aux;
-        memset(&aux, 0, sizeof(aux));
-        or_tderiv td;
-        if (adj) {
-            or_derive_tangent(J->geom, p->dt, &d, &td);
-            aux.td = &td;
-            aux.cot_rec = adj[jb].cot_rec;
-            aux.cot_sig = adj[jb].cot_sig;
-        }
-        aux.nval = or_nval(J->geom->nclass, J->nbvec);
-        aux.Tstep = p->TN / p->timepoints;
-        {
-            /* sum(g) between flushes stays below 2^31: the largest k <= 16 with
-             * 2^k max(nx, ny, nz) < 2^31 */
-            int64_t m = J->geom->nx;
-            if (J->geom->ny > m) m = J->geom->ny;
-            if (J->geom->nz > m) m = J->geom->nz;
-            if (m < 1) m = 1;
-            int k = 16;
-            while (k > 0 && (m << k) > 2147483647LL) k--;
-            aux.pgmask = ((int64_t)1 << k) - 1;
-        }
-        int T = p->timepoints;
-        double *TD = (double *)malloc(sizeof(double) * T);
-        or_diff_times(p->dt, p->TN, T, TD);
-        int nbq = J->nbvec > 0 ? J->nbvec : 1;
-        aux.qscale = (double *)calloc((size_t)T * nbq, sizeof(double));
-        for (int r = 0; r < T; r++)
-            for (int j = 0; j < J->nbvec; j++) aux.qscale[(size_t)r * nbq + j] = sqrt(J->bval[j] / TD[r]);
-        /* PGSE boundaries */
-        if (J->ngrad > 0) {
-            int64_t *abe = (int64_t *)malloc(sizeof(int64_t) * 3 * J->ngrad);
-            or_pgse_lobes(p->dt, p->TN, J->ngrad, J->grad, abe);
-            int64_t *tmp = (int64_t *)malloc(sizeof(int64_t) * (3 * (size_t)J->ngrad + 1));
-            int n = 0;
-            tmp[n++] = p->TN;
-            for (int j = 0; j < J->ngrad; j++)
-                for (int q = 0; q < 3; q++) tmp[n++] = abe[3 * j + q];
-            /* sort unique, drop 0 */
-            for (int a = 1; a < n; a++) {
-                int64_t v = tmp[a];
-                int b = a - 1;
-                while (b >= 0 && tmp[b] > v) {
-                    tmp[b + 1] = tmp[b];
-                    b--;
-                }
-                tmp[b + 1] = v;
-            }
-            aux.bounds[0] = 0;
-            int nb = 0;
-            for (int a = 0; a < n; a++) {
-                if (tmp[a] == 0) continue;
-                if (nb > 0 && aux.bounds[nb] == tmp[a]) continue;
-                if (nb >= OR_MAX_BOUNDS) {
-                    free(abe);
-                    free(tmp);
-                    return fail("too many distinct PGSE lobe boundaries");
-                }
-                aux.bounds[++nb] = tmp[a];
-            }
-            free(tmp);
-            aux.nbound = nb;
-            aux.gk = (int32_t *)malloc(sizeof(int32_t) * 3 * J->ngrad);
-            for (int j = 0; j < J->ngrad; j++)
-                for (int q = 0; q < 3; q++) {
-                    int64_t v = abe[3 * j + q];
-                    int k = 0;
-                    while (aux.bounds[k] != v) k++;
-                    aux.gk[3 * j + q] = k;
-                }
-            free(abe);
-            aux.gorder = (int32_t *)malloc(sizeof(int32_t) * J->ngrad);
-            for (int j = 0; j < J->ngrad; j++) aux.gorder[j] = j;
-            g_sort_aux = &aux;
-            qsort(aux.gorder, J->ngrad, sizeof(int32_t), cmp_triple);
-        }
-        int64_t nbatch = (J->nwalk + OR_BATCH - 1) / OR_BATCH;
-        double *pgrad = adj ? (double *)calloc((size_t)nbatch * 16, sizeof(double)) : NULL;
-        double *part = (double *)calloc((size_t)nbatch * T * aux.nval, sizeof(double));
-        double *psig = J->ngrad > 0 ? (double *)calloc((size_t)nbatch * J->ngrad, sizeof(double)) : NULL;
-        int ok = 1;
-        uint64_t cnt_tot[5] = {0, 0, 0, 0, 0};
-        int lanew = aux.nval > J->ngrad ? aux.nval : J->ngrad;
+    or_jobrun *jr = (or_jobrun *)calloc((size_t)(njobs > 0 ? njobs : 1), sizeof(or_jobrun));
+    int ok = 1;
+    int64_t nwork = 0;
+    int maxnb = 0, maxng = 0, maxlane = 0;
+    for (int jb = 0; jb < njobs && ok; jb++) {
+        ok = or_job_setup(&jobs[jb], p, adj, jb, &jr[jb]);
+        if (!ok) break;
+        jr[jb].b0 = nwork;
+        nwork += jr[jb].nbatch;
+        if (jr[jb].aux.nbound > maxnb) maxnb = jr[jb].aux.nbound;
+        if (jobs[jb].ngrad > maxng) maxng = jobs[jb].ngrad;
+        int lanew = jr[jb].aux.nval > jobs[jb].ngrad ? jr[jb].aux.nval : jobs[jb].ngrad;
+        if (lanew > maxlane) maxlane = lanew;
+    }
+    if (!ok) {
+        for (int jb = 0; jb < njobs; jb++) or_jobrun_free(&jr[jb]);
+        free(jr);
+        return 0;
+    }
+    uint64_t *cnt_tot = (uint64_t *)calloc((size_t)(njobs > 0 ? njobs : 1) * 5, sizeof(uint64_t));
+    int run_ok = 1;
 #ifdef _OPENMP
-        int nth = p->nthreads > 0 ? p->nthreads : omp_get_max_threads();
+    int nth = p->nthreads > 0 ? p->nthreads : omp_get_max_threads();
 #pragma omp parallel num_threads(nth)
 #endif
-        {
-            uint64_t cnt_acc[5] = {0, 0, 0, 0, 0};
-            or_jobaux la = aux;
-            la.snapbuf = (double *)malloc(sizeof(double) * OR_BATCH * (aux.nbound + 1) * 3 + 8);
-            la.phasebuf = (double *)malloc(sizeof(double) * OR_BATCH * (J->ngrad + 1));
-            la.snapJ = (double *)calloc((size_t)OR_BATCH * (aux.nbound + 1) * 9 + 8, sizeof(double));
-            double *lanebuf = (double *)malloc(sizeof(double) * OR_BATCH * lanew);
+    {
+        uint64_t *cnt_acc = (uint64_t *)calloc((size_t)(njobs > 0 ? njobs : 1) * 5, sizeof(uint64_t));
+        double *snapbuf = (double *)malloc(sizeof(double) * OR_BATCH * (maxnb + 1) * 3 + 8);
+        double *phasebuf = (double *)malloc(sizeof(double) * OR_BATCH * (maxng + 1));
+        const size_t nsj = (size_t)OR_BATCH * (maxnb + 1) * 9 + 8;
+        double *snapJ = (double *)calloc(nsj, sizeof(double));
+        double *lanebuf = (double *)malloc(sizeof(double) * OR_BATCH * (maxlane > 0 ? maxlane : 1));
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 1)
 #endif
-            for (int64_t b = 0; b < nbatch; b++) {
-                int r;
-                if (la.snapJ) memset(la.snapJ, 0, sizeof(double) * ((size_t)OR_BATCH * (aux.nbound + 1) * 9 + 8));
-                double *pg = pgrad ? pgrad + (size_t)b * 16 : NULL;
-                if (p->prec == OR_PREC_F32)
-                    r = or_batch_f32(J, &d, p, &la, b, part + (size_t)b * T * aux.nval,
-                                     psig ? psig + (size_t)b * J->ngrad : NULL, lanebuf, O, cnt_acc, pg);
+        for (int64_t w = 0; w < nwork; w++) {
+            int lo = 0, hi = njobs - 1; /* the job whose batches hold work item w */
+            while (lo < hi) {
+                int mid = (lo + hi + 1) / 2;
+                if (jr[mid].b0 <= w)
+                    lo = mid;
                 else
-                    r = or_batch_f64(J, &d, p, &la, b, part + (size_t)b * T * aux.nval,
-                                     psig ? psig + (size_t)b * J->ngrad : NULL, lanebuf, O, cnt_acc, pg);
-                if (!r) {
+                    hi = mid - 1;
+            }
+            const int jb = lo;
+            const or_job *J = &jobs[jb];
+            or_jobrun *r = &jr[jb];
+            const int64_t b = w - r->b0;
+            or_jobaux la = r->aux;
+            la.snapbuf = snapbuf;
+            la.phasebuf = phasebuf;
+            la.snapJ = snapJ;
+            memset(snapJ, 0, sizeof(double) * nsj);
+            const int T = p->timepoints;
+            double *pg = r->pgrad ? r->pgrad + (size_t)b * 16 : NULL;
+            int rc;
+            if (p->prec == OR_PREC_F32)
+                rc = or_batch_f32(J, &r->d, p, &la, b, r->part + (size_t)b * T * la.nval,
+                                  r->psig ? r->psig + (size_t)b * J->ngrad : NULL, lanebuf, &outs[jb],
+                                  cnt_acc + 5 * jb, pg);
+            else
+                rc = or_batch_f64(J, &r->d, p, &la, b, r->part + (size_t)b * T * la.nval,
+                                  r->psig ? r->psig + (size_t)b * J->ngrad : NULL, lanebuf, &outs[jb],
+                                  cnt_acc + 5 * jb, pg);
+            if (!rc) {
 #ifdef _OPENMP
 #pragma omp atomic write
 #endif
-                    ok = 0;
-                }
+                run_ok = 0;
             }
+        }
 #ifdef _OPENMP
 #pragma omp critical
 #endif
-            for (int q = 0; q < 5; q++) cnt_tot[q] += cnt_acc[q];
-            free(la.snapbuf);
-            free(la.phasebuf);
-            free(la.snapJ);
-            free(lanebuf);
-        }
-        if (!ok) {
-            free(part);
-            free(psig);
-            free(TD);
-            free(aux.qscale);
-            free(aux.gk);
-            free(aux.gorder);
-            free(pgrad);
-            return fail("walker initialisation found no non-null voxel");
-        }
-        /* replica sums in batch order, then replicas in order */
-        size_t rv = (size_t)T * aux.nval;
-        double *rep = (double *)calloc((size_t)R * rv, sizeof(double));
-        for (int64_t b = 0; b < nbatch; b++) {
-            double *dst = rep + (size_t)(b % R) * rv;
-            const double *src = part + (size_t)b * rv;
-            for (size_t q = 0; q < rv; q++) dst[q] = dst[q] + src[q];
-        }
-        for (size_t q = 0; q < rv; q++) {
-            double s = 0.0;
-            for (int r = 0; r < R; r++) s = s + rep[(size_t)r * rv + q];
-            O->rec[q] = s;
-        }
-        if (O->rec_rep) memcpy(O->rec_rep, rep, sizeof(double) * R * rv);
-        free(rep);
-        if (J->ngrad > 0 && O->sig) {
-            double *srep = (double *)calloc((size_t)R * J->ngrad, sizeof(double));
-            for (int64_t b = 0; b < nbatch; b++)
-                for (int j = 0; j < J->ngrad; j++)
-                    srep[(size_t)(b % R) * J->ngrad + j] =
-                        srep[(size_t)(b % R) * J->ngrad + j] + psig[(size_t)b * J->ngrad + j];
-            for (int j = 0; j < J->ngrad; j++) {
-                double s = 0.0;
-                for (int r = 0; r < R; r++) s = s + srep[(size_t)r * J->ngrad + j];
-                O->sig[j] = s;
-            }
-            if (O->sig_rep) memcpy(O->sig_rep, srep, sizeof(double) * R * J->ngrad);
-            free(srep);
-        }
-        for (int q = 0; q < 5; q++) O->counters[q] = cnt_tot[q];
-        if (adj) {
-            double *grep = (double *)calloc((size_t)R * 16, sizeof(double));
-            for (int64_t b = 0; b < nbatch; b++)
-                for (int q = 0; q < 16; q++)
-                    grep[(size_t)(b % R) * 16 + q] = grep[(size_t)(b % R) * 16 + q] + pgrad[(size_t)b * 16 + q];
-            for (int q = 0; q < 16; q++) {
-                double s = 0.0;
-                for (int r = 0; r < R; r++) s = s + grep[(size_t)r * 16 + q];
-                adj[jb].grad[q] = s;
-            }
-            if (adj[jb].grad_rep) memcpy(adj[jb].grad_rep, grep, sizeof(double) * R * 16);
-            /* With exchange between non-null compartments a crossing changes
-             * the class (and the step length) discontinuously in D: the
-             * pathwise D derivative misses that jump, so it is reported as NaN. */
-            if (or_has_exchange(&d)) {
-                for (int c = 1; c < OR_NC_MAX; c++) {
-                    adj[jb].grad[c] = NAN;
-                    if (adj[jb].grad_rep)
-                        for (int r = 0; r < R; r++) adj[jb].grad_rep[(size_t)r * 16 + c] = NAN;
-                }
-            }
-            /* a pair closed by its kappa draws nothing: no kappa estimate */
-            if (or_kappa_closes_pair(J->geom, &d)) {
-                adj[jb].grad[8] = NAN;
-                if (adj[jb].grad_rep)
-                    for (int r = 0; r < R; r++) adj[jb].grad_rep[(size_t)r * 16 + 8] = NAN;
-            }
-            free(grep);
-            free(pgrad);
-        }
-        free(aux.gorder);
-        free(part);
-        free(psig);
-        free(TD);
-        free(aux.qscale);
-        free(aux.gk);
+        for (int q = 0; q < 5 * njobs; q++) cnt_tot[q] += cnt_acc[q];
+        free(cnt_acc);
+        free(snapbuf);
+        free(phasebuf);
+        free(snapJ);
+        free(lanebuf);
     }
+    if (run_ok) {
+        for (int jb = 0; jb < njobs; jb++) {
+            or_job_reduce(&jobs[jb], p, adj, jb, &jr[jb], &outs[jb], R);
+            for (int q = 0; q < 5; q++) outs[jb].counters[q] = cnt_tot[5 * jb + q];
+        }
+    }
+    for (int jb = 0; jb < njobs; jb++) or_jobrun_free(&jr[jb]);
+    free(jr);
+    free(cnt_tot);
+    if (!run_ok) return fail("walker initialisation found no non-null voxel");
     return 1;
 }

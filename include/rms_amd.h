@@ -133,12 +133,24 @@ typedef struct {
 /* Adjoint outputs: derivatives of sum(cot_rec * rec) + sum(cot_sig * sig)
  * with respect to D[c] (index c), T2[c] (4 + c) and kappa (8).
  * Estimators: T2 analytic (weights only); kappa score function of the
- * exchange draws; D pathwise (faces fixed, step ~ sqrt(D)).  The pathwise D
- * derivative is exact only while no walker can change class: for a geometry
- * with exchange between non-null classes the D entries are NaN.  The kappa
- * entry is NaN when a pair of non-null classes is impermeable (P < 1e-6)
- * only because of its kappa: no draw happens there, so the score function
- * has nothing to sum although P grows with kappa. */
+ * exchange draws against a leave-one-out baseline (the mean of the other
+ * walkers of the 64-walker batch); D pathwise (faces fixed, step ~ sqrt(D))
+ * plus the score of the exchange draws plus the boundary terms of the three
+ * places where a step's end jumps as D moves (smoothed perturbation
+ * analysis): reaching an exchange face (class change), reaching a
+ * lattice-edge mirror (origin mirrored), and passing a voxel edge in one face
+ * order or the other (where the two orders end in another class or with
+ * reflections off other axes).  Each boundary event is the derivative of its
+ * probability within the step times the difference of the walks continued
+ * on either side of it; one event per walker is kept by weighted reservoir
+ * sampling.  The continuations carry no PGSE sums: in a plan with a PGSE
+ * table the D entries are NaN for geometries that have exchange, a non-null
+ * voxel on a non-periodic lattice face, or a voxel edge where the face
+ * order matters (exactly two of the two face neighbours and their diagonal
+ * neighbour in the voxel's class); elsewhere the pathwise term is the whole
+ * derivative.  The kappa entry is NaN when a pair of non-null classes is
+ * impermeable (P < 1e-6) only because of its kappa: no draw happens there,
+ * so the score function has nothing to sum although P grows with kappa. */
 typedef struct {
     const double *cot_rec;       /* [timepoints * nval] or NULL */
     const double *cot_sig;       /* [ngrad] or NULL */

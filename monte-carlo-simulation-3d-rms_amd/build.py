@@ -42,17 +42,82 @@ def _run(cmd):
     return out
 
 
+# The tangent variants with the boundary terms of the D derivative hold three
+# walkers' worth of state and spill heavily.  The greedy VGPR allocator of
+# this toolchain (ROCm 7.2, clang 22) has placed a spill store of a value live
+# in every lane at the start of a reconvergence block, before the exec-mask
+# restore, so that lanes outside the branch never stored it and reloaded
+# garbage (a sin/cos polynomial coefficient, in walk_kernel<float, 21, 3>).
+# The basic allocator spills at definitions and reloads at uses only, which
+# is correct under any exec mask; that translation unit is built with it, and
+# the device assembly of every unit is scanned for the signature.
+JUMP_FLAGS = ["-mllvm", "-vgpr-regalloc=basic"]
+UNITS = (("rms_kernels.hip", []), ("rms_kernels_jump.hip", JUMP_FLAGS), ("rms_api.cpp", []), ("rms_host.cpp", []))
+
+
+def spill_before_exec_restore(asm):
+    """Blocks whose leading instructions store a VGPR spill before restoring
+    exec (the spill then misses the lanes that rejoin there).  Returns
+    (function, label) pairs."""
+    hits = []
+    func = None
+    lines = asm.split("\n")
+    for k, line in enumerate(lines):
+        if line.startswith("_ZN") and line.split(":")[0].isidentifier():
+            func = line.split(":")[0]
+        if not (line.startswith(".LBB") or line.startswith("; %bb")):
+            continue
+        spills = 0
+        for t in lines[k + 1:k + 200]:
+            t = t.strip()
+            if not t or t.startswith(";"):
+                continue
+            if "Folded Spill" in t:
+                spills += 1
+                continue
+            if ("Folded Reload" in t or t.startswith(("s_waitcnt", "s_nop", "v_readlane", "v_writelane"))
+                    or (t.startswith("s_") and "exec" not in t and not t.startswith(("s_cbranch", "s_branch")))):
+                continue
+            if t.startswith("s_or_b64 exec, exec") and spills:
+                hits.append((func, line.split(":")[0]))
+            break
+    return hits
+
+
 def build_lib(verbose=False):
     os.makedirs(LIB_DIR, exist_ok=True)
+    tmpdir = os.path.join(LIB_DIR, ".obj")
+    os.makedirs(tmpdir, exist_ok=True)
+    procs, objs, asms = [], [], []
+    for src, extra in UNITS:
+        obj = os.path.join(tmpdir, src + ".o")
+        cmd = [HIPCC, *HIP_FLAGS, *extra, "-c", "-o", obj, os.path.join(CSRC, src)]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        objs.append(obj)
+        if src.endswith(".hip"):
+            asm = os.path.join(tmpdir, src + ".s")
+            cmd = [HIPCC, *HIP_FLAGS, *extra, "--cuda-device-only", "-S", "-o", asm, os.path.join(CSRC, src)]
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+            asms.append(asm)
+    out = []
+    for cmd, pr in procs:
+        txt = pr.communicate()[0]
+        txt = "\n".join(l for l in txt.splitlines() if "packed-fp32-ops" not in l and l.strip())
+        if pr.returncode != 0:
+            raise RuntimeError("build failed: " + " ".join(cmd) + "\n" + txt)
+        if txt:
+            out.append(txt)
+    for asm in asms:
+        hits = spill_before_exec_restore(open(asm).read())
+        if hits:
+            raise RuntimeError(f"{os.path.basename(asm)}: VGPR spill stored before an exec restore in "
+                               + ", ".join(f"{f} {b}" for f, b in hits[:8]))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *HIP_FLAGS, "-shared", "-o", tmp,
-           os.path.join(CSRC, "rms_kernels.hip"),
-           os.path.join(CSRC, "rms_api.cpp"),
-           os.path.join(CSRC, "rms_host.cpp")]
-    out = _run(cmd)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    shutil.rmtree(tmpdir, ignore_errors=True)
     if verbose and out:
-        print(out)
+        print("\n".join(out))
     return LIB
 
 

@@ -27,6 +27,7 @@ struct GeomDev {
     uint8_t *d_cls = nullptr;     // class per voxel
     int64_t nvox = 0;
     bool has_walkable = false;
+    uint32_t flags = 0;           // geometry_flags: boundary terms of the D derivative
 };
 
 struct rms_ctx {
@@ -86,7 +87,7 @@ struct rms_plan {
     int maxbound = 0;
     // adjoint pass buffers, allocated by the first rms_plan_adjoint call
     bool adj_ready = false;
-    int adj_np = 1, adj_grid = 0;
+    int adj_np = 1, adj_grid = 0, adj_sig = 0;
     JobDesc *d_jobs_adj = nullptr;
     ReduceDesc *d_red_adj = nullptr;
     std::vector<ReduceDesc> red_adj;
@@ -261,6 +262,7 @@ static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int3
     gd.meta.label_class = nullptr;
     gd.nvox = nvox;
     gd.has_walkable = walk;
+    gd.flags = geometry_flags(g);
     // the labels and class map travel once; the kernels read only what is
     // built from them: a class per voxel (uint8) and a cell word (uint64)
     const size_t bytes = (size_t)nvox * g.label_bytes;
@@ -843,9 +845,10 @@ static rms_status adjoint_setup(rms_plan *p)
     }
     if ((st = palloc(p, &p->d_jobs_adj, sizeof(JobDesc) * nj)) != RMS_OK) return st;
     if ((st = pupload(p, &p->d_red_adj, p->red_adj.data(), p->red_adj.size())) != RMS_OK) return st;
+    p->adj_sig = adjoint_sig(p->sig);
     int bpc = 0;
     const bool f64 = p->prm.precision == RMS_PREC_F64;
-    if (adjoint_occupancy(f64, p->sig, p->adj_np, p->block, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+    if (adjoint_occupancy(f64, p->adj_sig, p->adj_np, p->block, &bpc) != hipSuccess || bpc < 1) bpc = 1;
     const int wpb = p->block / WAVE;
     const int64_t need = (p->nbatch + wpb - 1) / wpb;
     int64_t grid = p->prm.grid_blocks > 0 ? p->prm.grid_blocks : std::min(need, (int64_t)bpc * ctx->ncu);
@@ -895,7 +898,7 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, rms_tim
     a.jobs = p->d_jobs_adj;
     a.st.g = nullptr;  // single launch: no walker state is saved
     a.fin = FinalOut{};
-    HIPCHK(ctx, launch_adjoint(a, p->prm.precision == RMS_PREC_F64, p->sig, p->adj_np, p->adj_grid, p->block, s));
+    HIPCHK(ctx, launch_adjoint(a, p->prm.precision == RMS_PREC_F64, p->adj_sig, p->adj_np, p->adj_grid, p->block, s));
     HIPCHK(ctx, hipEventRecord(p->ev[1], s));
     HIPCHK(ctx, launch_reduce(p->d_red_adj, (int32_t)p->red_adj.size(), NPARAM, p->prm.replicas, s));
     HIPCHK(ctx, hipEventRecord(p->ev[2], s));
@@ -928,9 +931,11 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, rms_tim
             if (adj[j].grad_rep)
                 for (int r = 0; r < R; r++) adj[j].grad_rep[(size_t)r * NPARAM + c] = NAN;
         };
-        if (p->jobs[j].exchange) {
-            // a crossing between non-null classes changes the class (and the
-            // step length) discontinuously in D: no pathwise D derivative
+        if (!(p->adj_sig & SIG_JUMP) &&
+            (p->jobs[j].exchange || p->ctx->geoms[p->jobs[j].geom_id].flags != 0)) {
+            // exchange faces, lattice-edge mirrors and voxel edges make the
+            // walk's end discontinuous in D; their boundary terms are there
+            // for plans without PGSE tables only
             for (int c = 1; c < NC; c++) nan_entry(c);
         }
         if (p->jobs[j].kappa_closed) {

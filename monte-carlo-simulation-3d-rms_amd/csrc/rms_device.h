@@ -45,7 +45,16 @@ __device__ __forceinline__ U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint
 }
 
 // Streams of the counter layout.
-enum : uint32_t { STREAM_DIR = 0, STREAM_EXCH_A = 1, STREAM_EXCH_B = 2, STREAM_INIT_XY = 3, STREAM_INIT_Z = 4 };
+// Streams 5..7: reservoir decisions of the adjoint's exchange-face events.
+enum : uint32_t {
+    STREAM_DIR = 0,
+    STREAM_EXCH_A = 1,
+    STREAM_EXCH_B = 2,
+    STREAM_INIT_XY = 3,
+    STREAM_INIT_Z = 4,
+    STREAM_JUMP = 5,     // 5..7: exchange-face and lattice-edge event reservoir
+    STREAM_CORNER = 8    // 8..13: voxel-edge event reservoir
+};
 
 __device__ __forceinline__ U4 rng_block(uint32_t key0, uint32_t key1, uint32_t idx, uint32_t stream, uint64_t wid)
 {

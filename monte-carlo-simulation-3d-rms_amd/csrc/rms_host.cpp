@@ -111,6 +111,49 @@ bool has_exchange(const rms_geometry &g, const Derived &d)
     return false;
 }
 
+uint32_t geometry_flags(const rms_geometry &g)
+{
+    const int32_t N[3] = {g.nx, g.ny, g.nz};
+    auto cls_at = [&](const int32_t q[3]) -> int32_t {
+        const int64_t k = ((int64_t)q[0] * g.ny + q[1]) * g.nz + q[2];
+        const uint32_t lab = g.label_bytes == 1 ? (uint32_t)((const uint8_t *)g.labels)[k]
+                                                : (uint32_t)((const uint16_t *)g.labels)[k];
+        return g.label_class ? (int32_t)g.label_class[lab] : (int32_t)lab;
+    };
+    // 1 when the voxel at p + d (wrapped on periodic axes) has class c
+    auto is = [&](const int32_t p[3], const int32_t d[3], int32_t c) -> int {
+        int32_t q[3];
+        for (int k = 0; k < 3; k++) {
+            q[k] = p[k] + d[k];
+            if (q[k] >= 0 && q[k] < N[k]) continue;
+            if (!((g.periodic >> k) & 1)) return 0;
+            q[k] = q[k] < 0 ? q[k] + N[k] : q[k] - N[k];
+        }
+        return cls_at(q) == c ? 1 : 0;
+    };
+    static const int PA[3] = {0, 0, 1}, PB[3] = {1, 2, 2};
+    uint32_t f = 0;
+    int32_t p[3];
+    for (p[0] = 0; p[0] < N[0]; p[0]++)
+        for (p[1] = 0; p[1] < N[1]; p[1]++)
+            for (p[2] = 0; p[2] < N[2]; p[2]++) {
+                const int32_t c0 = cls_at(p);
+                if (c0 == 0) continue;
+                for (int k = 0; k < 3; k++)
+                    if (!((g.periodic >> k) & 1) && (p[k] == 0 || p[k] == N[k] - 1)) f |= GEOM_MIRROR;
+                if (!(f & GEOM_EDGE_ORDER))
+                    for (int pr = 0; pr < 3; pr++)
+                        for (int qd = 0; qd < 4; qd++) {
+                            int32_t da[3] = {0, 0, 0}, db[3] = {0, 0, 0}, dab[3] = {0, 0, 0};
+                            da[PA[pr]] = dab[PA[pr]] = (qd & 1) ? 1 : -1;
+                            db[PB[pr]] = dab[PB[pr]] = (qd & 2) ? 1 : -1;
+                            if (is(p, da, c0) + is(p, db, c0) + is(p, dab, c0) == 2) f |= GEOM_EDGE_ORDER;
+                        }
+                if (f == (GEOM_EDGE_ORDER | GEOM_MIRROR)) return f;
+            }
+    return f;
+}
+
 bool kappa_closes_pair(const rms_geometry &g, const Derived &d)
 {
     for (int a = 1; a < g.nclass; a++)

@@ -15,7 +15,33 @@
 namespace rms {
 
 // Kernel variant bits (the SIG template argument of walk_kernel).
-enum : int { SIG_NARROW = 1, SIG_PGSE = 2, SIG_EXCH = 4, SIG_PERIODIC = 8 };
+enum : int { SIG_NARROW = 1, SIG_PGSE = 2, SIG_EXCH = 4, SIG_PERIODIC = 8, SIG_JUMP = 16 };
+
+// The adjoint variant of a plan: the boundary terms of the D derivative
+// (exchange faces, lattice-edge mirrors, voxel edges) whenever no job has a
+// PGSE table (the continuations carry no PGSE sums).  The jump variants
+// handle exchange faces on the fast path.
+inline int adjoint_sig(int sig)
+{
+    return (sig & SIG_PGSE) == 0 ? (sig | SIG_JUMP | SIG_EXCH) : sig;
+}
+
+// Geometry properties that add boundary terms to the D derivative (bits of
+// geometry_flags):
+//   GEOM_EDGE_ORDER  some non-null voxel has, for a pair of axes (a, b) and a
+//                    quadrant (sa, sb), exactly two of its face neighbours
+//                    q + sa e_a, q + sb e_b and its diagonal neighbour
+//                    q + sa e_a + sb e_b in its own class: a step through
+//                    that voxel edge then ends in a different voxel
+//                    depending on which face the march takes first, as on
+//                    every staircase of a voxelised curved surface;
+//   GEOM_MIRROR      some non-null voxel touches a lattice face of an axis
+//                    that is not periodic: reaching it mirrors the origin.
+// Neighbours across a periodic axis wrap; outside the lattice otherwise is
+// not the voxel's class.  Without either property (and without exchange)
+// the pathwise D derivative is the whole derivative.
+enum : uint32_t { GEOM_EDGE_ORDER = 1u, GEOM_MIRROR = 2u };
+uint32_t geometry_flags(const rms_geometry &g);
 
 // Steps must stay below one voxel by this margin: the straight move ends at
 // e = fma(s, v, o) with o in [0, 1], and e must round below 2 in FP32 and FP64.

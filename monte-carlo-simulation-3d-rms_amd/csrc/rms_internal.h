@@ -19,6 +19,9 @@ hipError_t launch_build_cells(const void *labels, int32_t label_bytes, const uin
 // with the record/signal cotangents per walker.
 hipError_t launch_adjoint(const LaunchArgs &a, int prec_f64, int sig, int np, int grid, int block, hipStream_t s);
 hipError_t adjoint_occupancy(int prec_f64, int sig, int np, int block, int *blocks_per_cu);
+// The walk kernel variant with the boundary terms (sig bit 4) for np tangent
+// parameters (rms_kernels_jump.hip).
+const void *jump_walk_fn(int prec_f64, int sig, int np);
 
 // Allocation-free runs: device address of this GPU's scratch array (the
 // current device's copy), and the two table kernels that fill it.

@@ -9,8 +9,9 @@ directory: a geometry, its walkers, its b-table or PGSE table).
       PGSE 30 directions x b in {0.5,1,2,3} x Delta in {10,20,40} at delta = 5
   c3  256 axons with extra-axonal shells in uneven uint16 lattices, kappa in
       [0, 0.02], T2 80/60, D 2.0/1.5, 1e7 walkers, 5e4 steps, the PGSE table
-  c4  the c2 axon with extra-axonal space, kappa = 0.01, 1e6 walkers, 5e4 steps,
-      20 records, narrow-pulse 64 directions x b in {0.5,1,2,3}
+  c4  the c2 axon (periodic lattice) with extra-axonal space, kappa = 0.01,
+      1e6 walkers, 5e4 steps, 20 records, narrow-pulse 64 directions x b in
+      {0.5,1,2,3}
   c5  4096 axons with extra-axonal space, 70x70x1000 uint16, caliber CV,
       undulation and kappa drawn per geometry, 4e7 walkers, 2e4 steps,
       20 records, narrow-pulse 16 directions x b in {1,2}
@@ -121,9 +122,11 @@ def c4_labels():
     return G.axon(n=64, nz=1000, res=RES, r0=0.8, cv=0.2, undulation=0.5, seed=1235, ecs=True)
 
 
-def c4_geometry(Geometry, D=(0.0, 2.0, 1.5), T2=(math.inf, 80.0, 60.0), kappa=C4["kappa"], labels=None):
+def c4_geometry(Geometry, D=(0.0, 2.0, 1.5), T2=(math.inf, 80.0, 60.0), kappa=C4["kappa"], labels=None,
+                periodic="xyz"):
+    """The c2 lattice (periodic edges) with label 2 outside the axon."""
     lab = c4_labels() if labels is None else labels
-    return Geometry(lab, RES, list(D), T2=list(T2), kappa=kappa)
+    return Geometry(lab, RES, list(D), T2=list(T2), kappa=kappa, periodic=periodic)
 
 
 def c4_jobs(Geometry, nwalk=C4["nwalk"], labels=None, **phys):

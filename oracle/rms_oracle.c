@@ -418,6 +418,53 @@ static int or_kappa_closes_pair(const or_geom *g, const or_derived *d)
     return 0;
 }
 
+/* Geometry properties that add boundary terms to the D derivative: bit 0
+ * when some non-null voxel has, for a pair of axes (a, b) and a quadrant
+ * (sa, sb), exactly two of q + sa e_a, q + sb e_b, q + sa e_a + sb e_b in its
+ * own class (a step through that voxel edge ends in a different voxel
+ * depending on which face the march takes first, main.cu:199-219); bit 1
+ * when some non-null voxel touches a lattice face of a non-periodic axis
+ * (reaching it mirrors the origin, main.cu:302-310).  Neighbours across a
+ * periodic axis wrap. */
+static int or_is_class(const or_geom *G, const int32_t p[3], int da, int a, int db, int b, int32_t c)
+{
+    const int32_t N[3] = {G->nx, G->ny, G->nz};
+    int32_t q[3] = {p[0], p[1], p[2]};
+    q[a] += da;
+    q[b] += db;
+    for (int k = 0; k < 3; k++) {
+        if (q[k] >= 0 && q[k] < N[k]) continue;
+        if (!((G->periodic >> k) & 1)) return 0;
+        q[k] = q[k] < 0 ? q[k] + N[k] : q[k] - N[k];
+    }
+    return or_class_at(G, q[0], q[1], q[2]) == c;
+}
+
+static int or_geom_flags(const or_geom *G)
+{
+    const int32_t N[3] = {G->nx, G->ny, G->nz};
+    int f = 0;
+    int32_t p[3];
+    for (p[0] = 0; p[0] < N[0]; p[0]++)
+        for (p[1] = 0; p[1] < N[1]; p[1]++)
+            for (p[2] = 0; p[2] < N[2]; p[2]++) {
+                const int32_t c0 = or_class_at(G, p[0], p[1], p[2]);
+                if (c0 == 0) continue;
+                for (int k = 0; k < 3; k++)
+                    if (!((G->periodic >> k) & 1) && (p[k] == 0 || p[k] == N[k] - 1)) f |= 2;
+                for (int a = 0; a < 3; a++)
+                    for (int b = a + 1; b < 3; b++)
+                        for (int sa = -1; sa <= 1; sa += 2)
+                            for (int sb = -1; sb <= 1; sb += 2) {
+                                const int n = or_is_class(G, p, sa, a, 0, b, c0) +
+                                              or_is_class(G, p, 0, a, sb, b, c0) +
+                                              or_is_class(G, p, sa, a, sb, b, c0);
+                                if (n == 2) f |= 1;
+                            }
+            }
+    return f;
+}
+
 /* 1 when walkers can move between two different non-null classes. */
 static int or_has_exchange(const or_derived *d)
 {
@@ -529,6 +576,7 @@ typedef struct {
     const double *cot_rec; /* adjoint: [timepoints][nval] or NULL */
     const double *cot_sig; /* adjoint: [ngrad] or NULL */
     const or_tderiv *td;   /* adjoint constants or NULL */
+    int jump;              /* adjoint: D derivative across exchange faces (or_jump_step) */
     double *snapJ;    /* per thread [OR_BATCH][nb+1][3][3] tangent prefix sums */
 } or_jobaux;
 
@@ -721,7 +769,8 @@ static void or_jobrun_free(or_jobrun *jr)
 }
 
 /* Tables of one job: q scales, PGSE boundaries and lobe-triple order. */
-static int or_job_setup(const or_job *J, const or_params *p, const or_adj *adj, int jb, or_jobrun *jr)
+static int or_job_setup(const or_job *J, const or_params *p, const or_adj *adj, int jb, int plan_pgse,
+                        or_jobrun *jr)
 {
     if (!or_derive(J->geom, p->dt, &jr->d)) return 0;
     if (!validate_job(J, p, &jr->d)) return 0;
@@ -731,6 +780,10 @@ static int or_job_setup(const or_job *J, const or_params *p, const or_adj *adj, 
         aux->td = &jr->td;
         aux->cot_rec = adj[jb].cot_rec;
         aux->cot_sig = adj[jb].cot_sig;
+        /* boundary terms of the D derivative (exchange faces, lattice-edge
+         * mirrors, voxel edges) when no job of the run has a PGSE table (the
+         * continuations carry no PGSE prefix sums) */
+        aux->jump = !plan_pgse;
     }
     aux->nval = or_nval(J->geom->nclass, J->nbvec);
     aux->Tstep = p->TN / p->timepoints;
@@ -851,10 +904,11 @@ static void or_job_reduce(const or_job *J, const or_params *p, const or_adj *adj
             adj[jb].grad[q] = s;
         }
         if (adj[jb].grad_rep) memcpy(adj[jb].grad_rep, grep, sizeof(double) * R * 16);
-        /* With exchange between non-null compartments a crossing changes
-         * the class (and the step length) discontinuously in D: the
-         * pathwise D derivative misses that jump, so it is reported as NaN. */
-        if (or_has_exchange(&jr->d)) {
+        /* Exchange faces, lattice-edge mirrors and voxel edges make the
+         * walk's end discontinuous in D; their boundary terms (or_jump_step,
+         * or_corner_step) are there for runs without PGSE tables: with one
+         * the D entries are NaN when the geometry has any of them. */
+        if (!jr->aux.jump && (or_has_exchange(&jr->d) || or_geom_flags(J->geom) != 0)) {
             for (int c = 1; c < OR_NC_MAX; c++) {
                 adj[jb].grad[c] = NAN;
                 if (adj[jb].grad_rep)
@@ -882,8 +936,10 @@ static int or_run_impl(const or_job *jobs, int32_t njobs, const or_params *p, or
     int ok = 1;
     int64_t nwork = 0;
     int maxnb = 0, maxng = 0, maxlane = 0;
+    int plan_pgse = 0;
+    for (int jb = 0; jb < njobs; jb++) plan_pgse = plan_pgse || jobs[jb].ngrad > 0;
     for (int jb = 0; jb < njobs && ok; jb++) {
-        ok = or_job_setup(&jobs[jb], p, adj, jb, &jr[jb]);
+        ok = or_job_setup(&jobs[jb], p, adj, jb, plan_pgse, &jr[jb]);
         if (!ok) break;
         jr[jb].b0 = nwork;
         nwork += jr[jb].nbatch;

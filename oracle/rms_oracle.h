@@ -129,8 +129,10 @@ int or_run(const or_job *jobs, int32_t njobs, const or_params *p, or_out *outs);
 
 /* Adjoint pass: derivatives of sum(cot_rec*rec) + sum(cot_sig*sig) with
  * respect to D[c] (grad[c], c = 1..3), T2[c] (grad[4+c]) and kappa (grad[8]).
- * D entries are NaN when walkers can cross between non-null classes (the
- * pathwise derivative misses the jump of a class change); the kappa entry is
+ * D entries: pathwise, plus the score of the exchange draws and the jump of
+ * the class change at exchange faces when walkers can cross between
+ * non-null classes; NaN in that case when a job of the run has a PGSE table
+ * (the continuations of the jump term carry no PGSE sums).  The kappa entry is
  * NaN when a pair of non-null classes is impermeable only because of its
  * kappa (no draw happens there, so the score function sees nothing). */
 typedef struct {

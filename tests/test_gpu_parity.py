@@ -229,8 +229,10 @@ def _adj_jobs(rms, orc, jobs):
 def test_adjoint_bitexact(rms, orc, engine, prec, kappa, periodic):
     """Tangent pass (pathwise D, score-function kappa, analytic T2) against
     the oracle's adjoint on the same seeds: gradients and their replica
-    partials are bit-identical.  kappa = 0: impermeable compartments, the D
-    entries are defined; kappa > 0: exchange, D entries NaN on both sides."""
+    partials are bit-identical.  The plan has a PGSE table and the voxelised
+    blobs have voxel-edge order effects (and mirror edges unless periodic), so
+    the D entries are NaN on both sides; kappa = 0 closes the membranes (no
+    draw, kappa NaN)."""
     print(f"case adjoint {prec} kappa={kappa} periodic={periodic}", flush=True)
     lab = random_blobs((10, 9, 11), 3, seed=3)
     geom = dict(labels=lab, res=0.1, D=[0.0, 2.0, 0.6], T2=[50.0, 80.0, 20.0], kappa=kappa, periodic=periodic)
@@ -259,11 +261,39 @@ def test_adjoint_bitexact(rms, orc, engine, prec, kappa, periodic):
     np.testing.assert_array_equal(out2[0]["grad"], ref2[0]["grad"])
     g = out[0]["grad"]
     assert np.all(np.isfinite(g[4:8])) and g[5] != 0.0 and g[6] != 0.0
+    assert np.all(np.isnan(g[1:4]))
     if kappa > 0:
-        assert np.all(np.isnan(g[1:4])) and np.isfinite(g[8]) and g[8] != 0.0
+        assert np.isfinite(g[8]) and g[8] != 0.0
     else:
-        # kappa = 0 closes the membrane: no draw, no kappa estimate (NaN)
-        assert np.all(np.isfinite(g[1:4])) and g[1] != 0.0 and g[2] != 0.0 and np.isnan(g[8])
+        assert np.isnan(g[8])
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_adjoint_pgse_pathwise_bitexact(rms, orc, engine, prec):
+    """PGSE tangent pass where the pathwise D derivative is the whole
+    derivative (periodic planes: no mirror, no voxel-edge order effect): D,
+    T2 and their replica partials bit-identical to the oracle, D finite."""
+    print(f"case adjoint PGSE pathwise {prec}", flush=True)
+    lab = planes(7, 6, 12)
+    geom = dict(labels=lab, res=0.1, D=[0.0, 1.8], T2=[50.0, 60.0], periodic="xyz")
+    bval, bvec = np.array([1.5]), fib_dirs(1)
+    grad = pgse_table(fib_dirs(3), [0.5, 1.5], [0.03], 0.01)
+    jobs = [dict(geom=geom, nwalk=600, seed=21, bval=bval, bvec=bvec, grad=grad)]
+    dt, TN, T = 2e-4, 300, 30
+    rng = np.random.default_rng(4)
+    cot_rec = [rng.standard_normal((T, rms.nval(2, 1)))]
+    cot_sig = [rng.standard_normal(grad.shape[0])]
+    gj, oj = _adj_jobs(rms, orc, jobs)
+    plan = engine.plan(gj, dt, TN, T, precision=prec, replicas=8)
+    try:
+        out = plan.adjoint(cot_rec, cot_sig, want_replicas=True)
+    finally:
+        plan.close()
+    ref = orc.run(oj, dt, TN, T, prec=prec, replicas=8, cot_rec=cot_rec, cot_sig=cot_sig)
+    np.testing.assert_array_equal(out[0]["grad"], ref[0]["grad"])
+    np.testing.assert_array_equal(out[0]["grad_rep"], ref[0]["grad_rep"])
+    g = out[0]["grad"]
+    assert np.isfinite(g[1]) and g[1] != 0.0 and np.isfinite(g[5]) and g[5] != 0.0
 
 
 def test_adjoint_batched_jobs_bitexact(rms, orc, engine):

@@ -374,9 +374,17 @@ def form_gradient(rms, eng, args, A):
         adj_ms.append(plan.adj_timing["walk_ms"])
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    # the same cotangent through the boundary-term estimator of D (pathwise
+    # plus the jumps, reservoir-sampled), once, for its values and cost
+    cot = W.c4_cotangent(A, out["rec"], 3, TD, nb)
+    t1 = time.perf_counter()
+    bnd = plan.adjoint([cot], None, want_replicas=True, estimator="boundary")[0]
+    bnd_s = time.perf_counter() - t1
+    bnd_ms = plan.adj_timing["walk_ms"]
     plan.close()
     gr = adj["grad"]
     se = np.sqrt(32) * np.std(adj["grad_rep"], axis=0, ddof=1)
+    se_b = np.sqrt(32) * np.std(bnd["grad_rep"], axis=0, ddof=1)
     names = {"D_intra": 1, "D_extra": 2, "T2_intra": 5, "T2_extra": 6, "kappa": 8}
     fin = lambda x: None if not np.isfinite(x) else float(x)  # noqa: E731
 
@@ -389,17 +397,24 @@ def form_gradient(rms, eng, args, A):
     est, ses = _jackknife(fS, out["rec_rep"])
     rate = float(Wn) * TN * nrun / el
     s = np.array([math.sqrt(6 * dt * 2.0) / W.RES, math.sqrt(6 * dt * 1.5) / W.RES])
-    ops_adj = 2 * ops_per_step(float(s.mean()), pgse=False) + 20 * 2
+    # per walker-step of the problem: the forward walk, the tangent pass (the
+    # walk plus ~40 lane-ops of tangents) and four forward walks at D (1 +- h)
+    ops_all = 6 * ops_per_step(float(s.mean()), pgse=False) + 40
     kern = float(Wn) * TN / ((np.mean(fwd_ms) + np.mean(adj_ms)) / 1e3)
     rel = float(ses[0] / est[0])
     return {
         "walker_steps_per_s": rate, "kernel_walker_steps_per_s": kern,
         "forward_kernel_ms": float(np.mean(fwd_ms)), "adjoint_kernel_ms": float(np.mean(adj_ms)),
-        "roofline_frac_estimate": kern * ops_adj / 1e12 / VALU_PEAK_TOPS,
+        "adjoint": "tangent pass (T2 analytic, kappa score, D pathwise) + central differences of D with common "
+                   "random numbers at D (1 +- 0.05): 4 forward walks",
+        "roofline_frac_estimate": kern * ops_all / 1e12 / VALU_PEAK_TOPS,
         "geometries": 1, "walkers_per_geometry": Wn, "TN": TN, "records": T, "bvecs": nb, "kappa": c["kappa"],
         "objective": "sum_t [D_t(z) + K_t(z) + sum_j S_j(t)], narrow-pulse S = sigRe/sig0",
         "gradient": {k: fin(gr[v]) for k, v in names.items()},
         "gradient_se": {k: fin(se[v]) for k, v in names.items()},
+        "boundary_estimator": {"D_intra": fin(bnd["grad"][1]), "D_intra_se": fin(se_b[1]),
+                               "D_extra": fin(bnd["grad"][2]), "D_extra_se": fin(se_b[2]),
+                               "adjoint_kernel_ms": float(bnd_ms), "seconds": bnd_s},
         "D_end": float(est[0]), "D_end_se": float(ses[0]), "K_end": float(est[1]), "K_end_se": float(ses[1]),
         "S_end_b1_first": float(est[2]), "S_end_b1_first_se": float(ses[2]),
         "S_end_b2_first": float(est[3]), "S_end_b2_first_se": float(ses[3]),
@@ -438,6 +453,7 @@ def form_sweep(rms, eng, args, A):
     plan.close()
     dk = np.array([a["grad"][8] for a in adj])
     dT2 = np.array([a["grad"][5] for a in adj])
+    dD = np.array([a["grad"][1] for a in adj])
     stats = [_stat_summary(A, o, W.DT, TN, T, 3, 0) for o in outs]
     Dend = np.array([s["D_end"] for s in stats])
     rel = np.array([s["D_end_se"] / s["D_end"] for s in stats])
@@ -451,6 +467,8 @@ def form_sweep(rms, eng, args, A):
         "lattice_bytes_in_hbm": ng * 70 * 70 * 1000 * 9, "setup_s": setup_s,
         "D_end_median": float(np.median(Dend)), "D_end_range": [float(np.min(Dend)), float(np.max(Dend))],
         "dobj_dkappa_median": float(np.nanmedian(dk)), "dobj_dT2_intra_median": float(np.median(dT2)),
+        "dobj_dD_intra_median": float(np.nanmedian(dD)),
+        "adjoint": "tangent pass + central differences of D with common random numbers (4 forward walks)",
         "D_end_rel_se_median": float(np.median(rel)),
         "seconds_per_geometry_at_1pct_se": _se_cost(rate, nwg, TN, float(np.median(rel))),
     }

@@ -132,25 +132,48 @@ typedef struct {
 
 /* Adjoint outputs: derivatives of sum(cot_rec * rec) + sum(cot_sig * sig)
  * with respect to D[c] (index c), T2[c] (4 + c) and kappa (8).
- * Estimators: T2 analytic (weights only); kappa score function of the
- * exchange draws against a leave-one-out baseline (the mean of the other
- * walkers of the 64-walker batch); D pathwise (faces fixed, step ~ sqrt(D))
- * plus the score of the exchange draws plus the boundary terms of the three
- * places where a step's end jumps as D moves (smoothed perturbation
- * analysis): reaching an exchange face (class change), reaching a
- * lattice-edge mirror (origin mirrored), and passing a voxel edge in one face
- * order or the other (where the two orders end in another class or with
- * reflections off other axes).  Each boundary event is the derivative of its
- * probability within the step times the difference of the walks continued
- * on either side of it; one event per walker is kept by weighted reservoir
- * sampling.  The continuations carry no PGSE sums: in a plan with a PGSE
- * table the D entries are NaN for geometries that have exchange, a non-null
- * voxel on a non-periodic lattice face, or a voxel edge where the face
- * order matters (exactly two of the two face neighbours and their diagonal
- * neighbour in the voxel's class); elsewhere the pathwise term is the whole
- * derivative.  The kappa entry is NaN when a pair of non-null classes is
- * impermeable (P < 1e-6) only because of its kappa: no draw happens there,
- * so the score function has nothing to sum although P grows with kappa. */
+ * T2: analytic (the weights only; paths do not depend on T2).
+ * kappa: score function of the exchange draws against a leave-one-out
+ * baseline (the mean of the other walkers of the 64-walker batch).  NaN when
+ * a pair of non-null classes is impermeable (P < 1e-6) only because of its
+ * kappa: no draw happens there, so the score function has nothing to sum
+ * although P grows with kappa.
+ * D: chosen by rms_adjoint_opts.d_estimator (below).  The pathwise
+ * derivative (faces fixed, step ~ sqrt(D), reflections folded, plus the score
+ * of the exchange draws) is the whole derivative only while a step's end
+ * moves continuously with D.  It jumps at three places: reaching an exchange
+ * face (class change), reaching a lattice-edge mirror (origin mirrored), and
+ * passing a voxel edge where the face order decides the outcome (exactly two
+ * of the two face neighbours and their diagonal neighbour in the voxel's
+ * class, as on every staircase of a voxelised curved surface).  A job whose
+ * geometry has none of these is "continuous in D". */
+enum {
+    /* pathwise for jobs continuous in D, central differences with common
+     * random numbers for the others (the default of rms_plan_adjoint) */
+    RMS_D_AUTO = 0,
+    /* central differences with common random numbers for every job: two
+     * forward walks per non-null class at D_c (1 +- h), same seeds, the
+     * objective contracted per replica; O(h^2) bias, any table */
+    RMS_D_CRN_FD = 1,
+    /* pathwise plus the boundary terms of the jumps (smoothed perturbation
+     * analysis): each boundary event is the derivative of its probability
+     * within the step times the difference of the walks continued on either
+     * side; one event per walker is kept by weighted reservoir sampling.
+     * Unbiased, but each walker carries two continuations and every step
+     * searches the events, and the variance is large.  The continuations
+     * carry no PGSE sums: in a plan with a PGSE table the D entries of jobs
+     * not continuous in D are NaN. */
+    RMS_D_BOUNDARY = 2,
+    /* pathwise only: NaN for jobs not continuous in D (no extra passes) */
+    RMS_D_PATHWISE = 3
+};
+
+typedef struct {
+    int32_t d_estimator;         /* RMS_D_* */
+    int32_t reserved;
+    double fd_rel_step;          /* h of RMS_D_AUTO / RMS_D_CRN_FD, 0 = 0.05 */
+} rms_adjoint_opts;
+
 typedef struct {
     const double *cot_rec;       /* [timepoints * nval] or NULL */
     const double *cot_sig;       /* [ngrad] or NULL */
@@ -186,9 +209,15 @@ rms_status rms_plan_destroy(rms_plan *plan);
 rms_status rms_run(rms_ctx *ctx, const rms_job *jobs, int32_t njobs, const rms_run_params *params,
                    rms_job_out *outs, rms_timing *timing);
 
-/* Pathwise/score-function derivatives (no reference counterpart: the
- * reference has no gradients).  Recomputes every trajectory from its seed. */
+/* Derivatives of the record and signal sums (no reference counterpart: the
+ * reference has no gradients).  Recomputes every trajectory from its seed;
+ * D by RMS_D_AUTO. */
 rms_status rms_plan_adjoint(rms_plan *plan, const rms_job_adj *adj, rms_timing *timing);
+/* The same with the D estimator chosen by opts (NULL = defaults).  timing
+ * covers the tangent pass; with central differences walk_ms adds the forward
+ * walks at the perturbed D and launches counts them. */
+rms_status rms_plan_adjoint_ex(rms_plan *plan, const rms_job_adj *adj, const rms_adjoint_opts *opts,
+                               rms_timing *timing);
 
 /* ---- allocation-free run ----
  * No counterpart in the reference.  For callers that cannot allocate device

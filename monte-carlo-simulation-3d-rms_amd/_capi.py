@@ -24,8 +24,10 @@ STATUS = {
 EXPORTS = (
     "rms_create", "rms_destroy", "rms_last_error", "rms_version", "rms_add_geometry", "rms_derive",
     "rms_nval", "rms_plan_create", "rms_plan_execute", "rms_plan_destroy", "rms_run", "rms_plan_adjoint",
-    "rms_run_noalloc",
+    "rms_run_noalloc", "rms_plan_adjoint_ex",
 )
+# D estimators of the adjoint (rms_adjoint_opts.d_estimator)
+D_ESTIMATOR = {"auto": 0, "fd": 1, "boundary": 2, "pathwise": 3}
 SCRATCH_BYTES = 256 << 20  # RMS_SCRATCH_BYTES
 
 
@@ -122,6 +124,14 @@ class JobAdj(C.Structure):
     ]
 
 
+class AdjointOpts(C.Structure):
+    _fields_ = [
+        ("d_estimator", C.c_int32),
+        ("reserved", C.c_int32),
+        ("fd_rel_step", C.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -151,6 +161,7 @@ def lib():
     L.rms_run.argtypes = [C.c_void_p, C.POINTER(Job), C.c_int32, C.POINTER(RunParams), C.POINTER(JobOut),
                           C.POINTER(Timing)]
     L.rms_plan_adjoint.argtypes = [C.c_void_p, C.POINTER(JobAdj), C.POINTER(Timing)]
+    L.rms_plan_adjoint_ex.argtypes = [C.c_void_p, C.POINTER(JobAdj), C.POINTER(AdjointOpts), C.POINTER(Timing)]
     L.rms_run_noalloc.argtypes = [C.c_void_p, C.POINTER(Job), C.POINTER(RunParams), C.POINTER(JobOut),
                                   C.POINTER(Timing)]
     for name in EXPORTS:

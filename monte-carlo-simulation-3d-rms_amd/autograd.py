@@ -10,15 +10,23 @@ counter-based random stream) with forward-mode tangents, contracted with the
 incoming cotangents on the device.
 
 Estimators used by the adjoint pass:
-  D      pathwise derivative of the displacement (faces fixed, step length
-         d s_c / d D_c = s_c / (2 D_c)); NaN when walkers can cross between
-         two non-null classes (a class change at a crossing is a jump the
-         pathwise derivative does not contain);
+  D      chosen by d_estimator (rms_adjoint_opts): "auto" (the default) is
+         the pathwise derivative of the displacement (faces fixed, step
+         length d s_c / d D_c = s_c / (2 D_c), reflections folded, plus the
+         score of the exchange draws) where the walk's end moves continuously
+         with D, and central differences with common random numbers (the
+         same seeds at D_c (1 +- h)) where it jumps: exchange faces,
+         lattice-edge mirrors, voxel edges whose face order decides the
+         outcome.  "boundary" adds the boundary terms of those jumps to the
+         pathwise derivative instead (unbiased, slow, high variance; NaN with
+         a PGSE table), "fd" differences every job, "pathwise" returns NaN
+         where the walk jumps;
   T2     analytic: d/dT2_c exp(-sum t_c/T2_c) = (t_c / T2_c^2) exp(...);
   kappa  score function of the Bernoulli exchange draws,
-         d log P / d kappa = 1 / (kappa (1 + pi0)); NaN when a pair of
-         non-null classes is impermeable only because of kappa (kappa = 0),
-         where no draw happens.
+         d log P / d kappa = 1 / (kappa (1 + pi0)), against a leave-one-out
+         baseline (the mean of the other walkers of the 64-walker batch);
+         NaN when a pair of non-null classes is impermeable only because of
+         kappa (kappa = 0), where no draw happens.
 Class 0 (the null compartment) has no gradient.  The inputs are host tensors
 (float64); the simulation itself always runs on the engine's device.
 """
@@ -50,6 +58,8 @@ class _Simulate(torch.autograd.Function):
         ctx.plan = plan
         ctx.nc = nc
         ctx.has_kappa = kappa is not None
+        ctx.d_estimator = spec.get("d_estimator", "auto")
+        ctx.fd_step = spec.get("fd_step", 0.0)
         ctx.ngrad = 0 if out["sig"] is None else len(out["sig"])
         rec = torch.from_numpy(np.ascontiguousarray(out["rec"]))
         sig = torch.from_numpy(out["sig"]) if ctx.ngrad else torch.zeros(0, dtype=torch.float64)
@@ -61,7 +71,7 @@ class _Simulate(torch.autograd.Function):
         try:
             cr = None if g_rec is None else g_rec.detach().cpu().double().contiguous().numpy()
             cs = None if (g_sig is None or ctx.ngrad == 0) else g_sig.detach().cpu().double().contiguous().numpy()
-            g = plan.adjoint([cr], [cs])[0]["grad"]
+            g = plan.adjoint([cr], [cs], estimator=ctx.d_estimator, fd_step=ctx.fd_step)[0]["grad"]
         finally:
             plan.close()
             ctx.plan = None
@@ -73,17 +83,20 @@ class _Simulate(torch.autograd.Function):
 
 
 def simulate(engine, labels, res, D, T2, nwalk, dt, TN, timepoints=1000, kappa=None, seed=1, bval=None, bvec=None,
-             grad=None, precision="f32", replicas=32, label_class=None):
+             grad=None, precision="f32", replicas=32, label_class=None, d_estimator="auto", fd_step=0.0):
     """Differentiable simulation of one geometry.
 
     D, T2: float64 tensors of length nclass (T2 = inf for no relaxation);
     kappa: None (the reference rule P = min(1, sqrt(Dj/Di))) or a scalar
     tensor applied to every pair of non-null classes.  Returns (rec, sig):
-    rec [timepoints, nval] record sums, sig [ngrad] PGSE signal sums."""
+    rec [timepoints, nval] record sums, sig [ngrad] PGSE signal sums.
+    d_estimator / fd_step: the D estimator of the backward pass (above) and
+    the relative step of its central differences (0 = 0.05)."""
     D = torch.as_tensor(D, dtype=torch.float64)
     T2 = torch.as_tensor(T2, dtype=torch.float64)
     if kappa is not None:
         kappa = torch.as_tensor(kappa, dtype=torch.float64)
     spec = dict(engine=engine, labels=labels, res=res, nwalk=nwalk, dt=dt, TN=TN, timepoints=timepoints, seed=seed,
-                bval=bval, bvec=bvec, grad=grad, precision=precision, replicas=replicas, label_class=label_class)
+                bval=bval, bvec=bvec, grad=grad, precision=precision, replicas=replicas, label_class=label_class,
+                d_estimator=d_estimator, fd_step=fd_step)
     return _Simulate.apply(D, T2, kappa, spec)

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -87,7 +88,8 @@ struct rms_plan {
     int maxbound = 0;
     // adjoint pass buffers, allocated by the first rms_plan_adjoint call
     bool adj_ready = false;
-    int adj_np = 1, adj_grid = 0, adj_sig = 0;
+    int adj_np = 1;
+    int adj_grid[2] = {0, 0};  // workgroups of the tangent pass without / with the boundary terms
     JobDesc *d_jobs_adj = nullptr;
     ReduceDesc *d_red_adj = nullptr;
     std::vector<ReduceDesc> red_adj;
@@ -845,31 +847,147 @@ static rms_status adjoint_setup(rms_plan *p)
     }
     if ((st = palloc(p, &p->d_jobs_adj, sizeof(JobDesc) * nj)) != RMS_OK) return st;
     if ((st = pupload(p, &p->d_red_adj, p->red_adj.data(), p->red_adj.size())) != RMS_OK) return st;
-    p->adj_sig = adjoint_sig(p->sig);
-    int bpc = 0;
-    const bool f64 = p->prm.precision == RMS_PREC_F64;
-    if (adjoint_occupancy(f64, p->adj_sig, p->adj_np, p->block, &bpc) != hipSuccess || bpc < 1) bpc = 1;
-    const int wpb = p->block / WAVE;
-    const int64_t need = (p->nbatch + wpb - 1) / wpb;
-    int64_t grid = p->prm.grid_blocks > 0 ? p->prm.grid_blocks : std::min(need, (int64_t)bpc * ctx->ncu);
-    if (grid < 1) grid = 1;
-    if (p->xcd_split) grid = std::max<int64_t>(8, (grid / 8) * 8);
-    p->adj_grid = (int)grid;
     p->adj_ready = true;
     return RMS_OK;
 }
 
-static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, rms_timing *timing)
+// Workgroups of the tangent variant adj_sig: the resident limit (or the
+// plan's grid_blocks), a multiple of 8 when batches are split per XCD.
+static int adjoint_grid(rms_plan *p, int adj_sig)
+{
+    int &g = p->adj_grid[(adj_sig & SIG_JUMP) ? 1 : 0];
+    if (g > 0) return g;
+    int bpc = 0;
+    const bool f64 = p->prm.precision == RMS_PREC_F64;
+    if (adjoint_occupancy(f64, adj_sig, p->adj_np, p->block, &bpc) != hipSuccess || bpc < 1) bpc = 1;
+    const int wpb = p->block / WAVE;
+    const int64_t need = (p->nbatch + wpb - 1) / wpb;
+    int64_t grid = p->prm.grid_blocks > 0 ? p->prm.grid_blocks : std::min(need, (int64_t)bpc * p->ctx->ncu);
+    if (grid < 1) grid = 1;
+    if (p->xcd_split) grid = std::max<int64_t>(8, (grid / 8) * 8);
+    g = (int)grid;
+    return g;
+}
+
+// Objective of one job's forward results: sum(cot_rec * rec) + sum(cot_sig * sig)
+// over a record block rec [T * nval] and a signal block sig [ngrad].
+static double contract(const rms_job_adj &A, const double *rec, int64_t nq, const double *sig, int32_t ngrad)
+{
+    double f = 0.0;
+    if (A.cot_rec)
+        for (int64_t q = 0; q < nq; q++) f = f + A.cot_rec[q] * rec[q];
+    if (A.cot_sig && ngrad > 0)
+        for (int32_t g = 0; g < ngrad; g++) f = f + A.cot_sig[g] * sig[g];
+    return f;
+}
+
+// D entries by central differences with common random numbers: for each
+// non-null class c, the forward walk of the whole plan with D_c of the jobs
+// in `which` set to D_c (1 + h) and D_c (1 - h) (the other jobs and every
+// seed unchanged), each job's objective contracted per replica, and
+// dF/dD_c = (F+ - F-) / (D+ - D-).  A perturbation that leaves the plan's
+// kernel variant (a new exchange pair) or the step limit gives NaN.  The
+// plan's own descriptors are restored on the device before returning.
+static rms_status fd_d_entries(rms_plan *p, const rms_job_adj *adj, const std::vector<char> &which, double h,
+                               double *walk_ms, int *launches)
+{
+    rms_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const int nj = (int)p->jobs.size();
+    const int R = p->prm.replicas;
+    const int64_t T = p->prm.timepoints;
+    int maxc = 0;
+    for (int j = 0; j < nj; j++)
+        if (which[j]) maxc = std::max(maxc, (int)p->hjobs[j].nclass - 1);
+    const int64_t TN = p->prm.TN;
+    const int64_t ch = (p->prm.chunk_steps > 0 && p->prm.chunk_steps < TN) ? p->prm.chunk_steps : TN;
+    std::vector<double> Fp((size_t)nj * (1 + R)), Fm((size_t)nj * (1 + R));
+    std::vector<double> Dp(nj), Dm(nj);
+    rms_status st = RMS_OK;
+    for (int c = 1; c <= maxc && st == RMS_OK; c++) {
+        std::vector<char> bad(nj, 0);
+        for (int side = 0; side < 2 && st == RMS_OK; side++) {
+            std::vector<JobDesc> hj = p->hjobs;
+            for (int j = 0; j < nj; j++) {
+                if (!which[j] || p->hjobs[j].nclass <= c) continue;
+                rms_geometry g = ctx->geoms[p->jobs[j].geom_id].meta;
+                const double D0 = g.D[c];
+                g.D[c] = side == 0 ? D0 * (1.0 + h) : D0 * (1.0 - h);
+                (side == 0 ? Dp : Dm)[j] = g.D[c];
+                int bits = 0;
+                bool exch = false;
+                JobDesc D = p->hjobs[j];
+                const char *e = job_physics(g, p->prm.dt, D, bits, exch);
+                if (e || (bits & ~p->sig & (SIG_EXCH | SIG_PERIODIC)) != 0) {
+                    bad[j] = 1;
+                    continue;
+                }
+                hj[j] = D;
+            }
+            if (hipMemcpyAsync(p->d_jobs, hj.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s) != hipSuccess) {
+                st = fail(ctx, RMS_ERR_HIP, "adjoint: descriptor upload failed");
+                break;
+            }
+            if ((st = enqueue(p, true)) != RMS_OK) break;
+            if (hipMemcpyAsync(p->h_res, p->d_res, (size_t)p->res_all, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                st = fail(ctx, RMS_ERR_HIP, "adjoint: forward walk at a perturbed D failed");
+                break;
+            }
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, p->ev[0], p->ev[1]) == hipSuccess) *walk_ms += ms;
+            *launches += (int)((TN + ch - 1) / ch);
+            std::vector<double> &F = side == 0 ? Fp : Fm;
+            for (int j = 0; j < nj; j++) {
+                if (!which[j] || p->hjobs[j].nclass <= c) continue;
+                const JobHost &H = p->jobs[j];
+                const int64_t nq = T * H.nval;
+                F[(size_t)j * (1 + R)] = contract(adj[j], p->h_res + H.rec_tot, nq, p->h_res + H.sig_tot, H.ngrad);
+                for (int r = 0; r < R; r++)
+                    F[(size_t)j * (1 + R) + 1 + r] = contract(adj[j], p->h_res + H.rec_rep + (int64_t)r * nq, nq,
+                                                              p->h_res + H.sig_rep + (int64_t)r * H.ngrad, H.ngrad);
+            }
+        }
+        if (st != RMS_OK) break;
+        for (int j = 0; j < nj; j++) {
+            if (!which[j] || p->hjobs[j].nclass <= c) continue;
+            const double den = Dp[j] - Dm[j];
+            const double *fp = &Fp[(size_t)j * (1 + R)], *fm = &Fm[(size_t)j * (1 + R)];
+            adj[j].grad[c] = bad[j] ? NAN : (fp[0] - fm[0]) / den;
+            if (adj[j].grad_rep)
+                for (int r = 0; r < R; r++)
+                    adj[j].grad_rep[(size_t)r * NPARAM + c] = bad[j] ? NAN : (fp[1 + r] - fm[1 + r]) / den;
+        }
+    }
+    // the plan's own descriptors, whatever happened above
+    if (hipMemcpyAsync(p->d_jobs, p->hjobs.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        if (st == RMS_OK) st = fail(ctx, RMS_ERR_HIP, "adjoint: descriptor restore failed");
+    }
+    return st;
+}
+
+static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, const rms_adjoint_opts *opts,
+                                    rms_timing *timing)
 {
     rms_ctx *ctx = p->ctx;
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const int nj = (int)p->jobs.size();
     for (int j = 0; j < nj; j++)
         if (!adj[j].grad) return fail(ctx, RMS_ERR_INVALID_ARG, "adjoint: grad output is required for every job");
+    const int mode = opts ? opts->d_estimator : RMS_D_AUTO;
+    if (mode < RMS_D_AUTO || mode > RMS_D_PATHWISE)
+        return fail(ctx, RMS_ERR_INVALID_ARG, "adjoint: d_estimator must be one of RMS_D_AUTO .. RMS_D_PATHWISE");
+    const double h = (opts && opts->fd_rel_step != 0.0) ? opts->fd_rel_step : 0.05;
+    if (!(h > 0.0 && h < 0.5)) return fail(ctx, RMS_ERR_INVALID_ARG, "adjoint: fd_rel_step must be in (0, 0.5)");
     if (!p->adj_ready) {
         rms_status st = adjoint_setup(p);
         if (st != RMS_OK) return st;
     }
+    // the tangent variant: boundary terms only when asked for (and only
+    // without a PGSE table, where the continuations would need PGSE sums)
+    const int adj_sig = mode == RMS_D_BOUNDARY ? adjoint_sig(p->sig) : p->sig;
+    const int grid = adjoint_grid(p, adj_sig);
     hipStream_t s = ctx->stream;
     const int64_t T = p->prm.timepoints;
     std::vector<JobDesc> hj = p->hjobs;
@@ -898,7 +1016,7 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, rms_tim
     a.jobs = p->d_jobs_adj;
     a.st.g = nullptr;  // single launch: no walker state is saved
     a.fin = FinalOut{};
-    HIPCHK(ctx, launch_adjoint(a, p->prm.precision == RMS_PREC_F64, p->adj_sig, p->adj_np, p->adj_grid, p->block, s));
+    HIPCHK(ctx, launch_adjoint(a, p->prm.precision == RMS_PREC_F64, adj_sig, p->adj_np, grid, p->block, s));
     HIPCHK(ctx, hipEventRecord(p->ev[1], s));
     HIPCHK(ctx, launch_reduce(p->d_red_adj, (int32_t)p->red_adj.size(), NPARAM, p->prm.replicas, s));
     HIPCHK(ctx, hipEventRecord(p->ev[2], s));
@@ -908,19 +1026,19 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, rms_tim
     const size_t nbytes = sizeof(double) * (size_t)nj * NPARAM * (want_rep ? 1 + R : 1);
     HIPCHK(ctx, hipMemcpyAsync(p->h_adj_res, p->d_adj_res, nbytes, hipMemcpyDeviceToHost, s));
     HIPCHK(ctx, hipStreamSynchronize(s));
-    if (timing) {
-        std::memset(timing, 0, sizeof(*timing));
+    double walk_ms = 0.0, reduce_ms = 0.0, total_ms = 0.0;
+    {
         float ms = 0.f;
         HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev[0], p->ev[2]));
-        timing->total_ms = ms;
+        total_ms = ms;
         HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev[0], p->ev[1]));
-        timing->walk_ms = ms;
+        walk_ms = ms;
         HIPCHK(ctx, hipEventElapsedTime(&ms, p->ev[1], p->ev[2]));
-        timing->reduce_ms = ms;
-        timing->grid = p->adj_grid;
-        timing->block = p->block;
-        timing->launches = 1;
+        reduce_ms = ms;
     }
+    int launches = 1;
+    std::vector<char> fd(nj, 0);
+    bool any_fd = false;
     for (int j = 0; j < nj; j++) {
         std::memcpy(adj[j].grad, p->h_adj_res + (size_t)j * NPARAM, sizeof(double) * NPARAM);
         if (adj[j].grad_rep)
@@ -931,11 +1049,13 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, rms_tim
             if (adj[j].grad_rep)
                 for (int r = 0; r < R; r++) adj[j].grad_rep[(size_t)r * NPARAM + c] = NAN;
         };
-        if (!(p->adj_sig & SIG_JUMP) &&
-            (p->jobs[j].exchange || p->ctx->geoms[p->jobs[j].geom_id].flags != 0)) {
-            // exchange faces, lattice-edge mirrors and voxel edges make the
-            // walk's end discontinuous in D; their boundary terms are there
-            // for plans without PGSE tables only
+        // exchange faces, lattice-edge mirrors and voxel edges make the
+        // walk's end discontinuous in D
+        const bool jumps = p->jobs[j].exchange || p->ctx->geoms[p->jobs[j].geom_id].flags != 0;
+        if (mode == RMS_D_CRN_FD || (mode == RMS_D_AUTO && jumps)) {
+            fd[j] = 1;
+            any_fd = true;
+        } else if (jumps && !(adj_sig & SIG_JUMP)) {
             for (int c = 1; c < NC; c++) nan_entry(c);
         }
         if (p->jobs[j].kappa_closed) {
@@ -944,13 +1064,37 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, rms_tim
             nan_entry(8);
         }
     }
+    if (any_fd) {
+        const auto t0 = std::chrono::steady_clock::now();
+        double fd_ms = 0.0;
+        rms_status st = fd_d_entries(p, adj, fd, h, &fd_ms, &launches);
+        if (st != RMS_OK) return st;
+        walk_ms += fd_ms;
+        total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (timing) {
+        std::memset(timing, 0, sizeof(*timing));
+        timing->total_ms = total_ms;
+        timing->walk_ms = walk_ms;
+        timing->reduce_ms = reduce_ms;
+        timing->grid = grid;
+        timing->block = p->block;
+        timing->launches = launches;
+    }
     return RMS_OK;
 }
 
 extern "C" rms_status rms_plan_adjoint(rms_plan *p, const rms_job_adj *adj, rms_timing *timing)
 {
     if (!p || !adj) return RMS_ERR_INVALID_ARG;
-    return guard(p->ctx, [&] { return plan_adjoint_impl(p, adj, timing); });
+    return guard(p->ctx, [&] { return plan_adjoint_impl(p, adj, nullptr, timing); });
+}
+
+extern "C" rms_status rms_plan_adjoint_ex(rms_plan *p, const rms_job_adj *adj, const rms_adjoint_opts *opts,
+                                          rms_timing *timing)
+{
+    if (!p || !adj) return RMS_ERR_INVALID_ARG;
+    return guard(p->ctx, [&] { return plan_adjoint_impl(p, adj, opts, timing); });
 }
 
 // ---------------------------------------------------------- allocation free

@@ -226,8 +226,27 @@ const char *fill_job(const rms_geometry &g, const rms_job &J, const rms_run_para
         return "walker_base must be a non-negative multiple of 64 * replicas";
     if (J.walker_base > (int64_t)1 << 62 || J.nwalk > ((int64_t)1 << 62) - J.walker_base)
         return "walker ids must stay below 2^62";
+    D.nx = g.nx;
+    D.ny = g.ny;
+    D.nz = g.nz;
+    D.pg_log2 = pgse_flush_log2(g.nx, g.ny, g.nz);
+    D.periodic = g.periodic;
+    D.nbvec = J.nbvec;
+    D.ngrad = J.ngrad;
+    D.nwalk = J.nwalk;
+    D.wid0 = J.walker_base;
+    D.key0 = (uint32_t)J.seed;
+    D.key1 = (uint32_t)(J.seed >> 32);
+    D.res = g.res;
+    return job_physics(g, P.dt, D, sig_bits, exchange);
+}
+
+const char *job_physics(const rms_geometry &g, double dt, JobDesc &D, int &sig_bits, bool &exchange)
+{
+    sig_bits = 0;
+    exchange = false;
     Derived d;
-    const char *e = derive(g, P.dt, d);
+    const char *e = derive(g, dt, d);
     if (e) return e;
     for (int c = 1; c < g.nclass; c++) {
         if (!(g.D[c] > 0.0)) return "non-null classes need a positive diffusivity";
@@ -236,19 +255,7 @@ const char *fill_job(const rms_geometry &g, const rms_job &J, const rms_run_para
                    "less than one)";
         if (!(g.T2[c] > 0.0)) return "T2 must be positive";
     }
-    D.nx = g.nx;
-    D.ny = g.ny;
-    D.nz = g.nz;
-    D.pg_log2 = pgse_flush_log2(g.nx, g.ny, g.nz);
-    D.periodic = g.periodic;
     D.nclass = g.nclass;
-    D.nbvec = J.nbvec;
-    D.ngrad = J.ngrad;
-    D.nwalk = J.nwalk;
-    D.wid0 = J.walker_base;
-    D.key0 = (uint32_t)J.seed;
-    D.key1 = (uint32_t)(J.seed >> 32);
-    D.res = g.res;
     uint32_t pcat = 0;
     for (int i = 0; i < NC * NC; i++) pcat |= (uint32_t)(d.Pcat[i] & 3) << (2 * i);
     D.pcat = pcat;
@@ -265,7 +272,7 @@ const char *fill_job(const rms_geometry &g, const rms_job &J, const rms_run_para
     }
     D.xh12 = exchange_threshold(g.nclass, d.Pcat[1 * NC + 2], D.P_f[1 * NC + 2]);
     D.xh21 = exchange_threshold(g.nclass, d.Pcat[2 * NC + 1], D.P_f[2 * NC + 1]);
-    derive_tangent(g, P.dt, d, D);
+    derive_tangent(g, dt, d, D);
     exchange = has_exchange(g, d);
     // the walk kernel with faces between the two non-null classes on the fast
     // path (reflect at P = 0, draw at 0 < P < 1); it is also the only variant

@@ -84,6 +84,13 @@ const char *check_params(const rms_run_params &P);
 const char *fill_job(const rms_geometry &g, const rms_job &J, const rms_run_params &P, JobDesc &D, int &sig_bits,
                      bool &exchange);
 
+// The physics fields of a descriptor (nclass, steps, exchange
+// probabilities and their categories and FP32 thresholds, tangent
+// constants) from the geometry's per-class D, T2 and kappa at time step dt,
+// with the checks of fill_job on them; sig_bits and exchange as there.  The
+// central differences of the D derivative rebuild them at a perturbed D.
+const char *job_physics(const rms_geometry &g, double dt, JobDesc &D, int &sig_bits, bool &exchange);
+
 // PGSE lobe tables (window-sum form), allocation free.
 //   grad [ngrad*6] rows (Delta, delta, |g|, gx, gy, gz)
 //   perm [ngrad]: perm[k] = original index of the k-th sorted gradient

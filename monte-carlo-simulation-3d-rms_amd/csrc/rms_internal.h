@@ -22,6 +22,10 @@ hipError_t adjoint_occupancy(int prec_f64, int sig, int np, int block, int *bloc
 // The walk kernel variant with the boundary terms (sig bit 4) for np tangent
 // parameters (rms_kernels_jump.hip).
 const void *jump_walk_fn(int prec_f64, int sig, int np);
+// The tangent variant without a PGSE table and without the boundary terms
+// (sig bits 1 and 4 clear) for np tangent parameters
+// (rms_kernels_tangent.hip).
+const void *tangent_walk_fn(int prec_f64, int sig, int np);
 
 // Allocation-free runs: device address of this GPU's scratch array (the
 // current device's copy), and the two table kernels that fill it.

@@ -115,8 +115,9 @@ static const void *walk_fn_sig(int sig, std::integer_sequence<int, S...>)
     return fns[sig & 15];
 }
 
-// Tangent passes without the boundary terms run only for plans with a PGSE
-// table (bit 1): those variants alone are built.
+// Tangent passes with a PGSE table (bit 1); those without one are in
+// rms_kernels_tangent.hip, those with the boundary terms in
+// rms_kernels_jump.hip.
 template <typename R, int NP>
 static const void *tangent_fn(int sig)
 {
@@ -143,6 +144,7 @@ static const void *walk_fn(int prec_f64, int sig)
 static const void *walk_fn_np(int prec_f64, int sig, int np)
 {
     if ((sig & 16) && np > 0) return jump_walk_fn(prec_f64, sig, np);
+    if (!(sig & 2) && np > 0) return tangent_walk_fn(prec_f64, sig, np);
     switch (np) {
     case 0: return walk_fn<0>(prec_f64, sig);
     case 1: return walk_fn<1>(prec_f64, sig);

@@ -6,7 +6,7 @@ central differences of the forward pass with common random numbers."""
 import numpy as np
 import pytest
 
-from helpers import fib_dirs, random_blobs
+from helpers import fib_dirs, pgse_table, planes, random_blobs
 
 pytestmark = pytest.mark.gpu
 
@@ -33,7 +33,7 @@ def test_adjoint_with_exchange_bitexact(rms, orc, engine, prec, nclass, periodic
     plan = engine.plan([dict(geom=g, **job)], dt, TN, T, precision=prec, replicas=8)
     try:
         fwd = plan.execute()
-        adj = plan.adjoint(cot, None, want_replicas=True)
+        adj = plan.adjoint(cot, None, want_replicas=True, estimator="boundary")
     finally:
         plan.close()
     ref = orc.run([dict(geom=og, **job)], dt, TN, T, prec=prec, replicas=8, cot_rec=cot)
@@ -45,13 +45,109 @@ def test_adjoint_with_exchange_bitexact(rms, orc, engine, prec, nclass, periodic
     assert (fwd[0]["counters"][4] > 0) == (nclass > 2)  # exchange draws happened
 
 
+def _contract(cot_rec, rec, cot_sig, sig):
+    """sum(cot_rec * rec) + sum(cot_sig * sig), summed in row-major order."""
+    f = 0.0
+    for c, r in zip(np.ravel(cot_rec).tolist(), np.ravel(rec).tolist()):
+        f = f + c * r
+    if cot_sig is not None:
+        for c, r in zip(np.ravel(cot_sig).tolist(), np.ravel(sig).tolist()):
+            f = f + c * r
+    return f
+
+
+@pytest.mark.parametrize("prec,tables,periodic", [("f32", "narrow", ""), ("f64", "pgse", "xz"),
+                                                  ("f32", "both", "xyz")])
+def test_auto_adjoint_central_differences_match_oracle(rms, orc, engine, prec, tables, periodic):
+    """The default D estimator on a geometry whose walk jumps in D (exchange
+    between two non-null classes, voxel edges): T2 and kappa entries
+    bit-identical to the oracle's adjoint; each D entry, total and per
+    replica, equal to the central difference of the oracle's forward runs at
+    D_c (1 +- h) with the same seeds, contracted in the same order."""
+    print(f"case auto adjoint {prec} {tables} periodic={periodic}", flush=True)
+    lab = random_blobs((10, 11, 9), 3, seed=41)
+    D, T2, kappa, h = [0.0, 2.0, 0.8], [np.inf, 70.0, 40.0], 0.2, 0.03
+    job = dict(nwalk=900, seed=5)
+    if tables in ("narrow", "both"):
+        job.update(bval=np.array([0.7, 2.5]), bvec=fib_dirs(2))
+    if tables in ("pgse", "both"):
+        job.update(grad=pgse_table(fib_dirs(2), [1.0], [0.03], 0.01))
+    dt, TN, T, R = 2e-4, 400, 20, 4
+    nb = len(job["bval"]) if "bval" in job else 0
+    ng = job["grad"].shape[0] if "grad" in job else 0
+    rng = np.random.default_rng(9)
+    cot_rec = [rng.standard_normal((T, rms.nval(3, nb)))]
+    cot_sig = [rng.standard_normal(ng)] if ng else None
+    plan = engine.plan([dict(geom=rms.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic), **job)],
+                       dt, TN, T, precision=prec, replicas=R)
+    try:
+        adj = plan.adjoint(cot_rec, cot_sig, want_replicas=True, fd_step=h)[0]
+    finally:
+        plan.close()
+    og = orc.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic)
+    ref = orc.run([dict(geom=og, **job)], dt, TN, T, prec=prec, replicas=R, cot_rec=cot_rec, cot_sig=cot_sig)[0]
+    np.testing.assert_array_equal(adj["grad"][4:9], ref["grad"][4:9])
+    np.testing.assert_array_equal(adj["grad_rep"][:, 4:9], ref["grad_rep"][:, 4:9])
+    cs = None if cot_sig is None else cot_sig[0]
+    for c in (1, 2):
+        runs = []
+        for f in (1.0 + h, 1.0 - h):
+            Dc = list(D)
+            Dc[c] = D[c] * f
+            gc = orc.Geometry(lab, 0.1, Dc, T2=T2, kappa=kappa, periodic=periodic)
+            runs.append((Dc[c], orc.run([dict(geom=gc, **job)], dt, TN, T, prec=prec, replicas=R)[0]))
+        (dp, rp), (dm, rm) = runs
+        tot = (_contract(cot_rec[0], rp["rec"], cs, rp["sig"]) - _contract(cot_rec[0], rm["rec"], cs, rm["sig"]))
+        assert adj["grad"][c] == tot / (dp - dm), (c, adj["grad"][c], tot / (dp - dm))
+        for r in range(R):
+            sp = None if cs is None else rp["sig_rep"][r]
+            sm = None if cs is None else rm["sig_rep"][r]
+            fr = _contract(cot_rec[0], rp["rec_rep"][r], cs, sp) - _contract(cot_rec[0], rm["rec_rep"][r], cs, sm)
+            assert adj["grad_rep"][r, c] == fr / (dp - dm), (c, r)
+    assert np.all(np.isfinite(adj["grad"][1:3])) and np.all(adj["grad"][1:3] != 0.0)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_auto_adjoint_pathwise_where_continuous(rms, orc, engine, prec):
+    """Periodic impermeable planes (no exchange, no lattice-edge mirror, no
+    voxel edge whose face order matters): the default estimator is the
+    pathwise derivative, every entry bit-identical to the oracle's adjoint
+    and D finite, and asking for central differences gives another, nearby
+    value."""
+    print(f"case auto adjoint pathwise {prec}", flush=True)
+    lab = planes(7, 6, 12)
+    geom = dict(labels=lab, res=0.1, D=[0.0, 1.8], T2=[50.0, 60.0], periodic="xyz")
+    job = dict(nwalk=2000, seed=23, bval=np.array([1.5, 3.0]), bvec=fib_dirs(2))
+    dt, TN, T, R = 2e-4, 500, 25, 8
+    rng = np.random.default_rng(3)
+    cot = [rng.standard_normal((T, rms.nval(2, 2)))]
+    g = rms.Geometry(lab, 0.1, geom["D"], T2=geom["T2"], periodic="xyz")
+    plan = engine.plan([dict(geom=g, **job)], dt, TN, T, precision=prec, replicas=R)
+    try:
+        a = plan.adjoint(cot, None, want_replicas=True)[0]
+        f = plan.adjoint(cot, None, want_replicas=True, estimator="fd", fd_step=0.02)[0]
+    finally:
+        plan.close()
+    og = orc.Geometry(lab, 0.1, geom["D"], T2=geom["T2"], periodic="xyz")
+    ref = orc.run([dict(geom=og, **job)], dt, TN, T, prec=prec, replicas=R, cot_rec=cot)[0]
+    np.testing.assert_array_equal(a["grad"], ref["grad"])
+    np.testing.assert_array_equal(a["grad_rep"], ref["grad_rep"])
+    assert np.isfinite(a["grad"][1]) and a["grad"][1] != 0.0
+    np.testing.assert_array_equal(f["grad"][4:9], a["grad"][4:9])
+    se = np.sqrt(R) * np.std(f["grad_rep"][:, 1] - a["grad_rep"][:, 1], ddof=1)
+    assert f["grad"][1] != a["grad"][1] and abs(f["grad"][1] - a["grad"][1]) <= 4 * se + 1e-3 * abs(a["grad"][1])
+
+
 def test_c4_gradient_against_central_differences(rms, engine):
     """C4 (the axon with extra-axonal space, kappa = 0.01, 5e4 steps, 20
     records, 256 narrow-pulse b-vectors), objective sum_t [D_t(z) + K_t(z) +
-    sum_j S_j(t)] linearised at the base point (fixed cotangent).  Each of the
-    five derivatives (D_intra, D_extra, T2_intra, T2_extra, kappa) against
-    (F(theta + h) - F(theta - h)) / 2h with the same seed, within 1 % or 3
-    combined standard errors (32 batch-means replicas on both sides)."""
+    sum_j S_j(t)] linearised at the base point (fixed cotangent).  The five
+    derivatives of the default adjoint (D_intra and D_extra by the library's
+    central differences at h = 5 %, T2 analytic, kappa by its score) against
+    this test's central differences with the same seeds (D at h = 2.5 %, T2
+    at 0.1 %, kappa at 20 %), within 1 % or 3 combined standard errors (32
+    batch-means replicas on both sides); the boundary-term estimator's D
+    entries within 3 of its (large) standard errors of the same."""
     from rms_amd import analysis as A
     W = rms.workloads
     c = W.C4
@@ -66,6 +162,8 @@ def test_c4_gradient_against_central_differences(rms, engine):
         TD = A.diff_times(W.DT, TN, T)
         cot = W.c4_cotangent(A, fwd["rec"], 3, TD, int(jobs[0]["bval"].size))
         adj = plan.adjoint([cot], None, want_replicas=True)[0]
+        print("case C4 boundary-term estimator", flush=True)
+        bnd = plan.adjoint([cot], None, want_replicas=True, estimator="boundary")[0]
     finally:
         plan.close()
 
@@ -74,8 +172,11 @@ def test_c4_gradient_against_central_differences(rms, engine):
         r = engine.run(W.c4_jobs(rms.Geometry, N, lab, **p), W.DT, TN, T, precision="f32", replicas=R)[0]
         return (cot[None] * r["rec_rep"]).reshape(R, -1).sum(1)
 
+    def se(x):
+        return np.sqrt(R) * np.std(x, ddof=1)
+
     checks = []
-    for name, idx, h in (("D_intra", 1, 0.05 * 2.0), ("D_extra", 2, 0.05 * 1.5), ("T2_intra", 5, 1e-3 * 80.0),
+    for name, idx, h in (("D_intra", 1, 0.025 * 2.0), ("D_extra", 2, 0.025 * 1.5), ("T2_intra", 5, 1e-3 * 80.0),
                          ("T2_extra", 6, 1e-3 * 60.0), ("kappa", 8, 0.2 * c["kappa"])):
         print(f"case C4 central difference {name}", flush=True)
         if name.startswith("D_"):
@@ -91,9 +192,15 @@ def test_c4_gradient_against_central_differences(rms, engine):
             fd = (F_rep(T2=tuple(hi)) - F_rep(T2=tuple(lo))) / (2 * h)
         else:
             fd = (F_rep(kappa=c["kappa"] + h) - F_rep(kappa=c["kappa"] - h)) / (2 * h)
-        g_adj, se_adj = adj["grad"][idx], np.sqrt(R) * np.std(adj["grad_rep"][:, idx], ddof=1)
-        g_fd, se_fd = fd.sum(), np.sqrt(R) * np.std(fd, ddof=1)
+        g_adj, se_adj = adj["grad"][idx], se(adj["grad_rep"][:, idx])
+        g_fd, se_fd = fd.sum(), se(fd)
         ok = abs(g_adj - g_fd) <= max(0.01 * abs(g_fd), 3 * np.hypot(se_adj, se_fd))
         print(f"{name}: adjoint {g_adj:.6g} +- {se_adj:.3g}, central difference {g_fd:.6g} +- {se_fd:.3g}", flush=True)
         checks.append((name, ok, g_adj, se_adj, g_fd, se_fd))
+        if name.startswith("D_"):
+            g_b, se_b = bnd["grad"][idx], se(bnd["grad_rep"][:, idx])
+            ok_b = np.isfinite(g_b) and abs(g_b - g_fd) <= 3 * np.hypot(se_b, se_fd)
+            print(f"{name}: boundary terms {g_b:.6g} +- {se_b:.3g}", flush=True)
+            checks.append((name + " boundary", ok_b, g_b, se_b, g_fd, se_fd))
+    np.testing.assert_array_equal(bnd["grad"][4:9], adj["grad"][4:9])
     assert all(c[1] for c in checks), checks

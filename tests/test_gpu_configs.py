@@ -155,7 +155,7 @@ def _adjoint_pair(rms, orc, engine, W, gj, oj, TN, T, nclass):
     fwd = plan.execute()
     TD = A.diff_times(W.DT, TN, T)
     cots = [W.c4_cotangent(A, f["rec"], nclass, TD, int(j["bval"].size)) for f, j in zip(fwd, gj)]
-    adj = plan.adjoint(cots, None, want_replicas=True)
+    adj = plan.adjoint(cots, None, want_replicas=True, estimator="boundary")
     plan.close()
     ref = orc.run(oj, W.DT, TN, T, prec="f32", replicas=32, cot_rec=cots)
     assert_same(fwd, ref)

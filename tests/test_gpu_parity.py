@@ -249,8 +249,8 @@ def test_adjoint_bitexact(rms, orc, engine, prec, kappa, periodic):
     plan = engine.plan(gj, dt, TN, T, precision=prec, replicas=8)
     try:
         fwd = plan.execute(final_state=False)
-        out = plan.adjoint(cot_rec, cot_sig, want_replicas=True)
-        out2 = plan.adjoint(cot_rec, None, want_replicas=True)
+        out = plan.adjoint(cot_rec, cot_sig, want_replicas=True, estimator="boundary")
+        out2 = plan.adjoint(cot_rec, None, want_replicas=True, estimator="boundary")
     finally:
         plan.close()
     ref = orc.run(oj, dt, TN, T, prec=prec, replicas=8, cot_rec=cot_rec, cot_sig=cot_sig)
@@ -286,7 +286,7 @@ def test_adjoint_pgse_pathwise_bitexact(rms, orc, engine, prec):
     gj, oj = _adj_jobs(rms, orc, jobs)
     plan = engine.plan(gj, dt, TN, T, precision=prec, replicas=8)
     try:
-        out = plan.adjoint(cot_rec, cot_sig, want_replicas=True)
+        out = plan.adjoint(cot_rec, cot_sig, want_replicas=True, estimator="boundary")
     finally:
         plan.close()
     ref = orc.run(oj, dt, TN, T, prec=prec, replicas=8, cot_rec=cot_rec, cot_sig=cot_sig)
@@ -308,7 +308,7 @@ def test_adjoint_batched_jobs_bitexact(rms, orc, engine):
     gj, oj = _adj_jobs(rms, orc, jobs)
     plan = engine.plan(gj, dt, TN, T, precision="f32", replicas=4)
     try:
-        out = plan.adjoint(cot, None, want_replicas=True)
+        out = plan.adjoint(cot, None, want_replicas=True, estimator="boundary")
     finally:
         plan.close()
     ref = orc.run(oj, dt, TN, T, prec="f32", replicas=4, cot_rec=cot)
@@ -354,7 +354,7 @@ def test_autograd_matches_adjoint(rms, orc, engine):
     T2 = torch.tensor([40.0, 70.0, 30.0], dtype=torch.float64, requires_grad=True)
     kap = torch.tensor(0.0, dtype=torch.float64, requires_grad=True)
     rec, sig = ag.simulate(engine, lab, 0.1, D, T2, nwalk=300, dt=dt, TN=TN, timepoints=T, kappa=kap, seed=4,
-                           grad=grad, precision="f32", replicas=4)
+                           grad=grad, precision="f32", replicas=4, d_estimator="boundary")
     rng = np.random.default_rng(2)
     wr = torch.from_numpy(rng.standard_normal(tuple(rec.shape)))
     ws = torch.from_numpy(rng.standard_normal(tuple(sig.shape)))

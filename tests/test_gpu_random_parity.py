@@ -76,7 +76,7 @@ def test_random_inputs_adjoint_bitexact(rms, orc, engine):
         plan = engine.plan([dict(geom=gg, **job)], 2e-4, TN, T, precision=prec, replicas=4)
         try:
             plan.execute(final_state=False)
-            a = plan.adjoint(cot_rec, cot_sig, want_replicas=True)[0]
+            a = plan.adjoint(cot_rec, cot_sig, want_replicas=True, estimator="boundary")[0]
         finally:
             plan.close()
         b = orc.run([dict(geom=og, **job)], 2e-4, TN, T, prec=prec, replicas=4, cot_rec=cot_rec, cot_sig=cot_sig)[0]

@@ -109,16 +109,36 @@ def _se_cost(rate, walkers_per_geom, TN, rel_se, target=0.01):
     return float(walkers_per_geom * (rel_se / target) ** 2 * TN / rate)
 
 
-def _timed_runs(plan, n, want_last=True):
+def _geom(rms, args):
+    """Geometry factory with the --storage choice of the cell words."""
+    def make(*a, **k):
+        return rms.Geometry(*a, storage=args.storage, **k)
+    return make
+
+
+def _hbm_bytes(eng, jobs):
+    """Device bytes of the jobs' geometries and the storages chosen."""
+    info = [eng.geometry_info(j["geom"]) for j in jobs]
+    return int(sum(b for b, _ in info)), sorted({s for _, s in info})
+
+
+def _timed_runs(plan, n, want_last=True, replicas_untimed=False):
+    """n timed runs; the replica sums (for standard errors) come with the
+    last one, or, with replicas_untimed, from one more run outside the timed
+    region (a many-job plan's replica block is large: 256 jobs x 32 replicas
+    x 1000 records x 25 values is 1.6 GB)."""
     import torch
     ks = []
     t0 = time.perf_counter()
     r = None
     for k in range(n):
-        r = plan.execute(want_replicas=(want_last and k == n - 1))
+        r = plan.execute(want_replicas=(want_last and not replicas_untimed and k == n - 1))
         ks.append(plan.timing["walk_ms"])
     torch.cuda.synchronize()
-    return r, time.perf_counter() - t0, float(np.mean(ks))
+    el = time.perf_counter() - t0
+    if want_last and replicas_untimed:
+        r = plan.execute(want_replicas=True)
+    return r, el, float(np.mean(ks))
 
 
 def main():
@@ -130,6 +150,8 @@ def main():
     ap.add_argument("--tn", type=int, default=100_000)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--storage", default="auto", choices=["auto", "dense", "bricks"],
+                    help="device layout of the cell words (auto: dense while HBM has room, then bricks)")
     ap.add_argument("--cpu-configs", default="c1,c2,c3,c4,c5")
     ap.add_argument("--graph", action="store_true", help="replay the headline run as a captured hipGraph")
     ap.add_argument("--periodic", default="xyz", help="axes of the single-axon lattice whose edges wrap around "
@@ -178,7 +200,7 @@ def main():
     s_vox = math.sqrt(6 * dt * D0) / res
     ops = ops_per_step(s_vox, pgse=True)
     eng = rms.Engine(0)
-    jobs = W.c2_jobs(rms.Geometry, args.walkers, args.periodic, lab)
+    jobs = W.c2_jobs(_geom(rms, args), args.walkers, args.periodic, lab)
     plan = eng.plan(jobs, dt, TN, timepoints=timepoints, precision=args.precision, replicas=32, use_graph=args.graph)
     for _ in range(args.warmup):
         plan.execute(want_replicas=False)
@@ -263,7 +285,7 @@ def form_plain_mirror(rms, eng, args, A):
     """The headline workload with the reference's mirror edges."""
     W = rms.workloads
     dt, TN, T = W.DT, args.tn, min(1000, args.tn)
-    plan = eng.plan(W.c2_jobs(rms.Geometry, args.walkers, ""), dt, TN, timepoints=T, precision=args.precision,
+    plan = eng.plan(W.c2_jobs(_geom(rms, args), args.walkers, ""), dt, TN, timepoints=T, precision=args.precision,
                     replicas=32)
     plan.execute(want_replicas=False)
     n = max(2, min(args.steps, 3))
@@ -286,7 +308,7 @@ def form_planes(rms, eng, args, A):
     last record."""
     W = rms.workloads
     c = W.C1
-    plan = eng.plan(W.c1_jobs(rms.Geometry), W.DT, c["TN"], timepoints=c["timepoints"], precision=args.precision,
+    plan = eng.plan(W.c1_jobs(_geom(rms, args)), W.DT, c["TN"], timepoints=c["timepoints"], precision=args.precision,
                     replicas=32)
     plan.execute(want_replicas=False)
     n = max(3, args.steps)
@@ -312,7 +334,7 @@ def form_batched(rms, eng, args, A):
     W = rms.workloads
     dt, TN = W.DT, args.batched_tn
     T = min(1000, TN)
-    jobs = W.c3_jobs(rms.Geometry, args.batched_walkers)
+    jobs = W.c3_jobs(_geom(rms, args), args.batched_walkers)
     nw = np.array([j["nwalk"] for j in jobs])
     t0 = time.perf_counter()
     best = autotune.tune(eng, jobs, dt, timepoints=10, precision=args.precision, probe_steps=200, reps=1,
@@ -324,9 +346,10 @@ def form_batched(rms, eng, args, A):
                     xcd_split=best["xcd_split"], use_graph=True)
     plan.execute(want_replicas=False)
     n = max(1, min(args.steps, 2))
-    r, el, kms = _timed_runs(plan, n)
+    r, el, kms = _timed_runs(plan, n, replicas_untimed=True)
     plan.close()
     total = float(nw.sum()) * TN
+    hbm, stores = _hbm_bytes(eng, jobs)
     stats = [_stat_summary(A, rr, dt, TN, T, 3, 360) for rr in r]
     rel = np.array([s["D_end_se"] / s["D_end"] for s in stats])
     s = np.array([math.sqrt(6 * dt * 2.0) / W.RES, math.sqrt(6 * dt * 1.5) / W.RES])
@@ -338,6 +361,7 @@ def form_batched(rms, eng, args, A):
         "roofline_frac_estimate": total / (kms / 1e3) * ops / 1e12 / VALU_PEAK_TOPS,
         "geometries": len(jobs), "walkers": int(nw.sum()), "walkers_per_geometry_median": float(np.median(nw)),
         "TN": TN, "records": T, "voxels": int(sum(j["geom"].labels.size for j in jobs)), "label_bytes": 2,
+        "lattice_bytes_in_hbm": hbm, "storage": stores,
         "autotune": {k: best[k] for k in ("block_threads", "grid_blocks", "xcd_split", "pack")},
         "autotune_s": tune_s,
         "geometry0": stats[best["order"].index(0)],
@@ -352,7 +376,7 @@ def form_gradient(rms, eng, args, A):
     W = rms.workloads
     c = W.C4
     dt, TN, T, Wn = W.DT, args.grad_tn, c["timepoints"], args.grad_walkers
-    jobs = W.c4_jobs(rms.Geometry, Wn)
+    jobs = W.c4_jobs(_geom(rms, args), Wn)
     nb = int(jobs[0]["bval"].size)
     TD = A.diff_times(dt, TN, T)
     plan = eng.plan(jobs, dt, TN, timepoints=T, precision=args.precision, replicas=32)
@@ -433,13 +457,14 @@ def form_sweep(rms, eng, args, A):
     t0 = time.perf_counter()
     jobs = []
     for k0 in range(0, ng, 64):
-        for jb in W.c5_jobs(rms.Geometry, range(k0, min(ng, k0 + 64)), nwg):
+        for jb in W.c5_jobs(_geom(rms, args), range(k0, min(ng, k0 + 64)), nwg):
             eng.add_geometry(jb["geom"])
             jb["geom"].labels = jb["geom"].labels[:0, :0, :0]  # the device holds the lattice now
             jobs.append(jb)
         print(f"sweep: {len(jobs)}/{ng} geometries uploaded ({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
               flush=True)
     setup_s = time.perf_counter() - t0
+    hbm, stores = _hbm_bytes(eng, jobs)
     TD = A.diff_times(W.DT, TN, T)
     nb = int(jobs[0]["bval"].size)
     plan = eng.plan(jobs, W.DT, TN, timepoints=T, precision=args.precision, replicas=32)
@@ -464,7 +489,8 @@ def form_sweep(rms, eng, args, A):
         "forward_kernel_ms": float(fwd_ms), "adjoint_kernel_ms": float(adj_ms),
         "geometries": ng, "of_geometries": c["ngeom"], "walkers_per_geometry": nwg, "walkers": int(nwg * ng),
         "TN": TN, "records": T, "bvecs": nb, "voxels": ng * 70 * 70 * 1000, "label_bytes": 2,
-        "lattice_bytes_in_hbm": ng * 70 * 70 * 1000 * 9, "setup_s": setup_s,
+        "lattice_bytes_in_hbm": hbm, "storage": stores,
+        "lattice_bytes_per_voxel": hbm / (ng * 70 * 70 * 1000), "setup_s": setup_s,
         "D_end_median": float(np.median(Dend)), "D_end_range": [float(np.min(Dend)), float(np.max(Dend))],
         "dobj_dkappa_median": float(np.nanmedian(dk)), "dobj_dT2_intra_median": float(np.median(dT2)),
         "dobj_dD_intra_median": float(np.nanmedian(dD)),

@@ -66,8 +66,20 @@ typedef struct {
     /* lattice edges: bit k set = axis k (x, y, z) wraps around (periodic
      * tiling); clear = the reference's mirror (main.cu:246-265, 302-310) */
     int32_t periodic;
-    int32_t reserved;
+    /* device storage of the cell words (the per-voxel neighbour relations the
+     * walk reads): RMS_STORE_DENSE one 8-byte word per voxel; RMS_STORE_BRICKS
+     * 2x2x4-voxel bricks, a 4-byte table entry per brick and the 16 words
+     * (one 128-byte line) only of bricks where a non-null voxel has a
+     * neighbour of another class or outside the lattice (about 2.6 bytes
+     * per voxel in all for an axon with extra-axonal space, against 9, at
+     * some 10 % of walk speed); RMS_STORE_AUTO keeps dense words while the
+     * device has room and picks bricks once the dense words would leave
+     * under 1/8 of its memory free.  Results are identical; the class
+     * lattice (1 byte per voxel) is kept either way. */
+    int32_t storage;
 } rms_geometry;
+
+enum { RMS_STORE_AUTO = 0, RMS_STORE_DENSE = 1, RMS_STORE_BRICKS = 2 };
 
 /* One simulation on one geometry (one reference setup directory). */
 typedef struct {
@@ -189,6 +201,12 @@ const char *rms_version(void);
 
 /* Upload a geometry; replaces thrust::device_vector<int> d_APix (main.cu:532). */
 rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id);
+
+/* Device bytes a geometry holds (class lattice and cell words, or brick
+ * table and stored bricks) and the storage chosen (RMS_STORE_DENSE or
+ * RMS_STORE_BRICKS).  No reference counterpart (the reference keeps int32
+ * labels, main.cu:532). */
+rms_status rms_geometry_info(rms_ctx *ctx, int32_t geom_id, int64_t *device_bytes, int32_t *storage);
 
 /* Derived per-class quantities of a geometry at time step dt, as the
  * reference host computes them (main.cu:429-458): step[c] in voxels,

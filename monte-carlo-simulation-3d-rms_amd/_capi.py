@@ -24,7 +24,7 @@ STATUS = {
 EXPORTS = (
     "rms_create", "rms_destroy", "rms_last_error", "rms_version", "rms_add_geometry", "rms_derive",
     "rms_nval", "rms_plan_create", "rms_plan_execute", "rms_plan_destroy", "rms_run", "rms_plan_adjoint",
-    "rms_run_noalloc", "rms_plan_adjoint_ex",
+    "rms_run_noalloc", "rms_plan_adjoint_ex", "rms_geometry_info",
 )
 # D estimators of the adjoint (rms_adjoint_opts.d_estimator)
 D_ESTIMATOR = {"auto": 0, "fd": 1, "boundary": 2, "pathwise": 3}
@@ -54,7 +54,7 @@ class Geometry(C.Structure):
         ("T2", C.c_double * NC_MAX),
         ("kappa", C.c_double * (NC_MAX * NC_MAX)),
         ("periodic", C.c_int32),
-        ("reserved", C.c_int32),
+        ("storage", C.c_int32),
     ]
 
 
@@ -152,6 +152,7 @@ def lib():
     L.rms_last_error.restype = C.c_char_p
     L.rms_version.restype = C.c_char_p
     L.rms_add_geometry.argtypes = [C.c_void_p, C.POINTER(Geometry), C.POINTER(C.c_int32)]
+    L.rms_geometry_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.rms_derive.argtypes = [C.POINTER(Geometry), C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
     L.rms_nval.argtypes = [C.c_int32, C.c_int32]
     L.rms_nval.restype = C.c_int32

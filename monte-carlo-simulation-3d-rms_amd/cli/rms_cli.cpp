@@ -174,7 +174,7 @@ int run_body(int argc, char **argv, bool pgse)
     }
     for (int i = 0; i < RMS_NC_MAX * RMS_NC_MAX; i++) g.kappa[i] = -1.0;
     g.periodic = o.periodic;
-    g.reserved = 0;
+    g.storage = RMS_STORE_AUTO;
     double step[RMS_NC_MAX], T2s[RMS_NC_MAX], P[RMS_NC_MAX * RMS_NC_MAX];
     if (rms_derive(&g, sp.dt, step, T2s, P) != RMS_OK) return die("invalid physical parameters");
     for (int i = 0; i < sp.Nc; i++)

@@ -24,8 +24,12 @@ using namespace rms;
 
 struct GeomDev {
     rms_geometry meta;     // labels / label_class pointers cleared
-    uint64_t *d_cells = nullptr;  // cell words (straight-move outcomes per crossing pattern)
+    uint64_t *d_cells = nullptr;  // cell words (straight-move outcomes per crossing pattern): dense or stored bricks
     uint8_t *d_cls = nullptr;     // class per voxel
+    uint32_t *d_btab = nullptr;   // brick table (brick storage), else null
+    int32_t nby = 0, nbz = 0;     // brick grid (brick storage)
+    int32_t storage = RMS_STORE_DENSE;
+    int64_t device_bytes = 0;
     int64_t nvox = 0;
     bool has_walkable = false;
     uint32_t flags = 0;           // geometry_flags: boundary terms of the D derivative
@@ -199,6 +203,7 @@ extern "C" rms_status rms_destroy(rms_ctx *ctx)
     for (auto &g : ctx->geoms) {
         if (g.d_cells) (void)hipFree(g.d_cells);
         if (g.d_cls) (void)hipFree(g.d_cls);
+        if (g.d_btab) (void)hipFree(g.d_btab);
     }
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -225,6 +230,48 @@ extern "C" rms_status rms_derive(const rms_geometry *geom, double dt, double *st
     return guard(nullptr, [&] { return derive_impl(geom, dt, step, T2s, P); });
 }
 
+// Replace a geometry's dense cell words by brick storage (see rms_types.h):
+// count the bricks to store per workgroup, prefix-sum the counts on the
+// host, write the table and the stored bricks, free the dense words.
+static hipError_t build_bricks(rms_ctx *ctx, GeomDev &gd)
+{
+    const rms_geometry &g = gd.meta;
+    const int32_t nbx = (g.nx + 1) / 2, nby = (g.ny + 1) / 2, nbz = (g.nz + 3) / 4;
+    const int64_t nbrick = (int64_t)nbx * nby * nbz;
+    const int64_t nblk = (nbrick + 255) / 256;
+    hipStream_t s = ctx->stream;
+    uint32_t *d_cnt = nullptr;
+    uint64_t *store = nullptr;
+    std::vector<uint32_t> cnt((size_t)nblk);
+    hipError_t e = hipMalloc(&d_cnt, sizeof(uint32_t) * (size_t)nblk);
+    if (e == hipSuccess) e = launch_brick_count(gd.d_cells, gd.d_cls, g.nx, g.ny, g.nz, nby, nbz, nbrick, d_cnt, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_cnt, sizeof(uint32_t) * (size_t)nblk, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    uint64_t nstored = 0;
+    for (int64_t k = 0; k < nblk && e == hipSuccess; k++) {
+        const uint32_t c = cnt[(size_t)k];
+        cnt[(size_t)k] = (uint32_t)nstored;
+        nstored += c;
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(d_cnt, cnt.data(), sizeof(uint32_t) * (size_t)nblk, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMalloc(&gd.d_btab, sizeof(uint32_t) * (size_t)nbrick);
+    if (e == hipSuccess) e = hipMalloc(&store, sizeof(uint64_t) * 16 * (size_t)std::max<uint64_t>(nstored, 1));
+    if (e == hipSuccess)
+        e = launch_brick_fill(gd.d_cells, gd.d_cls, g.nx, g.ny, g.nz, nby, nbz, nbrick, d_cnt, gd.d_btab, store, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (d_cnt) (void)hipFree(d_cnt);
+    if (e != hipSuccess) {
+        if (store) (void)hipFree(store);
+        return e;
+    }
+    (void)hipFree(gd.d_cells);
+    gd.d_cells = store;
+    gd.nby = nby;
+    gd.nbz = nbz;
+    gd.device_bytes = gd.nvox + 4 * nbrick + 128 * (int64_t)std::max<uint64_t>(nstored, 1);
+    return hipSuccess;
+}
+
 static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id)
 {
     if (!ctx || !geom || !geom_id) return fail(ctx, RMS_ERR_INVALID_ARG, "null argument");
@@ -238,6 +285,8 @@ static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int3
     if (g.nclass < 1 || g.nclass > NC) return fail(ctx, RMS_ERR_INVALID_ARG, "nclass must be in 1..4");
     if (!(g.res > 0.0)) return fail(ctx, RMS_ERR_INVALID_ARG, "voxel size must be positive");
     if (g.periodic & ~7) return fail(ctx, RMS_ERR_INVALID_ARG, "periodic: bits 0..2 (x, y, z) only");
+    if (g.storage < RMS_STORE_AUTO || g.storage > RMS_STORE_BRICKS)
+        return fail(ctx, RMS_ERR_INVALID_ARG, "storage must be RMS_STORE_AUTO, RMS_STORE_DENSE or RMS_STORE_BRICKS");
     if (g.label_class) {
         if (g.nlabel < 1) return fail(ctx, RMS_ERR_INVALID_ARG, "nlabel must be >= 1 with a class map");
         for (int i = 0; i < g.nlabel; i++)
@@ -283,9 +332,21 @@ static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int3
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (d_lab) (void)hipFree(d_lab);
     if (d_map) (void)hipFree(d_map);
+    gd.storage = g.storage;
+    if (gd.storage == RMS_STORE_AUTO) {
+        // dense words (the faster walk) while the device has room, bricks
+        // once keeping them would leave under 1/8 of its memory free
+        size_t fr = 0, tot = 0;
+        const bool known = hipMemGetInfo(&fr, &tot) == hipSuccess;
+        gd.storage = (known && (double)fr < 8.0 * (double)nvox + (double)tot / 8.0) ? RMS_STORE_BRICKS
+                                                                                     : RMS_STORE_DENSE;
+    }
+    gd.device_bytes = nvox * 9;
+    if (e == hipSuccess && gd.storage == RMS_STORE_BRICKS) e = build_bricks(ctx, gd);
     if (e != hipSuccess) {
         if (gd.d_cells) (void)hipFree(gd.d_cells);
         if (gd.d_cls) (void)hipFree(gd.d_cls);
+        if (gd.d_btab) (void)hipFree(gd.d_btab);
         ctx->err = std::string("geometry upload: ") + hipGetErrorString(e);
         return e == hipErrorOutOfMemory ? RMS_ERR_OUT_OF_MEMORY : RMS_ERR_HIP;
     }
@@ -294,6 +355,7 @@ static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int3
     } catch (...) {
         (void)hipFree(gd.d_cells);
         (void)hipFree(gd.d_cls);
+        if (gd.d_btab) (void)hipFree(gd.d_btab);
         throw;
     }
     *geom_id = (int32_t)ctx->geoms.size() - 1;
@@ -303,6 +365,15 @@ static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int3
 extern "C" rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id)
 {
     return guard(ctx, [&] { return add_geometry_impl(ctx, geom, geom_id); });
+}
+
+extern "C" rms_status rms_geometry_info(rms_ctx *ctx, int32_t geom_id, int64_t *device_bytes, int32_t *storage)
+{
+    if (!ctx) return RMS_ERR_INVALID_ARG;
+    if (geom_id < 0 || geom_id >= (int32_t)ctx->geoms.size()) return fail(ctx, RMS_ERR_INVALID_ARG, "bad geom_id");
+    if (device_bytes) *device_bytes = ctx->geoms[geom_id].device_bytes;
+    if (storage) *storage = ctx->geoms[geom_id].storage;
+    return RMS_OK;
 }
 
 // ---------------------------------------------------------------- plans
@@ -426,6 +497,9 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         sig |= bits;
         D.cells = G.d_cells;
         D.cls = G.d_cls;
+        D.btab = G.d_btab;
+        D.nby = G.nby;
+        D.nbz = G.nbz;
         H.exchange = exch;
         H.kappa_closed = kappa_closes_pair(G.meta, d);
         H.geom_id = J.geom_id;
@@ -1167,6 +1241,9 @@ static rms_status run_noalloc_impl(rms_ctx *ctx, const rms_job *job, const rms_r
     auto dp = [&](int64_t off) { return (void *)(S + off); };
     D.cells = G.d_cells;
     D.cls = G.d_cls;
+    D.btab = G.d_btab;
+    D.nby = G.nby;
+    D.nbz = G.nbz;
     D.batch0 = 0;
     D.wbase = 0;
     D.part = (double *)dp(L.part);

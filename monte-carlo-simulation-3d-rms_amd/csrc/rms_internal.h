@@ -14,6 +14,15 @@ hipError_t launch_fill_zero(void *p, int64_t bytes, hipStream_t s);
 // Cell words and classes of a label lattice (see rms_types.h).
 hipError_t launch_build_cells(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
                               int32_t ny, int32_t nz, uint64_t *cells, uint8_t *cls, hipStream_t s);
+// Brick storage of the cell words (2x2x4 bricks, nbrick = nbx*nby*nbz in
+// brick order): pass 1 counts the bricks to store per workgroup of 256
+// bricks (counts[ceil(nbrick/256)]); pass 2, given the exclusive prefix sums
+// of those counts, writes the brick table and the stored bricks.
+hipError_t launch_brick_count(const uint64_t *cells, const uint8_t *cls, int32_t nx, int32_t ny, int32_t nz,
+                              int32_t nby, int32_t nbz, int64_t nbrick, uint32_t *counts, hipStream_t s);
+hipError_t launch_brick_fill(const uint64_t *cells, const uint8_t *cls, int32_t nx, int32_t ny, int32_t nz,
+                             int32_t nby, int32_t nbz, int64_t nbrick, const uint32_t *first, uint32_t *btab,
+                             uint64_t *store, hipStream_t s);
 
 // Tangent (adjoint) pass: same walk with forward-mode tangents, contracted
 // with the record/signal cotangents per walker.

@@ -78,6 +78,93 @@ __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, cons
     }
 }
 
+// Brick storage of the cell words.  Brick (bx, by, bz) holds the voxels
+// x in {2bx, 2bx+1}, y in {2by, 2by+1}, z in 4bz..4bz+3 (those inside the
+// lattice); it needs its words stored when one of its non-null voxels has a
+// non-zero word (null voxels never hold a walker, so their words are never
+// read).
+__device__ __forceinline__ bool brick_needed(const uint64_t *cells, const uint8_t *cls, int32_t nx, int32_t ny,
+                                             int32_t nz, int32_t bx, int32_t by, int32_t bz)
+{
+    bool need = false;
+    for (int32_t q = 0; q < 16; q++) {
+        const int32_t x = 2 * bx + (q >> 3), y = 2 * by + ((q >> 2) & 1), z = 4 * bz + (q & 3);
+        if (x >= nx || y >= ny || z >= nz) continue;
+        const int64_t lin = ((int64_t)x * ny + y) * nz + z;
+        need = need || (cls[lin] != 0 && cells[lin] != 0ull);
+    }
+    return need;
+}
+
+// Pass 1: number of stored bricks per workgroup of 256 bricks.
+__global__ void brick_count_kernel(const uint64_t *cells, const uint8_t *cls, int32_t nx, int32_t ny, int32_t nz,
+                                   int32_t nby, int32_t nbz, int64_t nbrick, uint32_t *counts)
+{
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool need = false;
+    if (b < nbrick) {
+        const int32_t bz = (int32_t)(b % nbz);
+        const int64_t t = b / nbz;
+        need = brick_needed(cells, cls, nx, ny, nz, (int32_t)(t / nby), (int32_t)(t % nby), bz);
+    }
+    const int n = __syncthreads_count(need ? 1 : 0);
+    if (threadIdx.x == 0) counts[blockIdx.x] = (uint32_t)n;
+}
+
+// Pass 2: table entries and stored bricks; a workgroup's stored bricks take
+// slots first[blockIdx.x], ... in brick order (the exclusive prefix sums of
+// pass 1's counts).  Voxels of a stored brick outside the lattice hold 0.
+__global__ void brick_fill_kernel(const uint64_t *cells, const uint8_t *cls, int32_t nx, int32_t ny, int32_t nz,
+                                  int32_t nby, int32_t nbz, int64_t nbrick, const uint32_t *first, uint32_t *btab,
+                                  uint64_t *store)
+{
+    __shared__ uint32_t wave_n[4];
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int32_t bx = 0, by = 0, bz = 0;
+    bool need = false;
+    if (b < nbrick) {
+        bz = (int32_t)(b % nbz);
+        const int64_t t = b / nbz;
+        bx = (int32_t)(t / nby);
+        by = (int32_t)(t % nby);
+        need = brick_needed(cells, cls, nx, ny, nz, bx, by, bz);
+    }
+    const uint64_t bal = __ballot(need);
+    const int lane = (int)(threadIdx.x & 63u), wv = (int)(threadIdx.x >> 6);
+    if (lane == 0) wave_n[wv] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t before = first[blockIdx.x];
+    for (int k = 0; k < wv; k++) before += wave_n[k];
+    before += (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    if (b >= nbrick) return;
+    btab[b] = need ? before : BRICK_ZERO;
+    if (!need) return;
+    for (int32_t q = 0; q < 16; q++) {
+        const int32_t x = 2 * bx + (q >> 3), y = 2 * by + ((q >> 2) & 1), z = 4 * bz + (q & 3);
+        const bool in = x < nx && y < ny && z < nz;
+        store[(uint64_t)before * 16u + (uint32_t)q] = in ? cells[((int64_t)x * ny + y) * nz + z] : 0ull;
+    }
+}
+
+hipError_t launch_brick_count(const uint64_t *cells, const uint8_t *cls, int32_t nx, int32_t ny, int32_t nz,
+                              int32_t nby, int32_t nbz, int64_t nbrick, uint32_t *counts, hipStream_t s)
+{
+    const int64_t grid = (nbrick + 255) / 256;
+    hipLaunchKernelGGL(brick_count_kernel, dim3((unsigned)grid), dim3(256), 0, s, cells, cls, nx, ny, nz, nby, nbz,
+                       nbrick, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_fill(const uint64_t *cells, const uint8_t *cls, int32_t nx, int32_t ny, int32_t nz,
+                             int32_t nby, int32_t nbz, int64_t nbrick, const uint32_t *first, uint32_t *btab,
+                             uint64_t *store, hipStream_t s)
+{
+    const int64_t grid = (nbrick + 255) / 256;
+    hipLaunchKernelGGL(brick_fill_kernel, dim3((unsigned)grid), dim3(256), 0, s, cells, cls, nx, ny, nz, nby, nbz,
+                       nbrick, first, btab, store);
+    return hipGetLastError();
+}
+
 // Scratch of the allocation-free entry point (rms_run_noalloc): every device
 // buffer of such a run beyond its inputs and outputs.  One copy per GPU, held
 // for the life of the process.  In host code `scratch` is only a handle:

@@ -27,11 +27,18 @@ constexpr uint64_t CELL_SAME = 0;
 constexpr uint64_t CELL_NULL = 1;
 constexpr uint64_t CELL_OTHER = 2;
 constexpr uint64_t CELL_EDGE = 3;
+constexpr uint32_t BRICK_ZERO = 0xFFFFFFFFu;
 
 struct JobDesc {
     // geometry
-    const uint64_t *cells;     // cell words, see above
+    const uint64_t *cells;     // cell words, see above (dense, or the stored bricks)
     const uint8_t *cls;        // class of each voxel
+    // brick storage (null: dense): entry per 2x2x4 brick, index
+    // ((x>>1)*nby + (y>>1))*nbz + (z>>2); BRICK_ZERO when every non-null voxel
+    // of the brick has the word 0, else the brick's slot in cells (16 words,
+    // local index (x&1)*8 + (y&1)*4 + (z&3))
+    const uint32_t *btab;
+    int32_t nby, nbz;
     int32_t nx, ny, nz;
     int32_t pg_log2;           // PGSE sums flush at least every 2^pg_log2 steps
     int32_t nclass;

@@ -107,6 +107,8 @@ template <typename R>
 struct JobLocal {
     const uint64_t *cells;
     const uint8_t *cls;
+    const uint32_t *btab;    // brick table (null: dense cell words)
+    int32_t nby, nbz;
     int32_t nx, ny, nz, sx;  // sx = ny*nz
     int32_t nclass;
     uint32_t key0, key1, pcat;
@@ -139,6 +141,9 @@ __device__ __forceinline__ JobLocal<R> job_local(const JobDesc &J)
     JobLocal<R> L;
     L.cells = J.cells;
     L.cls = J.cls;
+    L.btab = J.btab;
+    L.nby = J.nby;
+    L.nbz = J.nbz;
     L.nx = J.nx;
     L.ny = J.ny;
     L.nz = J.nz;
@@ -223,6 +228,25 @@ template <typename R>
 __device__ __forceinline__ int32_t class_at(const JobLocal<R> &L, int32_t lin)
 {
     return (int32_t)L.cls[lin];
+}
+
+// Cell word of voxel (x, y, z), linear index lin.  Dense storage reads
+// cells[lin]; brick storage reads the brick's table entry and, unless the
+// brick holds only zero words, the word in the brick's 128-byte slot.  The
+// storage is per job, so the branch is uniform over a wavefront.
+template <typename R>
+__device__ __forceinline__ uint64_t cell_at(const JobLocal<R> &L, int32_t x, int32_t y, int32_t z, int32_t lin)
+{
+    if (L.btab == nullptr) return L.cells[lin];
+    const uint32_t e = L.btab[((x >> 1) * L.nby + (y >> 1)) * L.nbz + (z >> 2)];
+    if (e == BRICK_ZERO) return 0ull;
+    return L.cells[(uint64_t)e * 16u + (uint32_t)(((x & 1) << 3) | ((y & 1) << 2) | (z & 3))];
+}
+
+template <typename R>
+__device__ __forceinline__ uint64_t cell_of(const JobLocal<R> &L, const Walker<R> &w)
+{
+    return cell_at<R>(L, w.g.x, w.g.y, w.g.z, w.lin);
 }
 
 // Move the time spent in the current class into its FP64 total.  Every class
@@ -1160,8 +1184,8 @@ __device__ void corner_events(const Walker<R> &w, const Tan<R, NP> &T, const Job
             Wj = Wj + om;
             if (corner_uniform<R>(L, wid, i, 4 * pr + qd) * Wj < om) {
                 has = true;
-                store_cont<R>(cont, 0, t, B0, L.cells[B0.lin]);
-                store_cont<R>(cont, 1, t, B1, L.cells[B1.lin]);
+                store_cont<R>(cont, 0, t, B0, cell_of<R>(L, B0));
+                store_cont<R>(cont, 1, t, B1, cell_of<R>(L, B1));
 #pragma unroll
                 for (int p = 0; p < NP; p++) rho[p] = wp[p] / om;
             }
@@ -1432,7 +1456,7 @@ void walk_kernel(LaunchArgs a)
         uint32_t b0 = 0, b1 = 0;
         double *part = J.part + lb * (int64_t)a.timepoints * nval;
         int32_t r = rnext / Tstep;  // record index of rnext
-        uint64_t cell = L.cells[w.lin];
+        uint64_t cell = cell_of<R>(L, w);
         // adjoint boundary terms: the kept pair of continuations (in LDS,
         // continuation 0 without the event, 1 with it) and the reservoir
         const int tix = (int)threadIdx.x;
@@ -1457,7 +1481,7 @@ void walk_kernel(LaunchArgs a)
                         const int32_t l0 = a.lin;
                         if (!fast_step<R, 0, EXCH, PER>(a, L, J, ca, v, wid, i, xhi, 0, T0))
                             march<R, 0, EXCH, PER>(a, L, J, wid, i, v, xhi, 0, T0);
-                        if (a.lin != l0) ca = L.cells[a.lin];
+                        if (a.lin != l0) ca = cell_of<R>(L, a);
                         store_cont<R>(cont, m, tix, a, ca);
                     }
                 }
@@ -1495,8 +1519,8 @@ void walk_kernel(LaunchArgs a)
                         Wj = Wj + om;
                         if (jump_uniform<R>(L, wid, i, m) * Wj < om) {
                             has = true;
-                            store_cont<R>(cont, 0, tix, B0, L.cells[B0.lin]);
-                            store_cont<R>(cont, 1, tix, B1, L.cells[B1.lin]);
+                            store_cont<R>(cont, 0, tix, B0, cell_of<R>(L, B0));
+                            store_cont<R>(cont, 1, tix, B1, cell_of<R>(L, B1));
 #pragma unroll
                             for (int p = 0; p < NP; p++) rho[p] = wp[p] / om;
                         }
@@ -1514,7 +1538,7 @@ void walk_kernel(LaunchArgs a)
             const int32_t npg = i - i0;  // PGSE terms since the last flush
             if (!fast_step<R, NP, EXCH, PER>(w, L, J, cell, v, wid, i, xhi, npg, T))
                 march<R, NP, EXCH, PER>(w, L, J, wid, i, v, xhi, npg, T);
-            if (w.lin != lin0) cell = L.cells[w.lin];  // only lanes that changed voxel read
+            if (w.lin != lin0) cell = cell_of<R>(L, w);  // only lanes that changed voxel read
             if (SIG & 2) {
                 // running sums of the displacement in voxels, (g + o) - (gi + oi),
                 // as sum(o) in FP64 and sum(g) in int32 until the next flush; the

@@ -31,9 +31,13 @@ class Geometry:
     to every pair, a matrix per pair.  label_class optionally maps labels to
     classes (uint16 lattices with many labels).  periodic names the axes
     ("x", "y", "z" in any combination) whose lattice edges wrap around; the
-    others are the reference's mirrors (main.cu:246-265)."""
+    others are the reference's mirrors (main.cu:246-265).  storage: "auto",
+    "dense" or "bricks", the device layout of the cell words
+    (rms_geometry.storage; results are identical)."""
 
-    def __init__(self, labels, res, D, T2=None, kappa=None, label_class=None, periodic=""):
+    STORAGE = {"auto": 0, "dense": 1, "bricks": 2}
+
+    def __init__(self, labels, res, D, T2=None, kappa=None, label_class=None, periodic="", storage="auto"):
         labels = np.asarray(labels)
         if labels.ndim != 3:
             raise ValueError("labels must be a 3-D array")
@@ -63,6 +67,9 @@ class Geometry:
         if any(a not in "xyz" for a in periodic):
             raise ValueError("periodic: axes among 'x', 'y', 'z'")
         self.periodic = sum(1 << "xyz".index(a) for a in set(periodic))
+        if storage not in self.STORAGE:
+            raise ValueError("storage: 'auto', 'dense' or 'bricks'")
+        self.storage = storage
 
     @property
     def nclass(self):
@@ -87,6 +94,7 @@ class Geometry:
         for i in range(capi.NC_MAX * capi.NC_MAX):
             g.kappa[i] = float(self.kappa.flat[i])
         g.periodic = self.periodic
+        g.storage = self.STORAGE[self.storage]
         return g
 
     def derive(self, dt):
@@ -161,6 +169,13 @@ class Engine:
         self._check(capi.lib().rms_add_geometry(self._h, C.byref(g), C.byref(gid)), "rms_add_geometry")
         self._geom_ids[key] = (gid.value, geom)
         return gid.value
+
+    def geometry_info(self, geom: Geometry):
+        """(device bytes, "dense" or "bricks") of an uploaded geometry."""
+        gid = self.add_geometry(geom)
+        nb, st = C.c_int64(), C.c_int32()
+        self._check(capi.lib().rms_geometry_info(self._h, gid, C.byref(nb), C.byref(st)), "rms_geometry_info")
+        return int(nb.value), {1: "dense", 2: "bricks"}[st.value]
 
     def plan(self, jobs, dt, TN, timepoints=1000, precision="f32", replicas=32, chunk_steps=0, use_graph=False,
              block_threads=0, grid_blocks=0, xcd_split=0):

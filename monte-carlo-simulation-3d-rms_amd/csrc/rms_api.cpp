@@ -495,6 +495,7 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         Derived d;
         (void)derive(G.meta, P.dt, d);
         sig |= bits;
+        if (G.d_btab) sig |= SIG_BRICKS;
         D.cells = G.d_cells;
         D.cls = G.d_cls;
         D.btab = G.d_btab;
@@ -1222,6 +1223,7 @@ static rms_status run_noalloc_impl(rms_ctx *ctx, const rms_job *job, const rms_r
     if (const char *e = fill_job(G.meta, *job, P, D, sig, exch)) return bad(RMS_ERR_INVALID_ARG, e);
     if (job->nbvec > 0) sig |= SIG_NARROW;
     if (job->ngrad > 0) sig |= SIG_PGSE;
+    if (G.d_btab) sig |= SIG_BRICKS;
     // PGSE lobe tables on the stack
     int32_t perm[NOALLOC_MAX_GRAD], trip[4 * (NOALLOC_MAX_TRIPLE + 1)];
     uint64_t keys[NOALLOC_MAX_GRAD];

@@ -420,6 +420,8 @@ const char *check_desc(const JobDesc &D, int sig)
         return "job with a gradient table in a plan without the PGSE variant";
     }
     if ((sig & SIG_PERIODIC) == 0 && D.periodic != 0) return "periodic job in a plan without the periodic variant";
+    if ((sig & SIG_BRICKS) == 0 && D.btab != nullptr) return "brick-stored job in a plan without the brick variant";
+    if (D.btab != nullptr && (D.nby < 1 || D.nbz < 1)) return "brick-stored job without its brick grid";
     if ((D.periodic & ~7) != 0) return "periodic: bits 0..2 (x, y, z) only";
     return nullptr;
 }

@@ -15,7 +15,7 @@
 namespace rms {
 
 // Kernel variant bits (the SIG template argument of walk_kernel).
-enum : int { SIG_NARROW = 1, SIG_PGSE = 2, SIG_EXCH = 4, SIG_PERIODIC = 8, SIG_JUMP = 16 };
+enum : int { SIG_NARROW = 1, SIG_PGSE = 2, SIG_EXCH = 4, SIG_PERIODIC = 8, SIG_JUMP = 16, SIG_BRICKS = 32 };
 
 // The adjoint variant of a plan: the boundary terms of the D derivative
 // (exchange faces, lattice-edge mirrors, voxel edges) whenever no job has a

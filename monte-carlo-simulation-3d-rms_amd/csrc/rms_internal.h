@@ -35,6 +35,9 @@ const void *jump_walk_fn(int prec_f64, int sig, int np);
 // (sig bits 1 and 4 clear) for np tangent parameters
 // (rms_kernels_tangent.hip).
 const void *tangent_walk_fn(int prec_f64, int sig, int np);
+// The forward and tangent variants for plans with a brick-stored geometry
+// (sig bit 5, without bit 4) (rms_kernels_bricks.hip).
+const void *brick_walk_fn(int prec_f64, int sig, int np);
 
 // Allocation-free runs: device address of this GPU's scratch array (the
 // current device's copy), and the two table kernels that fill it.

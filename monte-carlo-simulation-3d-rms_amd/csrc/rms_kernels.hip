@@ -231,6 +231,7 @@ static const void *walk_fn(int prec_f64, int sig)
 static const void *walk_fn_np(int prec_f64, int sig, int np)
 {
     if ((sig & 16) && np > 0) return jump_walk_fn(prec_f64, sig, np);
+    if (sig & 32) return brick_walk_fn(prec_f64, sig, np);
     if (!(sig & 2) && np > 0) return tangent_walk_fn(prec_f64, sig, np);
     switch (np) {
     case 0: return walk_fn<0>(prec_f64, sig);

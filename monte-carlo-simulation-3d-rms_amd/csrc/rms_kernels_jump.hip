@@ -7,13 +7,16 @@
 namespace rms {
 
 // Adjoint variants with the jump term (bit 4): exchange faces (bit 2), no
-// PGSE table, with or without the b-table (bit 0) and periodic edges (bit 3).
+// PGSE table, with or without the b-table (bit 0), periodic edges (bit 3)
+// and brick-stored cell words (bit 5).
 template <typename R, int NP>
 static const void *jump_fn(int sig)
 {
     static const void *const fns[] = {(const void *)walk_kernel<R, 20, NP>, (const void *)walk_kernel<R, 21, NP>,
-                                      (const void *)walk_kernel<R, 28, NP>, (const void *)walk_kernel<R, 29, NP>};
-    return fns[(sig & 1) | ((sig >> 2) & 2)];
+                                      (const void *)walk_kernel<R, 28, NP>, (const void *)walk_kernel<R, 29, NP>,
+                                      (const void *)walk_kernel<R, 52, NP>, (const void *)walk_kernel<R, 53, NP>,
+                                      (const void *)walk_kernel<R, 60, NP>, (const void *)walk_kernel<R, 61, NP>};
+    return fns[(sig & 1) | ((sig >> 2) & 2) | ((sig >> 3) & 4)];
 }
 
 template <int NP>

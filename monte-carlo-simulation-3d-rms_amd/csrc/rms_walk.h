@@ -233,20 +233,26 @@ __device__ __forceinline__ int32_t class_at(const JobLocal<R> &L, int32_t lin)
 // Cell word of voxel (x, y, z), linear index lin.  Dense storage reads
 // cells[lin]; brick storage reads the brick's table entry and, unless the
 // brick holds only zero words, the word in the brick's 128-byte slot.  The
-// storage is per job, so the branch is uniform over a wavefront.
-template <typename R>
+// kernel variants with BR (plans with a brick-stored geometry) choose per
+// job, with a branch uniform over the wavefront; the others read dense words
+// only.
+template <typename R, bool BR>
 __device__ __forceinline__ uint64_t cell_at(const JobLocal<R> &L, int32_t x, int32_t y, int32_t z, int32_t lin)
 {
-    if (L.btab == nullptr) return L.cells[lin];
-    const uint32_t e = L.btab[((x >> 1) * L.nby + (y >> 1)) * L.nbz + (z >> 2)];
-    if (e == BRICK_ZERO) return 0ull;
-    return L.cells[(uint64_t)e * 16u + (uint32_t)(((x & 1) << 3) | ((y & 1) << 2) | (z & 3))];
+    if constexpr (BR) {
+        if (L.btab != nullptr) {
+            const uint32_t e = L.btab[((x >> 1) * L.nby + (y >> 1)) * L.nbz + (z >> 2)];
+            if (e == BRICK_ZERO) return 0ull;
+            return L.cells[(uint64_t)e * 16u + (uint32_t)(((x & 1) << 3) | ((y & 1) << 2) | (z & 3))];
+        }
+    }
+    return L.cells[lin];
 }
 
-template <typename R>
+template <typename R, bool BR>
 __device__ __forceinline__ uint64_t cell_of(const JobLocal<R> &L, const Walker<R> &w)
 {
-    return cell_at<R>(L, w.g.x, w.g.y, w.g.z, w.lin);
+    return cell_at<R, BR>(L, w.g.x, w.g.y, w.g.z, w.lin);
 }
 
 // Move the time spent in the current class into its FP64 total.  Every class
@@ -1127,7 +1133,7 @@ __device__ __forceinline__ bool same_outcome(const Walker<R> &x, const V3<R> &vx
 
 // The voxel-edge events of step i from the walker's state w before it (its
 // D tangents T, direction vr, step s in voxels); updates the reservoir.
-template <typename R, int NP, bool EXCH, bool PER>
+template <typename R, int NP, bool EXCH, bool PER, bool BR>
 __device__ void corner_events(const Walker<R> &w, const Tan<R, NP> &T, const JobLocal<R> &L, const JobDesc &J,
                               const V3<R> &vr, double s, uint64_t wid, int32_t i, uint32_t xhi, bool &has,
                               double &Wj, double *rho, uint32_t *cont, int t)
@@ -1184,8 +1190,8 @@ __device__ void corner_events(const Walker<R> &w, const Tan<R, NP> &T, const Job
             Wj = Wj + om;
             if (corner_uniform<R>(L, wid, i, 4 * pr + qd) * Wj < om) {
                 has = true;
-                store_cont<R>(cont, 0, t, B0, cell_of<R>(L, B0));
-                store_cont<R>(cont, 1, t, B1, cell_of<R>(L, B1));
+                store_cont<R>(cont, 0, t, B0, cell_of<R, BR>(L, B0));
+                store_cont<R>(cont, 1, t, B1, cell_of<R, BR>(L, B1));
 #pragma unroll
                 for (int p = 0; p < NP; p++) rho[p] = wp[p] / om;
             }
@@ -1344,7 +1350,8 @@ __device__ __forceinline__ void adj_record(const double *cot, int nclass, int nb
 // The persistent walk kernel.  SIG: bit 0 narrow-pulse b-table signal,
 // bit 1 PGSE signal, bit 2 exchange faces on the fast path, bit 3 periodic
 // lattice edges (the march wraps; without it every edge is a mirror), bit 4
-// (adjoint only, no PGSE) the jump of the D derivative at exchange faces.
+// (adjoint only, no PGSE) the jump of the D derivative at exchange faces,
+// bit 5 a geometry of the plan has its cell words in bricks.
 // NP > 0 instantiates the adjoint pass: the same walk with tangents for D of
 // classes 1..NP, contracted with the cotangents of the record rows and PGSE
 // signal into per-batch parameter gradients.
@@ -1359,6 +1366,7 @@ void walk_kernel(LaunchArgs a)
     constexpr bool JUMP = NP > 0 && (SIG & 16) != 0;
     constexpr bool EXCH = (SIG & 4) != 0;
     constexpr bool PER = (SIG & 8) != 0;
+    constexpr bool BR = (SIG & 32) != 0;
     // sin/cos table of the FP32 direction draw, staged in LDS
     __shared__ float2 tab[256];
     // continuations of the kept boundary events (jump variants only)
@@ -1456,7 +1464,7 @@ void walk_kernel(LaunchArgs a)
         uint32_t b0 = 0, b1 = 0;
         double *part = J.part + lb * (int64_t)a.timepoints * nval;
         int32_t r = rnext / Tstep;  // record index of rnext
-        uint64_t cell = cell_of<R>(L, w);
+        uint64_t cell = cell_of<R, BR>(L, w);
         // adjoint boundary terms: the kept pair of continuations (in LDS,
         // continuation 0 without the event, 1 with it) and the reservoir
         const int tix = (int)threadIdx.x;
@@ -1481,7 +1489,7 @@ void walk_kernel(LaunchArgs a)
                         const int32_t l0 = a.lin;
                         if (!fast_step<R, 0, EXCH, PER>(a, L, J, ca, v, wid, i, xhi, 0, T0))
                             march<R, 0, EXCH, PER>(a, L, J, wid, i, v, xhi, 0, T0);
-                        if (a.lin != l0) ca = cell_of<R>(L, a);
+                        if (a.lin != l0) ca = cell_of<R, BR>(L, a);
                         store_cont<R>(cont, m, tix, a, ca);
                     }
                 }
@@ -1519,8 +1527,8 @@ void walk_kernel(LaunchArgs a)
                         Wj = Wj + om;
                         if (jump_uniform<R>(L, wid, i, m) * Wj < om) {
                             has = true;
-                            store_cont<R>(cont, 0, tix, B0, cell_of<R>(L, B0));
-                            store_cont<R>(cont, 1, tix, B1, cell_of<R>(L, B1));
+                            store_cont<R>(cont, 0, tix, B0, cell_of<R, BR>(L, B0));
+                            store_cont<R>(cont, 1, tix, B1, cell_of<R, BR>(L, B1));
 #pragma unroll
                             for (int p = 0; p < NP; p++) rho[p] = wp[p] / om;
                         }
@@ -1531,14 +1539,14 @@ void walk_kernel(LaunchArgs a)
                 // reach, with no reflection or class change)
                 if (cell != 0ull) {
                     const double sd = sel4(w.c, L.sd0, L.sd1, L.sd2, L.sd3);
-                    corner_events<R, NP, EXCH, PER>(w, T, L, J, v, sd, wid, i, xhi, has, Wj, rho, cont, tix);
+                    corner_events<R, NP, EXCH, PER, BR>(w, T, L, J, v, sd, wid, i, xhi, has, Wj, rho, cont, tix);
                 }
             }
             const int32_t lin0 = w.lin;
             const int32_t npg = i - i0;  // PGSE terms since the last flush
             if (!fast_step<R, NP, EXCH, PER>(w, L, J, cell, v, wid, i, xhi, npg, T))
                 march<R, NP, EXCH, PER>(w, L, J, wid, i, v, xhi, npg, T);
-            if (w.lin != lin0) cell = cell_of<R>(L, w);  // only lanes that changed voxel read
+            if (w.lin != lin0) cell = cell_of<R, BR>(L, w);  // only lanes that changed voxel read
             if (SIG & 2) {
                 // running sums of the displacement in voxels, (g + o) - (gi + oi),
                 // as sum(o) in FP64 and sum(g) in int32 until the next flush; the

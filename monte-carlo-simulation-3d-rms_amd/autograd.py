@@ -100,3 +100,79 @@ def simulate(engine, labels, res, D, T2, nwalk, dt, TN, timepoints=1000, kappa=N
                 bval=bval, bvec=bvec, grad=grad, precision=precision, replicas=replicas, label_class=label_class,
                 d_estimator=d_estimator, fd_step=fd_step)
     return _Simulate.apply(D, T2, kappa, spec)
+
+
+class _SimulateMany(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, D, T2, kappa, spec):
+        jobs = []
+        for j, js in enumerate(spec["jobs"]):
+            kap = None if kappa is None else float(kappa[j].detach().item())
+            geom = Geometry(js["labels"], js.get("res", spec["res"]), D[j].detach().cpu().double().numpy().tolist(),
+                            T2=T2[j].detach().cpu().double().numpy().tolist(), kappa=kap,
+                            label_class=js.get("label_class"), periodic=js.get("periodic", ""),
+                            storage=js.get("storage", "auto"))
+            jobs.append(dict(geom=geom, nwalk=js["nwalk"], seed=js.get("seed", 1 + j), bval=js.get("bval"),
+                             bvec=js.get("bvec"), grad=js.get("grad")))
+        plan = spec["engine"].plan(jobs, spec["dt"], spec["TN"], spec.get("timepoints", 1000),
+                                   precision=spec.get("precision", "f32"), replicas=spec.get("replicas", 32))
+        try:
+            outs = plan.execute(final_state=False, want_replicas=False)
+        except Exception:
+            plan.close()
+            raise
+        ctx.plan = plan
+        ctx.nc = D.shape[1]
+        ctx.has_kappa = kappa is not None
+        ctx.d_estimator = spec.get("d_estimator", "auto")
+        ctx.fd_step = spec.get("fd_step", 0.0)
+        ctx.ngrad = 0 if outs[0]["sig"] is None else len(outs[0]["sig"])
+        rec = torch.from_numpy(np.stack([np.ascontiguousarray(o["rec"]) for o in outs]))
+        sig = (torch.from_numpy(np.stack([o["sig"] for o in outs])) if ctx.ngrad
+               else torch.zeros((len(outs), 0), dtype=torch.float64))
+        return rec, sig
+
+    @staticmethod
+    def backward(ctx, g_rec, g_sig):
+        plan = ctx.plan
+        try:
+            nj = len(plan.jobs)
+            cr = [None] * nj if g_rec is None else [g_rec[j].detach().cpu().double().contiguous().numpy()
+                                                    for j in range(nj)]
+            cs = None if (g_sig is None or ctx.ngrad == 0) else [g_sig[j].detach().cpu().double().contiguous().numpy()
+                                                                for j in range(nj)]
+            g = np.stack([a["grad"] for a in plan.adjoint(cr, cs, estimator=ctx.d_estimator, fd_step=ctx.fd_step)])
+        finally:
+            plan.close()
+            ctx.plan = None
+        nc = ctx.nc
+        gD = torch.from_numpy(g[:, :nc].copy())
+        gT = torch.from_numpy(g[:, 4:4 + nc].copy())
+        gk = torch.from_numpy(g[:, 8].copy()) if ctx.has_kappa else None
+        return gD, gT, gk, None
+
+
+def simulate_many(engine, jobs, D, T2, dt, TN, timepoints=1000, kappa=None, res=0.1, precision="f32", replicas=32,
+                  d_estimator="auto", fd_step=0.0):
+    """Differentiable simulation of many geometries in one plan (one
+    persistent launch forward, one tangent pass backward).
+
+    jobs: list of dicts with "labels" (and optionally "res", "nwalk",
+    "seed", "bval", "bvec", "grad", "label_class", "periodic", "storage"),
+    every geometry with the same class count and b-table size; D, T2:
+    float64 tensors [njobs, nclass]; kappa: None (the reference rule) or a
+    tensor [njobs], one permeability per geometry.  Returns (rec, sig): rec
+    [njobs, timepoints, nval] record sums, sig [njobs, ngrad] PGSE signal
+    sums.  The gradient of each geometry is the one the single-geometry
+    `simulate` returns for it."""
+    D = torch.as_tensor(D, dtype=torch.float64)
+    T2 = torch.as_tensor(T2, dtype=torch.float64)
+    if D.ndim != 2 or T2.shape != D.shape or D.shape[0] != len(jobs):
+        raise ValueError("D and T2 must be [njobs, nclass]")
+    if kappa is not None:
+        kappa = torch.as_tensor(kappa, dtype=torch.float64).reshape(-1)
+        if kappa.numel() != len(jobs):
+            raise ValueError("kappa must have one value per job")
+    spec = dict(engine=engine, jobs=jobs, dt=dt, TN=TN, timepoints=timepoints, res=res, precision=precision,
+                replicas=replicas, d_estimator=d_estimator, fd_step=fd_step)
+    return _SimulateMany.apply(D, T2, kappa, spec)

@@ -411,3 +411,40 @@ def test_near_max_lattice_bitexact(rms, orc, engine):
     gpu, ref = _both(rms, orc, engine, jobs, "f32", dt=2e-4, TN=300, timepoints=10)
     _assert_same(gpu, ref)
     assert int(np.max(gpu[0]["fin_g"][0])) > 1000 and int(np.max(gpu[0]["fin_g"][2])) > 1400
+
+
+def test_autograd_many_geometries_equals_one_at_a_time(rms, engine):
+    """autograd.simulate_many: three geometries with their own D, T2 and
+    kappa in one plan; the records and the gradients of each (default D
+    estimator: central differences where exchange makes the walk jump) are
+    bit-identical to autograd.simulate on that geometry alone."""
+    import importlib
+    import torch
+    ag = importlib.import_module(rms.__name__ + ".autograd")
+    print("case autograd many geometries", flush=True)
+    labs = [random_blobs((9, 8, 10), 3, seed=40 + k) for k in range(3)]
+    bval, bvec = np.array([0.7, 2.0]), fib_dirs(2)
+    jobs = [dict(labels=labs[k], nwalk=400 + 100 * k, seed=7 + k, bval=bval, bvec=bvec) for k in range(3)]
+    D0 = np.array([[0.0, 2.0, 0.9], [0.0, 1.6, 1.1], [0.0, 2.2, 0.5]])
+    T20 = np.array([[np.inf, 70.0, 30.0], [np.inf, 60.0, 40.0], [np.inf, 90.0, 25.0]])
+    k0 = np.array([0.05, 0.2, 0.1])
+    dt, TN, T = 2e-4, 300, 10
+    D = torch.tensor(D0, requires_grad=True)
+    T2 = torch.tensor(T20, requires_grad=True)
+    kap = torch.tensor(k0, requires_grad=True)
+    rec, sig = ag.simulate_many(engine, jobs, D, T2, dt, TN, timepoints=T, kappa=kap, replicas=4)
+    assert tuple(rec.shape) == (3, T, rms.nval(3, 2)) and tuple(sig.shape) == (3, 0)
+    wr = torch.from_numpy(np.random.default_rng(6).standard_normal(tuple(rec.shape)))
+    (wr * rec).sum().backward()
+    for k in range(3):
+        Dk = torch.tensor(D0[k], requires_grad=True)
+        Tk = torch.tensor(T20[k], requires_grad=True)
+        kk = torch.tensor(k0[k], requires_grad=True)
+        r1, _ = ag.simulate(engine, labs[k], 0.1, Dk, Tk, nwalk=jobs[k]["nwalk"], dt=dt, TN=TN, timepoints=T,
+                            kappa=kk, seed=jobs[k]["seed"], bval=bval, bvec=bvec, replicas=4)
+        (wr[k] * r1).sum().backward()
+        np.testing.assert_array_equal(rec[k].detach().numpy(), r1.detach().numpy())
+        np.testing.assert_array_equal(D.grad[k].numpy(), Dk.grad.numpy())
+        np.testing.assert_array_equal(T2.grad[k, 1:].numpy(), Tk.grad[1:].numpy())
+        assert float(kap.grad[k]) == float(kk.grad)
+        assert np.all(np.isfinite(D.grad[k, 1:].numpy())) and np.isfinite(float(kap.grad[k]))

@@ -113,3 +113,18 @@ def test_bricks_footprint_and_auto_choice(rms, engine):
     jb = W.c5_jobs(lambda *a, **k: rms.Geometry(*a, storage="bricks", **k), [0], 4096)
     jd = W.c5_jobs(lambda *a, **k: rms.Geometry(*a, storage="dense", **k), [0], 4096)
     _same(engine.run(jb, **kw), engine.run(jd, **kw))
+
+
+def test_bricks_allocation_free_run_equals_run(rms, engine):
+    """rms_run_noalloc on a brick-stored geometry (the brick kernel variant
+    through the static scratch array) equals rms_run on it and on its dense
+    twin."""
+    print("case bricks allocation-free run", flush=True)
+    lab = random_blobs((9, 10, 11), 3, seed=90)
+    job = dict(nwalk=700, seed=17, bval=np.array([1.0, 2.0]), bvec=fib_dirs(2))
+    kw = dict(dt=2e-4, TN=300, timepoints=10, precision="f32", replicas=4)
+    gb = rms.Geometry(lab, 0.1, [0.0, 2.0, 0.8], T2=[np.inf, 70.0, 40.0], kappa=0.2, storage="bricks")
+    gd = rms.Geometry(lab, 0.1, [0.0, 2.0, 0.8], T2=[np.inf, 70.0, 40.0], kappa=0.2, storage="dense")
+    na = engine.run_noalloc(dict(geom=gb, **job), **kw)
+    _same([na], engine.run([dict(geom=gb, **job)], **kw))
+    _same([na], engine.run([dict(geom=gd, **job)], **kw))

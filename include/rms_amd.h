@@ -203,10 +203,16 @@ const char *rms_version(void);
 rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, int32_t *geom_id);
 
 /* Device bytes a geometry holds (class lattice and cell words, or brick
- * table and stored bricks) and the storage chosen (RMS_STORE_DENSE or
- * RMS_STORE_BRICKS).  No reference counterpart (the reference keeps int32
- * labels, main.cu:532). */
-rms_status rms_geometry_info(rms_ctx *ctx, int32_t geom_id, int64_t *device_bytes, int32_t *storage);
+ * table and stored bricks), the storage chosen (RMS_STORE_DENSE or
+ * RMS_STORE_BRICKS), and how steps through a periodic side are walked:
+ * wrap_cells = 1 when more than 1/256 of the non-null voxels lie on a
+ * periodic side (the fast path crosses into the voxel on the other side), 0
+ * otherwise (the face march takes those steps).  Results are the same
+ * physics either way; the two give different roundings.  No reference
+ * counterpart (the reference keeps int32 labels, main.cu:532, and mirrors
+ * every edge).  NULL outputs are skipped. */
+rms_status rms_geometry_info(rms_ctx *ctx, int32_t geom_id, int64_t *device_bytes, int32_t *storage,
+                             int32_t *wrap_cells);
 
 /* Derived per-class quantities of a geometry at time step dt, as the
  * reference host computes them (main.cu:429-458): step[c] in voxels,

@@ -152,7 +152,8 @@ def lib():
     L.rms_last_error.restype = C.c_char_p
     L.rms_version.restype = C.c_char_p
     L.rms_add_geometry.argtypes = [C.c_void_p, C.POINTER(Geometry), C.POINTER(C.c_int32)]
-    L.rms_geometry_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    L.rms_geometry_info.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                    C.POINTER(C.c_int32)]
     L.rms_derive.argtypes = [C.POINTER(Geometry), C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
     L.rms_nval.argtypes = [C.c_int32, C.c_int32]
     L.rms_nval.restype = C.c_int32

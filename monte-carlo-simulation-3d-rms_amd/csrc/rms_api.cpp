@@ -30,6 +30,7 @@ struct GeomDev {
     int32_t nby = 0, nbz = 0;     // brick grid (brick storage)
     int32_t storage = RMS_STORE_DENSE;
     int64_t device_bytes = 0;
+    bool wrapcells = false;       // cell words name the voxels across periodic sides (wrap_cells_rule)
     int64_t nvox = 0;
     bool has_walkable = false;
     uint32_t flags = 0;           // geometry_flags: boundary terms of the D derivative
@@ -314,6 +315,7 @@ static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int3
     gd.nvox = nvox;
     gd.has_walkable = walk;
     gd.flags = geometry_flags(g);
+    gd.wrapcells = wrap_cells_rule(g);
     // the labels and class map travel once; the kernels read only what is
     // built from them: a class per voxel (uint8) and a cell word (uint64)
     const size_t bytes = (size_t)nvox * g.label_bytes;
@@ -328,7 +330,8 @@ static rms_status add_geometry_impl(rms_ctx *ctx, const rms_geometry *geom, int3
         if (e == hipSuccess) e = hipMemcpy(d_map, g.label_class, (size_t)g.nlabel, hipMemcpyHostToDevice);
     }
     if (e == hipSuccess)
-        e = launch_build_cells(d_lab, g.label_bytes, d_map, g.nx, g.ny, g.nz, gd.d_cells, gd.d_cls, ctx->stream);
+        e = launch_build_cells(d_lab, g.label_bytes, d_map, g.nx, g.ny, g.nz, gd.wrapcells ? g.periodic : 0,
+                               gd.d_cells, gd.d_cls, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (d_lab) (void)hipFree(d_lab);
     if (d_map) (void)hipFree(d_map);
@@ -367,12 +370,14 @@ extern "C" rms_status rms_add_geometry(rms_ctx *ctx, const rms_geometry *geom, i
     return guard(ctx, [&] { return add_geometry_impl(ctx, geom, geom_id); });
 }
 
-extern "C" rms_status rms_geometry_info(rms_ctx *ctx, int32_t geom_id, int64_t *device_bytes, int32_t *storage)
+extern "C" rms_status rms_geometry_info(rms_ctx *ctx, int32_t geom_id, int64_t *device_bytes, int32_t *storage,
+                                        int32_t *wrap_cells)
 {
     if (!ctx) return RMS_ERR_INVALID_ARG;
     if (geom_id < 0 || geom_id >= (int32_t)ctx->geoms.size()) return fail(ctx, RMS_ERR_INVALID_ARG, "bad geom_id");
     if (device_bytes) *device_bytes = ctx->geoms[geom_id].device_bytes;
     if (storage) *storage = ctx->geoms[geom_id].storage;
+    if (wrap_cells) *wrap_cells = ctx->geoms[geom_id].wrapcells ? 1 : 0;
     return RMS_OK;
 }
 
@@ -496,6 +501,8 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         (void)derive(G.meta, P.dt, d);
         sig |= bits;
         if (G.d_btab) sig |= SIG_BRICKS;
+        if (G.wrapcells) sig |= SIG_WRAP;
+        D.wrapcells = G.wrapcells ? 1 : 0;
         D.cells = G.d_cells;
         D.cls = G.d_cls;
         D.btab = G.d_btab;
@@ -1224,6 +1231,7 @@ static rms_status run_noalloc_impl(rms_ctx *ctx, const rms_job *job, const rms_r
     if (job->nbvec > 0) sig |= SIG_NARROW;
     if (job->ngrad > 0) sig |= SIG_PGSE;
     if (G.d_btab) sig |= SIG_BRICKS;
+    if (G.wrapcells) sig |= SIG_WRAP;
     // PGSE lobe tables on the stack
     int32_t perm[NOALLOC_MAX_GRAD], trip[4 * (NOALLOC_MAX_TRIPLE + 1)];
     uint64_t keys[NOALLOC_MAX_GRAD];
@@ -1246,6 +1254,7 @@ static rms_status run_noalloc_impl(rms_ctx *ctx, const rms_job *job, const rms_r
     D.btab = G.d_btab;
     D.nby = G.nby;
     D.nbz = G.nbz;
+    D.wrapcells = G.wrapcells ? 1 : 0;
     D.batch0 = 0;
     D.wbase = 0;
     D.part = (double *)dp(L.part);

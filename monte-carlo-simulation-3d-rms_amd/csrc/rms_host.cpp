@@ -154,6 +154,27 @@ uint32_t geometry_flags(const rms_geometry &g)
     return f;
 }
 
+bool wrap_cells_rule(const rms_geometry &g)
+{
+    if ((g.periodic & 7) == 0) return false;
+    int64_t walk = 0, side = 0;
+    for (int32_t x = 0; x < g.nx; x++)
+        for (int32_t y = 0; y < g.ny; y++)
+            for (int32_t z = 0; z < g.nz; z++) {
+                const int64_t k = ((int64_t)x * g.ny + y) * g.nz + z;
+                const uint32_t lab = g.label_bytes == 1 ? (uint32_t)((const uint8_t *)g.labels)[k]
+                                                        : (uint32_t)((const uint16_t *)g.labels)[k];
+                const int32_t c = g.label_class ? (int32_t)g.label_class[lab] : (int32_t)lab;
+                if (c == 0) continue;
+                walk++;
+                const bool on = (((g.periodic >> 0) & 1) && (x == 0 || x == g.nx - 1)) ||
+                                (((g.periodic >> 1) & 1) && (y == 0 || y == g.ny - 1)) ||
+                                (((g.periodic >> 2) & 1) && (z == 0 || z == g.nz - 1));
+                side += on ? 1 : 0;
+            }
+    return 256 * side > walk;
+}
+
 bool kappa_closes_pair(const rms_geometry &g, const Derived &d)
 {
     for (int a = 1; a < g.nclass; a++)
@@ -421,6 +442,10 @@ const char *check_desc(const JobDesc &D, int sig)
     }
     if ((sig & SIG_PERIODIC) == 0 && D.periodic != 0) return "periodic job in a plan without the periodic variant";
     if ((sig & SIG_BRICKS) == 0 && D.btab != nullptr) return "brick-stored job in a plan without the brick variant";
+    // wrapped cell words let a straight move leave the lattice: only the
+    // SIG_WRAP variants bring the walker back in
+    if ((sig & SIG_WRAP) == 0 && D.wrapcells != 0) return "job with wrapped cell words in a plan without the wrap variant";
+    if (D.wrapcells != 0 && D.periodic == 0) return "wrapped cell words without a periodic axis";
     if (D.btab != nullptr && (D.nby < 1 || D.nbz < 1)) return "brick-stored job without its brick grid";
     if ((D.periodic & ~7) != 0) return "periodic: bits 0..2 (x, y, z) only";
     return nullptr;

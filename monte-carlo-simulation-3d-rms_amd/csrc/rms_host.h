@@ -15,7 +15,8 @@
 namespace rms {
 
 // Kernel variant bits (the SIG template argument of walk_kernel).
-enum : int { SIG_NARROW = 1, SIG_PGSE = 2, SIG_EXCH = 4, SIG_PERIODIC = 8, SIG_JUMP = 16, SIG_BRICKS = 32 };
+enum : int { SIG_NARROW = 1, SIG_PGSE = 2, SIG_EXCH = 4, SIG_PERIODIC = 8, SIG_JUMP = 16, SIG_BRICKS = 32,
+             SIG_WRAP = 64 };
 
 // The adjoint variant of a plan: the boundary terms of the D derivative
 // (exchange faces, lattice-edge mirrors, voxel edges) whenever no job has a
@@ -42,6 +43,15 @@ inline int adjoint_sig(int sig)
 // the pathwise D derivative is the whole derivative.
 enum : uint32_t { GEOM_EDGE_ORDER = 1u, GEOM_MIRROR = 2u };
 uint32_t geometry_flags(const rms_geometry &g);
+
+// How steps through a periodic side are walked.  True: the cell words of
+// voxels on the side name the voxels on the other side of the lattice and
+// the fast path takes those steps (kernel variant SIG_WRAP: three compares
+// per step); false: the words mark the side as outside the lattice and the
+// march takes them (no cost while walkers stay away from the sides).  True
+// when the lattice has a periodic axis and more than 1/256 of its non-null
+// voxels lie on a periodic side.
+bool wrap_cells_rule(const rms_geometry &g);
 
 // Steps must stay below one voxel by this margin: the straight move ends at
 // e = fma(s, v, o) with o in [0, 1], and e must round below 2 in FP32 and FP64.

@@ -11,9 +11,11 @@ hipError_t launch_walk(const LaunchArgs &a, int prec_f64, int sig, int grid, int
 hipError_t walk_occupancy(int prec_f64, int sig, int block, int *blocks_per_cu);
 hipError_t launch_reduce(const ReduceDesc *rd_dev, int32_t nred, int64_t max_nq, int32_t R, hipStream_t s);
 hipError_t launch_fill_zero(void *p, int64_t bytes, hipStream_t s);
-// Cell words and classes of a label lattice (see rms_types.h).
+// Cell words and classes of a label lattice (see rms_types.h); periodic
+// axes (bits 0..2) wrap the neighbourhood.
 hipError_t launch_build_cells(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
-                              int32_t ny, int32_t nz, uint64_t *cells, uint8_t *cls, hipStream_t s);
+                              int32_t ny, int32_t nz, int32_t periodic, uint64_t *cells, uint8_t *cls,
+                              hipStream_t s);
 // Brick storage of the cell words (2x2x4 bricks, nbrick = nbx*nby*nbz in
 // brick order): pass 1 counts the bricks to store per workgroup of 256
 // bricks (counts[ceil(nbrick/256)]); pass 2, given the exclusive prefix sums
@@ -38,6 +40,10 @@ const void *tangent_walk_fn(int prec_f64, int sig, int np);
 // The forward and tangent variants for plans with a brick-stored geometry
 // (sig bit 5, without bit 4) (rms_kernels_bricks.hip).
 const void *brick_walk_fn(int prec_f64, int sig, int np);
+// The forward and tangent variants for plans with a geometry whose cell
+// words are wrapped across periodic sides (sig bits 3 and 6, without bit 4)
+// (rms_kernels_wrap.hip).
+const void *wrap_walk_fn(int prec_f64, int sig, int np);
 
 // Allocation-free runs: device address of this GPU's scratch array (the
 // current device's copy), and the two table kernels that fill it.

@@ -45,9 +45,11 @@ __global__ void fill_zero_kernel(unsigned long long *p, int64_t n)
 
 // Cell words and classes of a label lattice: one thread per voxel reads its
 // 27-voxel neighbourhood and writes each neighbour's relation to the voxel's
-// class (see rms_types.h).
+// class (see rms_types.h).  Along a periodic axis the neighbour beyond a
+// lattice side is the voxel on the other side; along the others it is
+// outside (CELL_EDGE, a mirror).
 __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
-                                   int32_t ny, int32_t nz, uint64_t *cells, uint8_t *cls_out)
+                                   int32_t ny, int32_t nz, int32_t periodic, uint64_t *cells, uint8_t *cls_out)
 {
     const int64_t n = (int64_t)nx * ny * nz;
     for (int64_t lin = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; lin < n;
@@ -57,6 +59,9 @@ __global__ void build_cells_kernel(const void *labels, int32_t label_bytes, cons
         const int32_t y = (int32_t)(t % ny);
         const int32_t x = (int32_t)(t / ny);
         auto class_of = [&](int32_t xx, int32_t yy, int32_t zz) -> int32_t {
+            if (periodic & 1) xx = xx < 0 ? nx - 1 : (xx >= nx ? 0 : xx);
+            if (periodic & 2) yy = yy < 0 ? ny - 1 : (yy >= ny ? 0 : yy);
+            if (periodic & 4) zz = zz < 0 ? nz - 1 : (zz >= nz ? 0 : zz);
             if (xx < 0 || xx >= nx || yy < 0 || yy >= ny || zz < 0 || zz >= nz) return -1;
             const int64_t k = ((int64_t)xx * ny + yy) * nz + zz;
             const uint32_t lab = label_bytes == 1 ? (uint32_t)((const uint8_t *)labels)[k]
@@ -231,6 +236,7 @@ static const void *walk_fn(int prec_f64, int sig)
 static const void *walk_fn_np(int prec_f64, int sig, int np)
 {
     if ((sig & 16) && np > 0) return jump_walk_fn(prec_f64, sig, np);
+    if (sig & 64) return wrap_walk_fn(prec_f64, sig, np);
     if (sig & 32) return brick_walk_fn(prec_f64, sig, np);
     if (!(sig & 2) && np > 0) return tangent_walk_fn(prec_f64, sig, np);
     switch (np) {
@@ -283,14 +289,14 @@ hipError_t launch_reduce(const ReduceDesc *rd_dev, int32_t njobs, int64_t max_nq
 }
 
 hipError_t launch_build_cells(const void *labels, int32_t label_bytes, const uint8_t *lclass, int32_t nx,
-                              int32_t ny, int32_t nz, uint64_t *cells, uint8_t *cls, hipStream_t s)
+                              int32_t ny, int32_t nz, int32_t periodic, uint64_t *cells, uint8_t *cls, hipStream_t s)
 {
     const int64_t n = (int64_t)nx * ny * nz;
     int64_t grid = (n + 255) / 256;
     if (grid > 65536) grid = 65536;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(build_cells_kernel, dim3((unsigned)grid), dim3(256), 0, s, labels, label_bytes, lclass, nx, ny,
-                       nz, cells, cls);
+                       nz, periodic, cells, cls);
     return hipGetLastError();
 }
 

@@ -56,7 +56,7 @@ struct JobDesc {
     // half (65536: never, P = 1 or more than two non-null classes)
     int32_t xh12, xh21;
     int32_t periodic;          // bit k: axis k wraps around instead of mirroring
-    int32_t pad1;
+    int32_t wrapcells;         // the cell words name the voxel across a periodic side (SIG bit 6 walks it)
     double res;
     float step_f[NC], inv_step_f[NC], P_f[NC * NC];
     double step_d[NC], inv_step_d[NC], P_d[NC * NC];

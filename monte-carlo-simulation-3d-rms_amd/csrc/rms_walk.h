@@ -461,7 +461,7 @@ __device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLoca
 // and the straight move folded back at the reflecting faces.  Returns false
 // when the step needs the march (a crossing into another class, or three
 // comparisons that are not transitive: a near tie).
-template <typename R, int NP, bool EXCH, bool PER>
+template <typename R, int NP, bool EXCH, bool PER, bool WRAP>
 __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t cell,
                                           const V3<R> &v, uint64_t wid, int32_t i, uint32_t xhi, int32_t npg,
                                           Tan<R, NP> &T)
@@ -510,6 +510,13 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
         refl = (rels | (rels >> 1)) & 0x15u;
         ndraw = (om != 0u && hc >= 0) ? 1u : 0u;
     }
+    if constexpr (WRAP) {
+        // with wrapped cell words a straight move out of the lattice goes
+        // through a periodic side (elsewhere the relation is CELL_EDGE,
+        // already rare)
+        rare = rare || (uint32_t)(w.g.x + cx) >= (uint32_t)L.nx || (uint32_t)(w.g.y + cy) >= (uint32_t)L.ny ||
+               (uint32_t)(w.g.z + cz) >= (uint32_t)L.nz;
+    }
     if (rare) {
         // a near tie goes to the march.  A lattice edge is a mirror: the
         // walker reflects and its origin is mirrored (main.cu:246-265,
@@ -518,8 +525,10 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
         const uint32_t edge = rels & (rels >> 1) & 0x15u;  // CELL_EDGE slots
         const int32_t kz = 3 - kx - ky;
         if (PER && edge != 0u) {
-            // a periodic edge wraps around: the march crosses into the voxel
-            // on the other side (decided before any state changes)
+            // cell words that mark a periodic side as outside the lattice
+            // (wrap_cells_rule false): the march crosses into the voxel on
+            // the other side (decided before any state changes).  Wrapped
+            // cell words name that voxel instead and never reach here.
             const uint32_t eaxes = ((edge >> (2 * kx)) & 1u) | (((edge >> (2 * ky)) & 1u) << 1) |
                                    (((edge >> (2 * kz)) & 1u) << 2);
             if ((eaxes & (uint32_t)L.periodic) != 0u) return false;
@@ -553,6 +562,27 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
             nedge = (uint32_t)__builtin_popcount(edge);
         }
         refl = (rels | (rels >> 1)) & 0x15u;
+        if constexpr (WRAP) {
+            // a face crossed out of the lattice is on a periodic side: the
+            // cell word named the voxel on the other side, where the walker
+            // goes; voxel index and origin move by one lattice length (the
+            // march's rule) before the common update adds the crossing
+            const int32_t ks[3] = {kx, ky, kz};
+            for3([&](auto K) __attribute__((always_inline)) {
+                constexpr int k = decltype(K)::value;
+                const int32_t ck = k == 0 ? cx : (k == 1 ? cy : cz);
+                const int32_t Nk = k == 0 ? L.nx : (k == 1 ? L.ny : L.nz);
+                const bool crossed = ((refl >> (2 * ks[k])) & 1u) == 0u;
+                if (crossed && (uint32_t)(at<k>(w.g) + ck) >= (uint32_t)Nk) {
+                    const int32_t sh = ck < 0 ? Nk : -Nk;
+                    const double xo = origin(at<k>(w.gi), at<k>(w.oi));
+                    at<k>(w.g) = at<k>(w.g) + sh;
+                    at<k>(w.gi) = at<k>(w.gi) + sh;
+                    at<k>(w.ps) = mirror_ps(at<k>(w.ps), npg, xo, origin(at<k>(w.gi), at<k>(w.oi)));
+                    w.lin += sh * (k == 0 ? L.sx : (k == 1 ? L.nz : 1));
+                }
+            });
+        }
     }
     // every face met is now crossed (CELL_SAME, or axis not crossed) or
     // reflected (CELL_NULL, a reflecting exchange face, a mirror); the ranks
@@ -1351,7 +1381,8 @@ __device__ __forceinline__ void adj_record(const double *cot, int nclass, int nb
 // bit 1 PGSE signal, bit 2 exchange faces on the fast path, bit 3 periodic
 // lattice edges (the march wraps; without it every edge is a mirror), bit 4
 // (adjoint only, no PGSE) the jump of the D derivative at exchange faces,
-// bit 5 a geometry of the plan has its cell words in bricks.
+// bit 5 a geometry of the plan has its cell words in bricks, bit 6 (with
+// bit 3) a geometry's cell words name the voxels across periodic sides.
 // NP > 0 instantiates the adjoint pass: the same walk with tangents for D of
 // classes 1..NP, contracted with the cotangents of the record rows and PGSE
 // signal into per-batch parameter gradients.
@@ -1367,6 +1398,7 @@ void walk_kernel(LaunchArgs a)
     constexpr bool EXCH = (SIG & 4) != 0;
     constexpr bool PER = (SIG & 8) != 0;
     constexpr bool BR = (SIG & 32) != 0;
+    constexpr bool WRAP = PER && (SIG & 64) != 0;
     // sin/cos table of the FP32 direction draw, staged in LDS
     __shared__ float2 tab[256];
     // continuations of the kept boundary events (jump variants only)
@@ -1487,7 +1519,7 @@ void walk_kernel(LaunchArgs a)
                         uint64_t ca;
                         load_cont<R>(cont, m, tix, L, a, ca);
                         const int32_t l0 = a.lin;
-                        if (!fast_step<R, 0, EXCH, PER>(a, L, J, ca, v, wid, i, xhi, 0, T0))
+                        if (!fast_step<R, 0, EXCH, PER, WRAP>(a, L, J, ca, v, wid, i, xhi, 0, T0))
                             march<R, 0, EXCH, PER>(a, L, J, wid, i, v, xhi, 0, T0);
                         if (a.lin != l0) ca = cell_of<R, BR>(L, a);
                         store_cont<R>(cont, m, tix, a, ca);
@@ -1544,7 +1576,7 @@ void walk_kernel(LaunchArgs a)
             }
             const int32_t lin0 = w.lin;
             const int32_t npg = i - i0;  // PGSE terms since the last flush
-            if (!fast_step<R, NP, EXCH, PER>(w, L, J, cell, v, wid, i, xhi, npg, T))
+            if (!fast_step<R, NP, EXCH, PER, WRAP>(w, L, J, cell, v, wid, i, xhi, npg, T))
                 march<R, NP, EXCH, PER>(w, L, J, wid, i, v, xhi, npg, T);
             if (w.lin != lin0) cell = cell_of<R, BR>(L, w);  // only lanes that changed voxel read
             if (SIG & 2) {

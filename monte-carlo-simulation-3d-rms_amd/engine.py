@@ -170,12 +170,16 @@ class Engine:
         self._geom_ids[key] = (gid.value, geom)
         return gid.value
 
-    def geometry_info(self, geom: Geometry):
-        """(device bytes, "dense" or "bricks") of an uploaded geometry."""
+    def geometry_info(self, geom: Geometry, wrap_cells=False):
+        """(device bytes, "dense" or "bricks") of an uploaded geometry, and
+        with wrap_cells whether steps through its periodic sides take the
+        fast path (the rule of rms_geometry_info)."""
         gid = self.add_geometry(geom)
-        nb, st = C.c_int64(), C.c_int32()
-        self._check(capi.lib().rms_geometry_info(self._h, gid, C.byref(nb), C.byref(st)), "rms_geometry_info")
-        return int(nb.value), {1: "dense", 2: "bricks"}[st.value]
+        nb, st, wc = C.c_int64(), C.c_int32(), C.c_int32()
+        self._check(capi.lib().rms_geometry_info(self._h, gid, C.byref(nb), C.byref(st), C.byref(wc)),
+                    "rms_geometry_info")
+        out = (int(nb.value), {1: "dense", 2: "bricks"}[st.value])
+        return out + (bool(wc.value),) if wrap_cells else out
 
     def plan(self, jobs, dt, TN, timepoints=1000, precision="f32", replicas=32, chunk_steps=0, use_graph=False,
              block_threads=0, grid_blocks=0, xcd_split=0):

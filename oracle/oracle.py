@@ -33,7 +33,7 @@ class _Geom(C.Structure):
         ("T2", C.c_double * NC_MAX),
         ("kappa", C.c_double * (NC_MAX * NC_MAX)),
         ("periodic", C.c_int32),
-        ("reserved", C.c_int32),
+        ("wrap_cells", C.c_int32),
     ]
 
 
@@ -129,6 +129,8 @@ def lib():
         L.or_sincos_f64.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.or_cos_f64.argtypes = [C.c_double]
         L.or_cos_f64.restype = C.c_double
+        L.or_wrap_cells_rule.argtypes = [C.POINTER(_Geom)]
+        L.or_wrap_cells_rule.restype = C.c_int32
         L.or_derive.argtypes = [C.POINTER(_Geom), C.c_double, C.POINTER(_Derived)]
         L.or_derive.restype = C.c_int
         L.or_nval.argtypes = [C.c_int32, C.c_int32]
@@ -237,6 +239,7 @@ class Geometry:
         for i in range(NC_MAX * NC_MAX):
             g.kappa[i] = self.kappa.flat[i]
         g.periodic = self.periodic
+        g.wrap_cells = lib().or_wrap_cells_rule(C.byref(g))
         return g
 
 

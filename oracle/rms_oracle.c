@@ -704,6 +704,23 @@ int or_fast_step_f64(const or_geom *geom, const or_derived *d, int32_t g[3], dou
     return r;
 }
 
+int32_t or_wrap_cells_rule(const or_geom *g)
+{
+    if ((g->periodic & 7) == 0) return 0;
+    int64_t walk = 0, side = 0;
+    for (int32_t x = 0; x < g->nx; x++)
+        for (int32_t y = 0; y < g->ny; y++)
+            for (int32_t z = 0; z < g->nz; z++) {
+                if (or_class_at(g, x, y, z) == 0) continue;
+                walk++;
+                const int on = (((g->periodic >> 0) & 1) && (x == 0 || x == g->nx - 1)) ||
+                               (((g->periodic >> 1) & 1) && (y == 0 || y == g->ny - 1)) ||
+                               (((g->periodic >> 2) & 1) && (z == 0 || z == g->nz - 1));
+                side += on;
+            }
+    return 256 * side > walk ? 1 : 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Driver                                                                */
 /* ------------------------------------------------------------------ */

@@ -41,8 +41,15 @@ typedef struct {
     double T2[OR_NC_MAX];        /* ms, +inf = no relaxation */
     double kappa[OR_NC_MAX * OR_NC_MAX]; /* um/ms, < 0 = diffusivity-ratio rule */
     int32_t periodic;            /* bit k: axis k wraps around instead of mirroring */
-    int32_t reserved;
+    /* 1: steps through a periodic side take the fast path (the engine's cell
+     * words then name the voxel on the other side); 0: the march takes them.
+     * Set by or_wrap_cells_rule. */
+    int32_t wrap_cells;
 } or_geom;
+
+/* The engine's choice of wrap_cells: 1 when the lattice has a periodic axis
+ * and more than 1/256 of its non-null voxels lie on a periodic side. */
+int32_t or_wrap_cells_rule(const or_geom *g);
 
 typedef struct {
     int32_t nclass;

@@ -107,6 +107,34 @@ def test_auto_adjoint_central_differences_match_oracle(rms, orc, engine, prec, t
     assert np.all(np.isfinite(adj["grad"][1:3])) and np.all(adj["grad"][1:3] != 0.0)
 
 
+def test_central_differences_nan_where_a_perturbation_needs_another_variant(rms, engine):
+    """Three non-null classes with equal D under the reference rule
+    P = min(1, sqrt(Dj/Di)): every pair crosses (P = 1) and the plan runs the
+    variant without exchange draws.  Moving D_c by +-h makes some pairs draw,
+    which that variant cannot walk, so the central differences of D are NaN
+    while T2 stays finite; with two non-null classes the plan draws anyway
+    and the differences are finite."""
+    print("case central differences outside the plan's variant", flush=True)
+    lab = random_blobs((8, 9, 10), 4, seed=12)
+    g = rms.Geometry(lab, 0.1, [0.0, 1.5, 1.5, 1.5], T2=[np.inf, 60.0, 50.0, 40.0])
+    job = dict(geom=g, nwalk=500, seed=4, bval=np.array([1.0]), bvec=fib_dirs(1))
+    plan = engine.plan([job], 2e-4, 200, 10, precision="f32", replicas=4)
+    try:
+        a = plan.adjoint([np.ones((10, rms.nval(4, 1)))], None, want_replicas=True)[0]
+    finally:
+        plan.close()
+    assert np.all(np.isnan(a["grad"][1:4])) and np.all(np.isnan(a["grad_rep"][:, 1:4]))
+    assert np.all(np.isfinite(a["grad"][5:8]))
+    lab3 = random_blobs((8, 9, 10), 3, seed=12)
+    g3 = rms.Geometry(lab3, 0.1, [0.0, 1.5, 1.5], T2=[np.inf, 60.0, 50.0])
+    plan = engine.plan([dict(job, geom=g3)], 2e-4, 200, 10, precision="f32", replicas=4)
+    try:
+        a3 = plan.adjoint([np.ones((10, rms.nval(3, 1)))], None)[0]
+    finally:
+        plan.close()
+    assert np.all(np.isfinite(a3["grad"][1:3]))
+
+
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_auto_adjoint_pathwise_where_continuous(rms, orc, engine, prec):
     """Periodic impermeable planes (no exchange, no lattice-edge mirror, no

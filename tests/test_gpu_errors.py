@@ -35,6 +35,26 @@ def test_invalid_inputs_rejected(rms, engine):
     g = rms.Geometry(ones, 0.1, [0.0, 2.0])
     g.periodic = 8
     _err(rms, lambda: engine.run([dict(ok, geom=g)], 2e-4, 100, 10), "periodic")
+    g = rms.Geometry(ones, 0.1, [0.0, 2.0])
+    g.storage = "tiles"
+    with pytest.raises(KeyError):
+        g.to_c()
+    # adjoint options: an unknown D estimator, a step of the central
+    # differences outside (0, 0.5)
+    plan = engine.plan([ok], 2e-4, 100, 10)
+    try:
+        cot = [np.ones((10, rms.nval(2, 0)))]
+        _err(rms, lambda: plan.adjoint(cot, None, fd_step=0.7), "fd_rel_step")
+        _err(rms, lambda: plan.adjoint(cot, None, fd_step=-0.1), "fd_rel_step")
+        capi = __import__(rms.__name__ + "._capi", fromlist=["D_ESTIMATOR"])
+        capi.D_ESTIMATOR["bogus"] = 9
+        try:
+            _err(rms, lambda: plan.adjoint(cot, None, estimator="bogus"), "d_estimator")
+        finally:
+            del capi.D_ESTIMATOR["bogus"]
+        assert np.isfinite(plan.adjoint(cot, None)[0]["grad"][1])
+    finally:
+        plan.close()
     # the context still runs a valid job
     r = engine.run([ok], 2e-4, 100, 10)[0]
     assert r["NPar_count"].sum(1).tolist() == [64.0] * 10

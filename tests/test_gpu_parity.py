@@ -448,3 +448,40 @@ def test_autograd_many_geometries_equals_one_at_a_time(rms, engine):
         np.testing.assert_array_equal(T2.grad[k, 1:].numpy(), Tk.grad[1:].numpy())
         assert float(kap.grad[k]) == float(kk.grad)
         assert np.all(np.isfinite(D.grad[k, 1:].numpy())) and np.isfinite(float(kap.grad[k]))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_periodic_sides_both_encodings_bitexact(rms, orc, engine, prec):
+    """Steps through a periodic side are walked by the fast path when more
+    than 1/256 of the non-null voxels lie on a periodic side (wrapped cell
+    words), by the march otherwise: the library's choice equals the oracle's
+    rule, and both lattices are bit-exact against the oracle."""
+    print(f"case periodic sides, both encodings, {prec}", flush=True)
+    # a tube along z through a 20 x 20 x 300 lattice: 2 of 300 slices on the
+    # z sides (0.7 % > 1/256, wrapped); a 30 x 30 x 600 one with the tube
+    # ending one voxel short of the z sides and touching them through a
+    # narrow neck (under 1/256, marched)
+    def tube(n, nz, r, neck):
+        x, y = np.meshgrid(np.arange(n) - n / 2 + 0.5, np.arange(n) - n / 2 + 0.5, indexing="ij")
+        lab = np.zeros((n, n, nz), np.uint8)
+        lab[x ** 2 + y ** 2 <= r * r] = 1
+        if neck:
+            lab[:, :, 0] = 0
+            lab[:, :, -1] = 0
+            lab[n // 2, n // 2, 0] = lab[n // 2, n // 2, -1] = 1
+        return lab
+    for lab, want in ((tube(20, 300, 5.0, False), True), (tube(30, 600, 8.0, True), False)):
+        g = rms.Geometry(lab, 0.1, [0.0, 2.0], T2=[np.inf, 70.0], periodic="xyz")
+        og = orc.Geometry(lab, 0.1, [0.0, 2.0], T2=[np.inf, 70.0], periodic="xyz")
+        assert engine.geometry_info(g, wrap_cells=True)[2] is want
+        assert orc.lib().or_wrap_cells_rule(og._c()) == int(want)
+        job = dict(nwalk=3000, seed=5, bval=np.array([1.0]), bvec=fib_dirs(1),
+                   grad=pgse_table(fib_dirs(2), [1.0], [0.03], 0.01))
+        kw = dict(dt=2e-4, TN=4000, timepoints=20)
+        out = engine.run([dict(geom=g, **job)], precision=prec, replicas=4, final_state=True, **kw)[0]
+        ref = orc.run([dict(geom=og, **job)], kw["dt"], kw["TN"], kw["timepoints"], prec=prec, replicas=4,
+                      final_state=True)[0]
+        np.testing.assert_array_equal(out["rec"], ref["rec"])
+        np.testing.assert_array_equal(out["sig"], ref["sig"])
+        np.testing.assert_array_equal(out["fin_gi"], ref["fin_gi"])
+        assert np.any(np.abs(out["fin_gi"][2]) >= lab.shape[2] - 1) or not want  # walkers wrapped in z

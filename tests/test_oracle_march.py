@@ -225,16 +225,17 @@ def test_fast_path_equals_march(orc):
 
 
 def test_fast_path_equals_march_periodic(orc):
-    """Periodic x and z, mirror y: the fast path hands every step that meets a
-    periodic lattice side to the march (which wraps the voxel index and moves
-    the origin by one lattice length) and otherwise equals the march: same end
-    voxel, counts and origin on random states, same position up to rounding."""
+    """Periodic x and z, mirror y: a face on a periodic lattice side leads to
+    the voxel on the other side (the voxel index wraps, the origin moves by
+    one lattice length), on the fast path as in the march; the fast path
+    equals the march on random states, wrapped steps included: same end
+    voxel, counts and origin, same position up to rounding."""
     rng = np.random.default_rng(12)
     lab = rng.integers(0, 3, (5, 6, 4)).astype(np.uint8)
     lab[rng.random(lab.shape) < 0.5] = 1
     g = orc.Geometry(lab, 0.1, [0.0, 2.0, 2.0], T2=[np.inf] * 3, kappa=0.0, periodic="xz")
     dt = 2e-4
-    applied = wrapped = 0
+    applied = wrapped = fast_wrapped = 0
     for _ in range(4000):
         while True:
             gv = rng.integers(0, [5, 6, 4])
@@ -254,12 +255,13 @@ def test_fast_path_equals_march_periodic(orc):
         if f is None:
             continue
         applied += 1
+        fast_wrapped += int(f["gi"][0] != gi[0] or f["gi"][2] != gi[2])
         np.testing.assert_array_equal(f["g"], m["g"])
         np.testing.assert_allclose(f["o"], m["o"], atol=1e-12)
         np.testing.assert_array_equal(f["cnt"][:3], m["cnt"][:3])
         np.testing.assert_array_equal(f["gi"], m["gi"])
         np.testing.assert_array_equal(f["oi"], m["oi"])
-    assert applied > 0.3 * 4000 and wrapped > 100
+    assert applied > 0.3 * 4000 and wrapped > 100 and fast_wrapped > 0.5 * wrapped
 
 
 def test_fast_path_three_faces_in_order(orc):

@@ -232,3 +232,30 @@ def test_c4_gradient_against_central_differences(rms, engine):
             checks.append((name + " boundary", ok_b, g_b, se_b, g_fd, se_fd))
     np.testing.assert_array_equal(bnd["grad"][4:9], adj["grad"][4:9])
     assert all(c[1] for c in checks), checks
+
+
+def test_central_differences_in_chunked_graph_plans(rms, engine):
+    """The central differences walk the plan's own launch sequence (chunks
+    here) at D (1 +- h) and restore its descriptors: the D entries equal
+    those of a single-launch plan, and execute() afterwards (a captured
+    graph) still returns the unperturbed records."""
+    print("case central differences in a chunked graph plan", flush=True)
+    lab = random_blobs((10, 9, 11), 3, seed=19)
+    g = rms.Geometry(lab, 0.1, [0.0, 1.8, 0.9], T2=[np.inf, 60.0, 40.0], kappa=0.15)
+    job = dict(geom=g, nwalk=900, seed=31, bval=np.array([0.5, 2.0]), bvec=fib_dirs(2))
+    dt, TN, T = 2e-4, 480, 12
+    cot = [np.random.default_rng(2).standard_normal((T, rms.nval(3, 2)))]
+    one = engine.plan([job], dt, TN, T, precision="f32", replicas=4)
+    ch = engine.plan([job], dt, TN, T, precision="f32", replicas=4, chunk_steps=100, use_graph=True)
+    try:
+        r0 = ch.execute()
+        a1 = one.adjoint(cot, None, want_replicas=True)[0]
+        a2 = ch.adjoint(cot, None, want_replicas=True)[0]
+        r1 = ch.execute()
+    finally:
+        one.close()
+        ch.close()
+    np.testing.assert_array_equal(a1["grad"], a2["grad"])
+    np.testing.assert_array_equal(a1["grad_rep"], a2["grad_rep"])
+    np.testing.assert_array_equal(r0[0]["rec"], r1[0]["rec"])
+    assert np.all(np.isfinite(a2["grad"][1:3]))

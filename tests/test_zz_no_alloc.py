@@ -41,6 +41,13 @@ def test_noalloc_equals_run_twice_second_smaller(rms, engine):
     a = engine.run_noalloc(plain, 2e-4, 100, 10, precision="f32", replicas=4)
     b = engine.run([plain], 2e-4, 100, 10, precision="f32", replicas=4)[0]
     _same(a, b)
+    print("case noalloc call 4: periodic lattice with wrapped cell words, 700 walkers, f32", flush=True)
+    gp = rms.Geometry(lab, 0.1, [0.0, 2.0, 1.1], T2=[np.inf, 70.0, 35.0], kappa=0.05, periodic="xyz")
+    assert engine.geometry_info(gp, wrap_cells=True)[2]
+    per = dict(geom=gp, nwalk=700, seed=9, bval=bval, bvec=bvec)
+    a = engine.run_noalloc(per, 2e-4, 400, 20, precision="f32", replicas=4)
+    b = engine.run([per], 2e-4, 400, 20, precision="f32", replicas=4)[0]
+    _same(a, b)
 
 
 def test_noalloc_refuses_runs_beyond_the_scratch(rms, engine):

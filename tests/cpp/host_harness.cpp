@@ -232,6 +232,94 @@ static void kappa_rules()
     check(kappa_closes_pair(g, d), "kappa_tiny_closes", "P < 1e-6 from a tiny kappa not flagged");
 }
 
+// The periodic-side rule: wrapped cell words when more than 1/256 of the
+// non-null voxels lie on a periodic side.  A column of non-null voxels along
+// z in a 3 x 3 x nz lattice, periodic in z only, has 2 side voxels out of nz:
+// exactly 1/256 at nz = 512 (not wrapped), above it at nz = 511.
+static void wrap_rule()
+{
+    auto column = [](int32_t nz, int32_t periodic, std::vector<uint8_t> &lab) {
+        lab.assign((size_t)9 * nz, 0);
+        for (int32_t z = 0; z < nz; z++) lab[(size_t)(1 * 3 + 1) * nz + z] = 1;
+        rms_geometry g;
+        std::memset(&g, 0, sizeof(g));
+        g.nx = g.ny = 3;
+        g.nz = nz;
+        g.label_bytes = 1;
+        g.labels = lab.data();
+        g.nclass = 2;
+        g.periodic = periodic;
+        return g;
+    };
+    std::vector<uint8_t> lab;
+    check(!wrap_cells_rule(column(512, 4, lab)), "wrap rule: 2 of 512 voxels on the sides is not above 1/256", "");
+    check(wrap_cells_rule(column(511, 4, lab)), "wrap rule: 2 of 511 voxels on the sides is above 1/256", "");
+    check(!wrap_cells_rule(column(16, 0, lab)), "wrap rule: no periodic axis", "");
+    // the column touches no x or y side: periodic x, y only counts nothing
+    check(!wrap_cells_rule(column(16, 3, lab)), "wrap rule: periodic axes the walkable voxels do not touch", "");
+}
+
+// Physics fields rebuilt at a perturbed D (the central differences of the
+// adjoint): equal to a descriptor built from scratch with that D, and the
+// variant bits they need follow the exchange categories (three non-null
+// classes with equal D cross freely, P = 1, and need no draws; moving one D
+// makes draws, which need the exchange variant).
+static void perturbed_physics()
+{
+    rms_geometry g = geom(4, 1.5, 1.5, -1.0);
+    g.D[3] = 1.5;
+    rms_job J;
+    std::memset(&J, 0, sizeof(J));
+    J.nwalk = 64;
+    const rms_run_params P = params(2e-4, 100, 10, 1);
+    JobDesc D0, D1;
+    int b0 = 0, b1 = 0;
+    bool e0 = false, e1 = false;
+    check(fill_job(g, J, P, D0, b0, e0) == nullptr && (b0 & SIG_EXCH) == 0 && e0,
+          "equal D: walkers cross between classes without draws", "");
+    rms_geometry gp = g;
+    gp.D[2] = 1.5 * 1.05;
+    JobDesc Dp = D0;
+    check(job_physics(gp, P.dt, Dp, b1, e1) == nullptr && (b1 & SIG_EXCH) != 0, "moved D: draws need the exchange variant",
+          "");
+    check(fill_job(gp, J, P, D1, b0, e0) == nullptr && std::memcmp(D1.step_d, Dp.step_d, sizeof(Dp.step_d)) == 0 &&
+              std::memcmp(D1.P_d, Dp.P_d, sizeof(Dp.P_d)) == 0 && D1.pcat == Dp.pcat &&
+              std::memcmp(D1.dlogP, Dp.dlogP, sizeof(Dp.dlogP)) == 0 && D1.xh12 == Dp.xh12,
+          "physics rebuilt at a perturbed D equal a fresh descriptor", "");
+    gp.D[2] = 1e6;
+    check(job_physics(gp, P.dt, Dp, b1, e1) != nullptr, "a perturbed step of a voxel or more is refused", "");
+}
+
+// Descriptors whose cell words only one kernel variant can walk.
+static void storage_variants()
+{
+    JobDesc D;
+    std::memset(&D, 0, sizeof(D));
+    uint64_t cells = 0;
+    uint8_t cls = 1;
+    double part = 0.0;
+    unsigned long long cnt = 0;
+    D.cells = &cells;
+    D.cls = &cls;
+    D.part = &part;
+    D.counters = &cnt;
+    D.nclass = 2;
+    D.nwalk = 1;
+    check(check_desc(D, 0) == nullptr, "dense descriptor in the dense variant", "");
+    uint32_t btab = 0;
+    D.btab = &btab;
+    D.nby = D.nbz = 1;
+    check(check_desc(D, 0) != nullptr, "brick descriptor refused by the dense variant", "");
+    check(check_desc(D, SIG_BRICKS) == nullptr, "brick descriptor in the brick variant", "");
+    D.btab = nullptr;
+    D.wrapcells = 1;
+    D.periodic = 4;
+    check(check_desc(D, SIG_PERIODIC) != nullptr, "wrapped cell words refused without the wrap variant", "");
+    check(check_desc(D, SIG_PERIODIC | SIG_WRAP) == nullptr, "wrapped cell words in the wrap variant", "");
+    D.periodic = 0;
+    check(check_desc(D, SIG_PERIODIC | SIG_WRAP) != nullptr, "wrapped cell words without a periodic axis", "");
+}
+
 int main()
 {
     pgse_tables_vs_masks();
@@ -239,6 +327,9 @@ int main()
     step_margin();
     walker_base_rule();
     kappa_rules();
+    wrap_rule();
+    perturbed_physics();
+    storage_variants();
     std::printf("%d failures\n", nfail);
     return nfail;
 }

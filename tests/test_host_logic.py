@@ -2,8 +2,9 @@
 with g++ without HIP): PGSE lobe tables against the literal per-step
 comparisons of main_PGSE.cu:341-345, the table checks every kernel variant
 relies on (sentinels for jobs without a table), the step margin that keeps
-the FP32 straight move inside two voxels, the walker-id base of shards and
-the kappa rules of the adjoint."""
+the FP32 straight move inside two voxels, the walker-id base of shards,
+the kappa rules of the adjoint, the periodic-side rule, the physics rebuilt
+at a perturbed D, and the descriptors only one storage variant can walk."""
 import os
 import subprocess
 
@@ -20,4 +21,21 @@ def test_host_descriptor_builder(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
-    assert r.stdout.count("ok ") >= 19
+    assert r.stdout.count("ok ") >= 33
+
+
+def test_oracle_wrap_rule_matches_the_library_rule(orc):
+    """The oracle's or_wrap_cells_rule on the lattices the host harness pins
+    the library's rule with (a column along z of a 3 x 3 x nz lattice):
+    2 of 512 side voxels is not above 1/256, 2 of 511 is."""
+    import numpy as np
+
+    def column(nz, periodic):
+        lab = np.zeros((3, 3, nz), np.uint8)
+        lab[1, 1, :] = 1
+        return orc.Geometry(lab, 0.1, [0.0, 2.0], periodic=periodic)
+
+    assert orc.lib().or_wrap_cells_rule(column(512, "z")._c()) == 0
+    assert orc.lib().or_wrap_cells_rule(column(511, "z")._c()) == 1
+    assert orc.lib().or_wrap_cells_rule(column(16, "")._c()) == 0
+    assert orc.lib().or_wrap_cells_rule(column(16, "xy")._c()) == 0

@@ -12,13 +12,16 @@ has batches to run, so the samples cut walkers, never steps):
   c2  the single axon with its 360-row PGSE table: 1e4 walkers x 1e5 steps
   c3  all 256 bundle geometries, 0.1 % of their walkers (>= 64 each), 5e4 steps
   c4  the axon with extra-axonal space: 1 % of the walkers (1e4) x 5e4 steps,
-      forward pass plus adjoint pass (rate = walkers x steps / both passes)
+      forward pass plus the engine's default adjoint: the tangent pass and
+      the central differences of D (two forward passes per non-null class at
+      D (1 +- 0.05)) (rate = walkers x steps / all passes)
   c5  64 of the 4096 sweep geometries, 160 walkers each, 2e4 steps, forward
-      plus adjoint
+      plus adjoint as for c4
 """
 from __future__ import annotations
 
 import argparse
+import copy
 import json
 import os
 import sys
@@ -76,11 +79,25 @@ def main():
         return r, time.perf_counter() - t0
 
     def fwd_adj(jobs, TN, T):
+        # the forward pass, the tangent pass, and the forward passes of the
+        # central differences of D, as the engine's default adjoint runs them
         r, t1 = timed(jobs, TN, T)
         TD = A.diff_times(W.DT, TN, T)
         cots = [W.c4_cotangent(A, x["rec"], x["_nclass"], TD, x["_nbvec"]) for x in r]
-        _, t2 = timed(jobs, TN, T, cot_rec=cots)
-        return t1 + t2
+        _, t2 = timed(jobs, TN, T, cot_rec=cots, boundary=False)
+        t3 = 0.0
+        for c in range(1, max(j["geom"].nclass for j in jobs)):
+            for f in (1.05, 0.95):
+                pj = []
+                for j in jobs:
+                    g = copy.copy(j["geom"])
+                    g.D = list(g.D)
+                    if c < g.nclass:
+                        g.D[c] = g.D[c] * f
+                    pj.append(dict(j, geom=g))
+                _, t = timed(pj, TN, T)
+                t3 += t
+        return t1 + t2 + t3
 
     if "c1" in todo:
         c = W.C1
@@ -110,7 +127,8 @@ def main():
         t = fwd_adj(W.c4_jobs(orc.Geometry, n), c["TN"], c["timepoints"])
         ws = n * c["TN"]
         out["c4"] = dict(value=ws / t, seconds=t, walker_steps=ws,
-                         sample=f"{n} walkers (1 %) x {c['TN']} steps, forward + adjoint")
+                         sample=f"{n} walkers (1 %) x {c['TN']} steps, forward + adjoint (tangent pass + "
+                                "4 forward passes of the central differences)")
     if "c5" in todo:
         c = W.C5
         jobs = W.c5_jobs(orc.Geometry, range(64), 160)

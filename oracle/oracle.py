@@ -71,6 +71,8 @@ class _Params(C.Structure):
         ("replicas", C.c_int32),
         ("pgse_bruteforce", C.c_int32),
         ("nthreads", C.c_int32),
+        ("no_boundary", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 
@@ -317,12 +319,14 @@ def nval(nclass, nbvec):
 
 
 def run(jobs, dt, TN, timepoints, prec="f32", replicas=32, pgse_bruteforce=False, nthreads=0,
-        final_state=False, cot_rec=None, cot_sig=None):
+        final_state=False, cot_rec=None, cot_sig=None, boundary=True):
     """Run a list of jobs; each job is a dict(geom, nwalk, seed, bval, bvec, grad).
 
     Returns per job a dict with the record totals split into the
     reference's output arrays (sig0, NPar_count, dx2, dx4, sigRe), the PGSE
-    signal, replica sums, counters and optionally the final walker state."""
+    signal, replica sums, counters and optionally the final walker state.
+    With cotangents, the adjoint pass; boundary=False leaves out the
+    boundary terms of the D derivative (the tangent pass alone)."""
     L = lib()
     n = len(jobs)
     cj = (_Job * n)()
@@ -377,7 +381,7 @@ def run(jobs, dt, TN, timepoints, prec="f32", replicas=32, pgse_bruteforce=False
         r["_nbvec"] = nb
         res.append(r)
     p = _Params(dt=float(dt), TN=int(TN), timepoints=int(timepoints), prec=PREC[prec], replicas=max(1, replicas),
-                pgse_bruteforce=int(bool(pgse_bruteforce)), nthreads=int(nthreads))
+                pgse_bruteforce=int(bool(pgse_bruteforce)), nthreads=int(nthreads), no_boundary=int(not boundary))
     if cot_rec is None and cot_sig is None:
         if not L.or_run(cj, n, C.byref(p), co):
             raise RuntimeError("oracle: " + L.or_last_error().decode())

@@ -800,7 +800,7 @@ static int or_job_setup(const or_job *J, const or_params *p, const or_adj *adj, 
         /* boundary terms of the D derivative (exchange faces, lattice-edge
          * mirrors, voxel edges) when no job of the run has a PGSE table (the
          * continuations carry no PGSE prefix sums) */
-        aux->jump = !plan_pgse;
+        aux->jump = !plan_pgse && !p->no_boundary;
     }
     aux->nval = or_nval(J->geom->nclass, J->nbvec);
     aux->Tstep = p->TN / p->timepoints;

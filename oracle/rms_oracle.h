@@ -80,6 +80,11 @@ typedef struct {
     int32_t replicas;            /* batch-means replicas (>= 1) */
     int32_t pgse_bruteforce;     /* 1: per-gradient per-step loop (slow) */
     int32_t nthreads;            /* OpenMP threads, 0 = default */
+    /* adjoint without a PGSE table: 0 = pathwise plus the boundary terms of
+     * D (the default), 1 = the tangent pass alone (pathwise D and the score
+     * of the draws, T2, kappa) */
+    int32_t no_boundary;
+    int32_t reserved;
 } or_params;
 
 typedef struct {

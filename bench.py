@@ -396,6 +396,9 @@ def form_gradient(rms, eng, args, A):
         out, adj = once(k == nrun - 1)
         fwd_ms.append(plan.timing["walk_ms"])
         adj_ms.append(plan.adj_timing["walk_ms"])
+    # walker-steps of the central differences' walks (the adjoint call's
+    # total less its tangent pass)
+    fd_ws = plan.adj_timing["walker_steps"] - float(Wn) * TN
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     # the same cotangent through the boundary-term estimator of D (pathwise
@@ -422,15 +425,19 @@ def form_gradient(rms, eng, args, A):
     rate = float(Wn) * TN * nrun / el
     s = np.array([math.sqrt(6 * dt * 2.0) / W.RES, math.sqrt(6 * dt * 1.5) / W.RES])
     # per walker-step of the problem: the forward walk, the tangent pass (the
-    # walk plus ~40 lane-ops of tangents) and four forward walks at D (1 +- h)
-    ops_all = 6 * ops_per_step(float(s.mean()), pgse=False) + 40
+    # walk plus ~40 lane-ops of tangents) and the forward walks of the central
+    # differences at D (1 +- h), on the walkers whose walks can change with D_c
+    ops1 = ops_per_step(float(s.mean()), pgse=False)
+    ops_all = 2 * ops1 + 40 + fd_ws / (float(Wn) * TN) * ops1
     kern = float(Wn) * TN / ((np.mean(fwd_ms) + np.mean(adj_ms)) / 1e3)
     rel = float(ses[0] / est[0])
     return {
         "walker_steps_per_s": rate, "kernel_walker_steps_per_s": kern,
         "forward_kernel_ms": float(np.mean(fwd_ms)), "adjoint_kernel_ms": float(np.mean(adj_ms)),
         "adjoint": "tangent pass (T2 analytic, kappa score, D pathwise) + central differences of D with common "
-                   "random numbers at D (1 +- 0.05): 4 forward walks",
+                   "random numbers at D (1 +- 0.05): 4 forward walks of the walkers whose walks can change with D_c",
+        "central_difference_walker_steps": fd_ws,
+        "central_difference_fraction_of_4_walks": fd_ws / (4.0 * Wn * TN),
         "roofline_frac_estimate": kern * ops_all / 1e12 / VALU_PEAK_TOPS,
         "geometries": 1, "walkers_per_geometry": Wn, "TN": TN, "records": T, "bvecs": nb, "kappa": c["kappa"],
         "objective": "sum_t [D_t(z) + K_t(z) + sum_j S_j(t)], narrow-pulse S = sigRe/sig0",
@@ -475,6 +482,7 @@ def form_sweep(rms, eng, args, A):
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     fwd_ms, adj_ms = plan.timing["walk_ms"], plan.adj_timing["walk_ms"]
+    fd_ws = plan.adj_timing["walker_steps"] - float(nwg) * ng * TN
     plan.close()
     dk = np.array([a["grad"][8] for a in adj])
     dT2 = np.array([a["grad"][5] for a in adj])
@@ -494,7 +502,9 @@ def form_sweep(rms, eng, args, A):
         "D_end_median": float(np.median(Dend)), "D_end_range": [float(np.min(Dend)), float(np.max(Dend))],
         "dobj_dkappa_median": float(np.nanmedian(dk)), "dobj_dT2_intra_median": float(np.median(dT2)),
         "dobj_dD_intra_median": float(np.nanmedian(dD)),
-        "adjoint": "tangent pass + central differences of D with common random numbers (4 forward walks)",
+        "adjoint": "tangent pass + central differences of D with common random numbers (4 forward walks of the "
+                   "walkers whose walks can change with D_c)",
+        "central_difference_fraction_of_4_walks": fd_ws / (4.0 * nwg * ng * TN),
         "D_end_rel_se_median": float(np.median(rel)),
         "seconds_per_geometry_at_1pct_se": _se_cost(rate, nwg, TN, float(np.median(rel))),
     }

@@ -140,6 +140,7 @@ typedef struct {
     int32_t block;               /* threads per workgroup */
     int32_t launches;            /* walk launches */
     int32_t reserved;
+    int64_t walker_steps;        /* walker-steps of every walk launch (walkers x steps) */
 } rms_timing;
 
 /* Adjoint outputs: derivatives of sum(cot_rec * rec) + sum(cot_sig * sig)
@@ -165,7 +166,11 @@ enum {
     RMS_D_AUTO = 0,
     /* central differences with common random numbers for every job: two
      * forward walks per non-null class at D_c (1 +- h), same seeds, the
-     * objective contracted per replica; O(h^2) bias, any table */
+     * objective contracted per replica; O(h^2) bias, any table.  The walks
+     * for class c take only the walkers whose walk can change with D_c: those
+     * the tangent pass saw in class c or drawing at an exchange face whose P
+     * depends on D_c (every other walker's walk is the same at both points,
+     * so it adds nothing to the difference); fd_all_walkers walks them all. */
     RMS_D_CRN_FD = 1,
     /* pathwise plus the boundary terms of the jumps (smoothed perturbation
      * analysis): each boundary event is the derivative of its probability
@@ -182,7 +187,7 @@ enum {
 
 typedef struct {
     int32_t d_estimator;         /* RMS_D_* */
-    int32_t reserved;
+    int32_t fd_all_walkers;      /* 1 = central differences walk every walker */
     double fd_rel_step;          /* h of RMS_D_AUTO / RMS_D_CRN_FD, 0 = 0.05 */
 } rms_adjoint_opts;
 

@@ -112,6 +112,7 @@ class Timing(C.Structure):
         ("block", C.c_int32),
         ("launches", C.c_int32),
         ("reserved", C.c_int32),
+        ("walker_steps", C.c_int64),
     ]
 
 
@@ -127,7 +128,7 @@ class JobAdj(C.Structure):
 class AdjointOpts(C.Structure):
     _fields_ = [
         ("d_estimator", C.c_int32),
-        ("reserved", C.c_int32),
+        ("fd_all_walkers", C.c_int32),
         ("fd_rel_step", C.c_double),
     ]
 

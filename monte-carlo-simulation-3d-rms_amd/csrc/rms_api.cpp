@@ -102,6 +102,12 @@ struct rms_plan {
     double *d_tsnap = nullptr;
     double *d_adj_res = nullptr;  // [njobs][NPARAM] totals, then [njobs][R][NPARAM]
     double *h_adj_res = nullptr;
+    // central differences on walker subsets: the tangent pass's per-walker
+    // marks, the walker lists, and the subset launch's batch table and
+    // reductions (allocated by the first adjoint call that needs them)
+    uint32_t *d_wvis = nullptr, *d_wlist = nullptr;
+    int32_t *d_bj_fd = nullptr;
+    ReduceDesc *d_red_fd = nullptr;
 };
 
 static const char *kVersion = "rms_amd 0.1 (gfx950)";
@@ -840,6 +846,7 @@ static rms_status plan_execute_impl(rms_plan *p, rms_job_out *outs, rms_timing *
         const int64_t TN = p->prm.TN;
         const int64_t ch = (p->prm.chunk_steps > 0 && p->prm.chunk_steps < TN) ? p->prm.chunk_steps : TN;
         timing->launches = (int32_t)((TN + ch - 1) / ch);
+        for (const JobHost &H : p->jobs) timing->walker_steps += H.nwalk * TN;
     }
     const unsigned long long *cnt = (const unsigned long long *)(p->h_res) + (p->d_counters - (unsigned long long *)p->d_res);
     if (rms_status st = init_failures(ctx, cnt, nj)) return st;
@@ -963,15 +970,120 @@ static double contract(const rms_job_adj &A, const double *rec, int64_t nq, cons
     return f;
 }
 
+// The plan's launch layout, put back by the destructor whatever happened
+// while the central differences walked their own.
+namespace {
+struct LayoutGuard {
+    rms_plan *p;
+    int64_t nbatch;
+    int32_t *bj;
+    ReduceDesc *d_red;
+    std::vector<ReduceDesc> red;
+    int grid;
+    explicit LayoutGuard(rms_plan *q)
+        : p(q), nbatch(q->nbatch), bj(q->d_batch_job), d_red(q->d_red), red(q->red), grid(q->grid)
+    {
+    }
+    ~LayoutGuard()
+    {
+        p->nbatch = nbatch;
+        p->d_batch_job = bj;
+        p->d_red = d_red;
+        p->red = red;
+        p->grid = grid;
+    }
+};
+}  // namespace
+
+// Launch layout of the walks for class c on walker subsets: job j (of those
+// in `which` with class c) walks the walkers whose tangent-pass marks meet
+// fd_walker_mask, in the order of their index, packed into whole batches
+// from batch 0; jobs with none are left out (inc[j] = 0, zero[j] = 1: their
+// difference is exactly zero), and a job whose mask cannot be trusted walks
+// every walker.  Descriptors go to `hj`, the batch table, walker lists and
+// reductions to the plan's subset buffers, and the plan points at them, with
+// its own grid (grid0) cut to the batches.
+static rms_status fd_subset_layout(rms_plan *p, const std::vector<char> &which, int c, double h,
+                                   const std::vector<uint32_t> &vis, int grid0, std::vector<JobDesc> &hj,
+                                   std::vector<char> &inc, std::vector<char> &zero)
+{
+    rms_ctx *ctx = p->ctx;
+    hipStream_t s = ctx->stream;
+    const int nj = (int)p->jobs.size();
+    hj = p->hjobs;
+    inc.assign(nj, 0);
+    zero.assign(nj, 0);
+    std::vector<uint32_t> list;
+    std::vector<int32_t> bj;
+    std::vector<ReduceDesc> rd;
+    int64_t nb = 0;
+    size_t ri = 0;  // the plan's reductions: per job the records, then the PGSE signal
+    for (int j = 0; j < nj; j++) {
+        const JobHost &H = p->jobs[j];
+        const size_t r0 = ri;
+        ri += H.ngrad > 0 ? 2 : 1;
+        if (!which[j] || p->hjobs[j].nclass <= c) continue;
+        uint32_t mask = 0;
+        const bool all = !fd_walker_mask(ctx->geoms[H.geom_id].meta, p->prm.dt, c, h, mask);
+        const int64_t at = nb * WAVE;
+        int64_t m = H.nwalk;
+        if (!all) {
+            list.resize((size_t)at);
+            for (int64_t k = 0; k < H.nwalk; k++)
+                if ((vis[(size_t)(H.wbase + k)] & mask) != 0u) list.push_back((uint32_t)k);
+            m = (int64_t)list.size() - at;
+        }
+        if (m == 0) {
+            zero[j] = 1;
+            continue;
+        }
+        const int64_t nbj = (m + WAVE - 1) / WAVE;
+        // padding lanes walk the last listed walker again (inactive: no records)
+        list.resize((size_t)(at + nbj * WAVE), all ? 0u : list.back());
+        JobDesc &D = hj[j];
+        D.nwalk = m;
+        D.batch0 = nb;
+        D.wbase = at;
+        D.wlist = all ? nullptr : p->d_wlist + at;
+        for (int64_t b = 0; b < nbj; b++) bj.push_back(j);
+        for (size_t r = r0; r < ri; r++) {
+            ReduceDesc e = p->red[r];
+            e.nb = nbj;
+            rd.push_back(e);
+        }
+        inc[j] = 1;
+        nb += nbj;
+    }
+    if (nb == 0) return RMS_OK;
+    HIPCHK(ctx, hipMemcpyAsync(p->d_wlist, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(p->d_bj_fd, bj.data(), sizeof(int32_t) * bj.size(), hipMemcpyHostToDevice, s));
+    HIPCHK(ctx, hipMemcpyAsync(p->d_red_fd, rd.data(), sizeof(ReduceDesc) * rd.size(), hipMemcpyHostToDevice, s));
+    // the host vectors die here: the copies must have read them
+    HIPCHK(ctx, hipStreamSynchronize(s));
+    p->nbatch = nb;
+    p->d_batch_job = p->d_bj_fd;
+    p->d_red = p->d_red_fd;
+    p->red = rd;
+    // the plan's grid, cut to the batches (a multiple of 8 when split per XCD)
+    const int wpb = p->block / WAVE;
+    int64_t need = (nb + wpb - 1) / wpb;
+    if (p->xcd_split) need = (need + 7) / 8 * 8;
+    p->grid = (int)std::min<int64_t>(grid0, need);
+    return RMS_OK;
+}
+
 // D entries by central differences with common random numbers: for each
-// non-null class c, the forward walk of the whole plan with D_c of the jobs
-// in `which` set to D_c (1 + h) and D_c (1 - h) (the other jobs and every
-// seed unchanged), each job's objective contracted per replica, and
-// dF/dD_c = (F+ - F-) / (D+ - D-).  A perturbation that leaves the plan's
-// kernel variant (a new exchange pair) or the step limit gives NaN.  The
-// plan's own descriptors are restored on the device before returning.
+// non-null class c, the forward walk of the jobs in `which` with D_c set to
+// D_c (1 + h) and D_c (1 - h) (every seed unchanged), each job's objective
+// contracted per replica, and dF/dD_c = (F+ - F-) / (D+ - D-).  With
+// `subsets` the walks for class c take only the walkers whose walk can
+// change with D_c (fd_walker_mask on the tangent pass's marks): the others
+// walk the same at both points and add nothing to F+ - F-.  Without it the
+// whole plan walks.  A perturbation that leaves the plan's kernel variant (a
+// new exchange pair) or the step limit gives NaN.  The plan's own
+// descriptors and layout are restored before returning.
 static rms_status fd_d_entries(rms_plan *p, const rms_job_adj *adj, const std::vector<char> &which, double h,
-                               double *walk_ms, int *launches)
+                               bool subsets, double *walk_ms, int *launches, int64_t *walked)
 {
     rms_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
@@ -985,60 +1097,85 @@ static rms_status fd_d_entries(rms_plan *p, const rms_job_adj *adj, const std::v
     const int64_t ch = (p->prm.chunk_steps > 0 && p->prm.chunk_steps < TN) ? p->prm.chunk_steps : TN;
     std::vector<double> Fp((size_t)nj * (1 + R)), Fm((size_t)nj * (1 + R));
     std::vector<double> Dp(nj), Dm(nj);
+    std::vector<uint32_t> vis;
+    if (subsets) {
+        vis.resize((size_t)p->ws);
+        HIPCHK(ctx, hipMemcpyAsync(vis.data(), p->d_wvis, sizeof(uint32_t) * vis.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(ctx, hipStreamSynchronize(s));
+    }
     rms_status st = RMS_OK;
-    for (int c = 1; c <= maxc && st == RMS_OK; c++) {
-        std::vector<char> bad(nj, 0);
-        for (int side = 0; side < 2 && st == RMS_OK; side++) {
-            std::vector<JobDesc> hj = p->hjobs;
-            for (int j = 0; j < nj; j++) {
-                if (!which[j] || p->hjobs[j].nclass <= c) continue;
-                rms_geometry g = ctx->geoms[p->jobs[j].geom_id].meta;
-                const double D0 = g.D[c];
-                g.D[c] = side == 0 ? D0 * (1.0 + h) : D0 * (1.0 - h);
-                (side == 0 ? Dp : Dm)[j] = g.D[c];
-                int bits = 0;
-                bool exch = false;
-                JobDesc D = p->hjobs[j];
-                const char *e = job_physics(g, p->prm.dt, D, bits, exch);
-                if (e || (bits & ~p->sig & (SIG_EXCH | SIG_PERIODIC)) != 0) {
-                    bad[j] = 1;
-                    continue;
+    {
+        LayoutGuard keep(p);
+        for (int c = 1; c <= maxc && st == RMS_OK; c++) {
+            std::vector<char> bad(nj, 0), inc(nj, 1), zero(nj, 0);
+            std::vector<JobDesc> base;
+            if (subsets) {
+                if ((st = fd_subset_layout(p, which, c, h, vis, keep.grid, base, inc, zero)) != RMS_OK) break;
+            } else {
+                base = p->hjobs;
+            }
+            bool any = false;
+            for (int j = 0; j < nj; j++) any = any || (inc[j] && which[j] && p->hjobs[j].nclass > c);
+            for (int side = 0; side < 2 && st == RMS_OK && any; side++) {
+                std::vector<JobDesc> hj = base;
+                for (int j = 0; j < nj; j++) {
+                    if (!which[j] || p->hjobs[j].nclass <= c || !inc[j]) continue;
+                    rms_geometry g = ctx->geoms[p->jobs[j].geom_id].meta;
+                    const double D0 = g.D[c];
+                    g.D[c] = side == 0 ? D0 * (1.0 + h) : D0 * (1.0 - h);
+                    (side == 0 ? Dp : Dm)[j] = g.D[c];
+                    int bits = 0;
+                    bool exch = false;
+                    JobDesc D = base[j];
+                    const char *e = job_physics(g, p->prm.dt, D, bits, exch);
+                    if (e || (bits & ~p->sig & (SIG_EXCH | SIG_PERIODIC)) != 0) {
+                        bad[j] = 1;
+                        continue;
+                    }
+                    hj[j] = D;
                 }
-                hj[j] = D;
+                if (hipMemcpyAsync(p->d_jobs, hj.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s) !=
+                    hipSuccess) {
+                    st = fail(ctx, RMS_ERR_HIP, "adjoint: descriptor upload failed");
+                    break;
+                }
+                if ((st = enqueue(p, true)) != RMS_OK) break;
+                if (hipMemcpyAsync(p->h_res, p->d_res, (size_t)p->res_all, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                    hipStreamSynchronize(s) != hipSuccess) {
+                    st = fail(ctx, RMS_ERR_HIP, "adjoint: forward walk at a perturbed D failed");
+                    break;
+                }
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, p->ev[0], p->ev[1]) == hipSuccess) *walk_ms += ms;
+                *launches += (int)((TN + ch - 1) / ch);
+                for (int j = 0; j < nj; j++)
+                    if (!subsets || inc[j]) *walked += hj[j].nwalk * TN;
+                std::vector<double> &F = side == 0 ? Fp : Fm;
+                for (int j = 0; j < nj; j++) {
+                    if (!which[j] || p->hjobs[j].nclass <= c || !inc[j]) continue;
+                    const JobHost &H = p->jobs[j];
+                    const int64_t nq = T * H.nval;
+                    F[(size_t)j * (1 + R)] =
+                        contract(adj[j], p->h_res + H.rec_tot, nq, p->h_res + H.sig_tot, H.ngrad);
+                    for (int r = 0; r < R; r++)
+                        F[(size_t)j * (1 + R) + 1 + r] =
+                            contract(adj[j], p->h_res + H.rec_rep + (int64_t)r * nq, nq,
+                                     p->h_res + H.sig_rep + (int64_t)r * H.ngrad, H.ngrad);
+                }
             }
-            if (hipMemcpyAsync(p->d_jobs, hj.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s) != hipSuccess) {
-                st = fail(ctx, RMS_ERR_HIP, "adjoint: descriptor upload failed");
-                break;
-            }
-            if ((st = enqueue(p, true)) != RMS_OK) break;
-            if (hipMemcpyAsync(p->h_res, p->d_res, (size_t)p->res_all, hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess) {
-                st = fail(ctx, RMS_ERR_HIP, "adjoint: forward walk at a perturbed D failed");
-                break;
-            }
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, p->ev[0], p->ev[1]) == hipSuccess) *walk_ms += ms;
-            *launches += (int)((TN + ch - 1) / ch);
-            std::vector<double> &F = side == 0 ? Fp : Fm;
+            if (st != RMS_OK) break;
             for (int j = 0; j < nj; j++) {
                 if (!which[j] || p->hjobs[j].nclass <= c) continue;
-                const JobHost &H = p->jobs[j];
-                const int64_t nq = T * H.nval;
-                F[(size_t)j * (1 + R)] = contract(adj[j], p->h_res + H.rec_tot, nq, p->h_res + H.sig_tot, H.ngrad);
-                for (int r = 0; r < R; r++)
-                    F[(size_t)j * (1 + R) + 1 + r] = contract(adj[j], p->h_res + H.rec_rep + (int64_t)r * nq, nq,
-                                                              p->h_res + H.sig_rep + (int64_t)r * H.ngrad, H.ngrad);
+                const double den = Dp[j] - Dm[j];
+                const double *fp = &Fp[(size_t)j * (1 + R)], *fm = &Fm[(size_t)j * (1 + R)];
+                // no walker of the job can walk differently: the difference is zero
+                const double g0 = bad[j] ? NAN : (zero[j] ? 0.0 : (fp[0] - fm[0]) / den);
+                adj[j].grad[c] = g0;
+                if (adj[j].grad_rep)
+                    for (int r = 0; r < R; r++)
+                        adj[j].grad_rep[(size_t)r * NPARAM + c] =
+                            bad[j] ? NAN : (zero[j] ? 0.0 : (fp[1 + r] - fm[1 + r]) / den);
             }
-        }
-        if (st != RMS_OK) break;
-        for (int j = 0; j < nj; j++) {
-            if (!which[j] || p->hjobs[j].nclass <= c) continue;
-            const double den = Dp[j] - Dm[j];
-            const double *fp = &Fp[(size_t)j * (1 + R)], *fm = &Fm[(size_t)j * (1 + R)];
-            adj[j].grad[c] = bad[j] ? NAN : (fp[0] - fm[0]) / den;
-            if (adj[j].grad_rep)
-                for (int r = 0; r < R; r++)
-                    adj[j].grad_rep[(size_t)r * NPARAM + c] = bad[j] ? NAN : (fp[1 + r] - fm[1 + r]) / den;
         }
     }
     // the plan's own descriptors, whatever happened above
@@ -1072,10 +1209,29 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, const r
     const int grid = adjoint_grid(p, adj_sig);
     hipStream_t s = ctx->stream;
     const int64_t T = p->prm.timepoints;
+    // jobs whose D entries come from central differences: every job with
+    // RMS_D_CRN_FD, those not continuous in D (exchange faces, lattice-edge
+    // mirrors, voxel edges) with RMS_D_AUTO
+    std::vector<char> fd(nj, 0);
+    bool any_fd = false;
+    for (int j = 0; j < nj; j++) {
+        const bool jumps = p->jobs[j].exchange || p->ctx->geoms[p->jobs[j].geom_id].flags != 0;
+        fd[j] = (mode == RMS_D_CRN_FD || (mode == RMS_D_AUTO && jumps)) ? 1 : 0;
+        any_fd = any_fd || fd[j];
+    }
+    const bool subsets = any_fd && !(opts && opts->fd_all_walkers);
+    if (subsets && !p->d_red_fd) {
+        rms_status st;
+        if ((st = palloc(p, &p->d_wvis, sizeof(uint32_t) * p->ws)) != RMS_OK) return st;
+        if ((st = palloc(p, &p->d_wlist, sizeof(uint32_t) * p->ws)) != RMS_OK) return st;
+        if ((st = palloc(p, &p->d_bj_fd, sizeof(int32_t) * p->nbatch)) != RMS_OK) return st;
+        if ((st = palloc(p, &p->d_red_fd, sizeof(ReduceDesc) * p->red.size())) != RMS_OK) return st;
+    }
     std::vector<JobDesc> hj = p->hjobs;
     for (int j = 0; j < nj; j++) {
         const JobHost &H = p->jobs[j];
         JobDesc &D = hj[j];
+        D.wvis = (subsets && fd[j]) ? p->d_wvis + H.wbase : nullptr;
         D.cot_rec = nullptr;
         D.cot_sig = nullptr;
         if (adj[j].cot_rec) {
@@ -1119,9 +1275,9 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, const r
         reduce_ms = ms;
     }
     int launches = 1;
-    std::vector<char> fd(nj, 0);
-    bool any_fd = false;
+    int64_t walked = 0;
     for (int j = 0; j < nj; j++) {
+        walked += p->jobs[j].nwalk * p->prm.TN;
         std::memcpy(adj[j].grad, p->h_adj_res + (size_t)j * NPARAM, sizeof(double) * NPARAM);
         if (adj[j].grad_rep)
             std::memcpy(adj[j].grad_rep, p->h_adj_res + (size_t)nj * NPARAM + (size_t)j * R * NPARAM,
@@ -1134,10 +1290,7 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, const r
         // exchange faces, lattice-edge mirrors and voxel edges make the
         // walk's end discontinuous in D
         const bool jumps = p->jobs[j].exchange || p->ctx->geoms[p->jobs[j].geom_id].flags != 0;
-        if (mode == RMS_D_CRN_FD || (mode == RMS_D_AUTO && jumps)) {
-            fd[j] = 1;
-            any_fd = true;
-        } else if (jumps && !(adj_sig & SIG_JUMP)) {
+        if (!fd[j] && jumps && !(adj_sig & SIG_JUMP)) {
             for (int c = 1; c < NC; c++) nan_entry(c);
         }
         if (p->jobs[j].kappa_closed) {
@@ -1149,7 +1302,7 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, const r
     if (any_fd) {
         const auto t0 = std::chrono::steady_clock::now();
         double fd_ms = 0.0;
-        rms_status st = fd_d_entries(p, adj, fd, h, &fd_ms, &launches);
+        rms_status st = fd_d_entries(p, adj, fd, h, subsets, &fd_ms, &launches, &walked);
         if (st != RMS_OK) return st;
         walk_ms += fd_ms;
         total_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1162,6 +1315,7 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, const r
         timing->grid = grid;
         timing->block = p->block;
         timing->launches = launches;
+        timing->walker_steps = walked;
     }
     return RMS_OK;
 }
@@ -1361,6 +1515,7 @@ static rms_status run_noalloc_impl(rms_ctx *ctx, const rms_job *job, const rms_r
         timing->grid = (int32_t)grid;
         timing->block = block;
         timing->launches = 1;
+        timing->walker_steps = job->nwalk * prm->TN;
     }
     return RMS_OK;
 #undef NA_HIP

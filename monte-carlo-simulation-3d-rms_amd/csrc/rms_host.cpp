@@ -175,6 +175,24 @@ bool wrap_cells_rule(const rms_geometry &g)
     return 256 * side > walk;
 }
 
+bool fd_walker_mask(const rms_geometry &g, double dt, int c, double h, uint32_t &mask)
+{
+    mask = 1u << c;
+    rms_geometry gp = g, gm = g;
+    gp.D[c] = g.D[c] * (1.0 + h);  // the points of the central differences
+    gm.D[c] = g.D[c] * (1.0 - h);
+    Derived d0, dp, dm;
+    if (derive(g, dt, d0) || derive(gp, dt, dp) || derive(gm, dt, dm)) return false;
+    for (int a = 1; a < g.nclass; a++)
+        for (int b = 1; b < g.nclass; b++) {
+            const int ab = a * NC + b;
+            if (a == b || (dp.P[ab] == d0.P[ab] && dm.P[ab] == d0.P[ab])) continue;
+            if (d0.Pcat[ab] != 2 || dp.Pcat[ab] != 2 || dm.Pcat[ab] != 2) return false;
+            mask |= 1u << (4 + ab);
+        }
+    return true;
+}
+
 bool kappa_closes_pair(const rms_geometry &g, const Derived &d)
 {
     for (int a = 1; a < g.nclass; a++)

@@ -53,6 +53,18 @@ uint32_t geometry_flags(const rms_geometry &g);
 // voxels lie on a periodic side.
 bool wrap_cells_rule(const rms_geometry &g);
 
+// Walkers whose walk can change when D_c moves to D_c (1 +- h) (the central
+// differences of the adjoint's D entries): those that visit class c, and those
+// that draw at an exchange face whose P changes with D_c, since the draw may
+// then decide the other way.  Every other walker takes each step with the
+// same length and decides each face with the same P, so its walk is the same
+// at both points and adds nothing to the difference.  mask = bits of the
+// tangent pass's per-walker marks (bit c, bit 4 + a*NC + b for draws from
+// class a into class b) that make a walker one of them.  False when a face
+// whose P changes crosses or reflects without a draw at one of the three
+// points (walkers that meet it leave no mark): then every walker is walked.
+bool fd_walker_mask(const rms_geometry &g, double dt, int c, double h, uint32_t &mask);
+
 // Steps must stay below one voxel by this margin: the straight move ends at
 // e = fma(s, v, o) with o in [0, 1], and e must round below 2 in FP32 and FP64.
 constexpr double STEP_MAX = 1.0 - 0x1p-20;

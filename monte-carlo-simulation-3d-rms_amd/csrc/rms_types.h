@@ -88,6 +88,12 @@ struct JobDesc {
     double *pgrad;             // [nbatch_job][NPARAM]
     int64_t wbase;             // first walker slot in the state arrays
     int64_t wid0;              // Philox walker id of the job's first walker
+    // walks of a subset of the job's walkers: launch walker k is walker
+    // wlist[k] of the job (null: walker k), the list padded to whole batches
+    const uint32_t *wlist;
+    // tangent pass output per walker (null: not recorded): bit c = visited
+    // class c, bit 4 + a*NC + b = drew at an exchange face from class a to b
+    uint32_t *wvis;
 };
 
 // Walker state saved between chunked launches (structure of arrays).

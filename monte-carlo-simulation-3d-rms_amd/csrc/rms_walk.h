@@ -100,6 +100,7 @@ struct Tan {
     double S[NP > 0 ? NP : 1];
     double Sk;
     V3<double> PJ[NP > 0 ? NP : 1];
+    uint32_t vis;  // classes visited (bit c), exchange draws from class a to b (bit 4 + a*NC + b)
 };
 
 // Per-job constants held in registers for the whole batch.
@@ -434,6 +435,7 @@ __device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLoca
     }
     if (cross) return false;
     if constexpr (NP > 0) {
+        if (ndraw > 0u) T.vis |= 1u << (4 + pidx);
         // scores of the reflecting Bernoulli(P) draws, one add per draw
         const double *sc4 = J.dlog1mP + 4 * pidx;
 #pragma unroll
@@ -780,6 +782,7 @@ __device__ __forceinline__ void march_core(Walker<R> &w, const JobLocal<R> &L, c
                     nd++;
                     cross = u < L.P[pidx];
                     if constexpr (NP > 0) {
+                        T.vis |= 1u << (4 + pidx);
                         // score of the Bernoulli(P) draw
                         const double *sc4 = cross ? (J.dlogP + 4 * pidx) : (J.dlog1mP + 4 * pidx);
 #pragma unroll
@@ -799,6 +802,7 @@ __device__ __forceinline__ void march_core(Walker<R> &w, const JobLocal<R> &L, c
                     if (cn != w.c) {
                         flush(w);
                         w.c = cn;
+                        if constexpr (NP > 0) T.vis |= 1u << cn;
                         w.sc = sel4(cn, L.st0, L.st1, L.st2, L.st3);
                         w.isc = sel4(cn, L.is0, L.is1, L.is2, L.is3);
                     }
@@ -1429,8 +1433,10 @@ void walk_kernel(LaunchArgs a)
         const JobDesc &J = a.jobs[jid];
         const JobLocal<R> L = job_local<R>(J);
         const int64_t lb = b - J.batch0;                   // batch within the job
-        const bool active = lb * WAVE + lane < J.nwalk;
-        const uint64_t wid = (uint64_t)(J.wid0 + lb * WAVE + lane);  // Philox walker id
+        const int64_t kw = lb * WAVE + lane;               // walker of the job's launch
+        const bool active = kw < J.nwalk;
+        // Philox walker id (a subset run walks the listed walkers of the job)
+        const uint64_t wid = (uint64_t)(J.wid0 + (J.wlist != nullptr ? (int64_t)J.wlist[kw] : kw));
         const int64_t slot = J.wbase + lb * WAVE + lane;    // state slot
         const int nbvec = (SIG & 1) ? J.nbvec : 0;
         const int nval = 1 + L.nclass + 21 + nbvec;
@@ -1474,6 +1480,7 @@ void walk_kernel(LaunchArgs a)
                 w.sg = {a.st.sg[slot], a.st.sg[ws + slot], a.st.sg[2 * ws + slot]};
             }
         }
+        if constexpr (NP > 0) T.vis = 1u << w.c;
         // step counters fit int32 (TN < 2^31 is checked on the host) and are
         // wave-uniform: next record step and next PGSE boundary at or after step0
         const int32_t step0 = (int32_t)a.step0, step1 = (int32_t)a.step1, Tstep = (int32_t)a.Tstep;
@@ -1802,6 +1809,7 @@ void walk_kernel(LaunchArgs a)
             double *pg = J.pgrad + lb * NPARAM;
 #pragma unroll
             for (int p = 0; p < NP; p++) put(GD[p], active, pg + 1 + p, lane);
+            if (J.wvis != nullptr && active) J.wvis[kw] = T.vis;
             put(GT0, active, pg + 4, lane);
             put(GT1, active, pg + 5, lane);
             put(GT2, active, pg + 6, lane);

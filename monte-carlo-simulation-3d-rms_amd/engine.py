@@ -322,14 +322,17 @@ class Plan:
             split_records(r, j["nclass"], j["nbvec"])
         return res
 
-    def adjoint(self, cot_rec, cot_sig=None, want_replicas=False, estimator="auto", fd_step=0.0):
+    def adjoint(self, cot_rec, cot_sig=None, want_replicas=False, estimator="auto", fd_step=0.0,
+                fd_all_walkers=False):
         """Vector-Jacobian product of the record/signal sums with respect to
         D[c] (entries 0..3), T2[c] (4..7) and kappa (8).  estimator picks the
         D entries (rms_adjoint_opts): "auto" pathwise where the walk is
         continuous in D, central differences with common random numbers
         elsewhere; "fd" central differences for every job; "boundary"
         pathwise plus the boundary terms; "pathwise" pathwise only (NaN where
-        the walk jumps).  fd_step is the relative step h (0 = 0.05)."""
+        the walk jumps).  fd_step is the relative step h (0 = 0.05).  The
+        central differences for class c walk only the walkers whose walk can
+        change with D_c, unless fd_all_walkers."""
         n = len(self.jobs)
         ca = (capi.JobAdj * n)()
         outs = []
@@ -346,7 +349,8 @@ class Plan:
             ca[k].grad_rep = _ptr(gr)
             outs.append({"grad": g, "grad_rep": gr})
         tm = capi.Timing()
-        opts = capi.AdjointOpts(d_estimator=capi.D_ESTIMATOR[estimator], fd_rel_step=float(fd_step))
+        opts = capi.AdjointOpts(d_estimator=capi.D_ESTIMATOR[estimator], fd_all_walkers=int(bool(fd_all_walkers)),
+                                fd_rel_step=float(fd_step))
         self.eng._check(capi.lib().rms_plan_adjoint_ex(self._h, ca, C.byref(opts), C.byref(tm)),
                         "rms_plan_adjoint_ex")
         self.adj_timing = {f: getattr(tm, f) for f, _ in capi.Timing._fields_}

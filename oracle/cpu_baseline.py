@@ -21,7 +21,6 @@ has batches to run, so the samples cut walkers, never steps):
 from __future__ import annotations
 
 import argparse
-import copy
 import json
 import os
 import sys
@@ -79,25 +78,16 @@ def main():
         return r, time.perf_counter() - t0
 
     def fwd_adj(jobs, TN, T):
-        # the forward pass, the tangent pass, and the forward passes of the
-        # central differences of D, as the engine's default adjoint runs them
+        # the forward pass, then the engine's default adjoint: the tangent
+        # pass and the forward passes of the central differences of D on the
+        # walkers whose walks can change with D_c
         r, t1 = timed(jobs, TN, T)
         TD = A.diff_times(W.DT, TN, T)
         cots = [W.c4_cotangent(A, x["rec"], x["_nclass"], TD, x["_nbvec"]) for x in r]
-        _, t2 = timed(jobs, TN, T, cot_rec=cots, boundary=False)
-        t3 = 0.0
-        for c in range(1, max(j["geom"].nclass for j in jobs)):
-            for f in (1.05, 0.95):
-                pj = []
-                for j in jobs:
-                    g = copy.copy(j["geom"])
-                    g.D = list(g.D)
-                    if c < g.nclass:
-                        g.D[c] = g.D[c] * f
-                    pj.append(dict(j, geom=g))
-                _, t = timed(pj, TN, T)
-                t3 += t
-        return t1 + t2 + t3
+        t0 = time.perf_counter()
+        orc.central_differences(jobs, W.DT, TN, T, cots, h=0.05, prec="f64", replicas=32, nthreads=nth,
+                                subsets=True)
+        return t1 + time.perf_counter() - t0
 
     if "c1" in todo:
         c = W.C1
@@ -128,7 +118,7 @@ def main():
         ws = n * c["TN"]
         out["c4"] = dict(value=ws / t, seconds=t, walker_steps=ws,
                          sample=f"{n} walkers (1 %) x {c['TN']} steps, forward + adjoint (tangent pass + "
-                                "4 forward passes of the central differences)")
+                                "the forward passes of the central differences on walker subsets)")
     if "c5" in todo:
         c = W.C5
         jobs = W.c5_jobs(orc.Geometry, range(64), 160)

@@ -7,6 +7,7 @@ sources carry the line-by-line citations.
 """
 from __future__ import annotations
 
+import copy
 import ctypes as C
 import os
 import subprocess
@@ -59,6 +60,7 @@ class _Job(C.Structure):
         ("ngrad", C.c_int32),
         ("grad", C.c_void_p),
         ("walker_base", C.c_int64),
+        ("wlist", C.c_void_p),
     ]
 
 
@@ -89,6 +91,7 @@ class _Out(C.Structure):
         ("fin_oi", C.c_void_p),
         ("fin_c", C.c_void_p),
         ("fin_cnt", C.c_void_p),
+        ("fin_vis", C.c_void_p),
     ]
 
 
@@ -319,14 +322,17 @@ def nval(nclass, nbvec):
 
 
 def run(jobs, dt, TN, timepoints, prec="f32", replicas=32, pgse_bruteforce=False, nthreads=0,
-        final_state=False, cot_rec=None, cot_sig=None, boundary=True):
-    """Run a list of jobs; each job is a dict(geom, nwalk, seed, bval, bvec, grad).
+        final_state=False, cot_rec=None, cot_sig=None, boundary=True, marks=False):
+    """Run a list of jobs; each job is a dict(geom, nwalk, seed, bval, bvec, grad,
+    optionally walker_base and wlist: walk walkers wlist[k] of the job).
 
     Returns per job a dict with the record totals split into the
     reference's output arrays (sig0, NPar_count, dx2, dx4, sigRe), the PGSE
     signal, replica sums, counters and optionally the final walker state.
     With cotangents, the adjoint pass; boundary=False leaves out the
-    boundary terms of the D derivative (the tangent pass alone)."""
+    boundary terms of the D derivative (the tangent pass alone); marks=True
+    returns its per-walker marks "vis" (bit c = visited class c, bit
+    4 + 4a + b = drew at an exchange face from class a into class b)."""
     L = lib()
     n = len(jobs)
     cj = (_Job * n)()
@@ -355,6 +361,13 @@ def run(jobs, dt, TN, timepoints, prec="f32", replicas=32, pgse_bruteforce=False
         cj[k].ngrad = ng
         cj[k].grad = _ptr(grad)
         cj[k].walker_base = int(jb.get("walker_base", 0))
+        wl = jb.get("wlist")
+        if wl is not None:
+            wl = np.ascontiguousarray(wl, np.uint32)
+            if wl.size != int(jb["nwalk"]):
+                raise ValueError("wlist needs nwalk entries")
+            keep.append(wl)
+        cj[k].wlist = _ptr(wl)
         nv = nval(geom.nclass, nb)
         R = max(1, replicas)
         r = {
@@ -377,6 +390,9 @@ def run(jobs, dt, TN, timepoints, prec="f32", replicas=32, pgse_bruteforce=False
             r["fin_cnt"] = np.zeros((5, W), np.uint32)
             for name in ("fin_g", "fin_o", "fin_gi", "fin_oi", "fin_c", "fin_cnt"):
                 setattr(co[k], name, _ptr(r[name]))
+        if marks:
+            r["vis"] = np.zeros(int(jb["nwalk"]), np.uint32)
+            co[k].fin_vis = _ptr(r["vis"])
         r["_nclass"] = geom.nclass
         r["_nbvec"] = nb
         res.append(r)
@@ -416,3 +432,102 @@ def split_records(r, nclass, nbvec):
     r["dx4"] = rec[:, 7 + nclass:22 + nclass]
     r["sigRe"] = rec[:, 22 + nclass:22 + nclass + nbvec]
     return r
+
+
+# ------------------------------------------------- central differences of D
+def fd_walker_mask(geom: Geometry, dt: float, c: int, h: float):
+    """Marks (bits of the adjoint pass's per-walker "vis") that make a walker's
+    walk depend on D_c between D_c (1 - h) and D_c (1 + h): class c, and the
+    draws at exchange faces whose P differs at one of the three points.  None
+    when such a face crosses or reflects without a draw at one of them (its
+    walkers carry no mark): every walker is needed."""
+    def P_at(f):
+        g = copy.copy(geom)
+        g.D = list(geom.D)
+        g.D[c] = geom.D[c] * f
+        d = derive(g, dt)
+        return d["P"], d["Pcat"]
+
+    P0, K0 = P_at(1.0)
+    Pp, Kp = P_at(1.0 + h)
+    Pm, Km = P_at(1.0 - h)
+    mask = 1 << c
+    for a in range(1, geom.nclass):
+        for b in range(1, geom.nclass):
+            if a == b or (Pp[a, b] == P0[a, b] and Pm[a, b] == P0[a, b]):
+                continue
+            if K0[a, b] != 2 or Kp[a, b] != 2 or Km[a, b] != 2:
+                return None
+            mask |= 1 << (4 + a * NC_MAX + b)
+    return mask
+
+
+def contract(cot_rec, rec, cot_sig=None, sig=None):
+    """sum(cot_rec * rec) + sum(cot_sig * sig), added in row-major order."""
+    f = 0.0
+    for x, y in zip(np.ravel(cot_rec).tolist(), np.ravel(rec).tolist()):
+        f = f + x * y
+    if cot_sig is not None and sig is not None:
+        for x, y in zip(np.ravel(cot_sig).tolist(), np.ravel(sig).tolist()):
+            f = f + x * y
+    return f
+
+
+def central_differences(jobs, dt, TN, timepoints, cot_rec, cot_sig=None, h=0.05, prec="f32", replicas=32,
+                        nthreads=0, subsets=True):
+    """D entries of the adjoint by central differences with common random
+    numbers: per non-null class c, forward runs with D_c (1 +- h) and the
+    same seeds, dF/dD_c = (F+ - F-) / (D+ - D-) per job, total and per
+    replica.  With subsets the runs for class c walk, per job, only the
+    walkers whose adjoint-pass marks meet fd_walker_mask (in index order;
+    none: the entry is 0), from one tangent pass (boundary=False) run here.
+    Returns (grad[njobs][4], grad_rep[njobs][replicas][4]); entry 0 unused."""
+    n = len(jobs)
+    R = max(1, replicas)
+    grad = np.zeros((n, 4))
+    grad_rep = np.zeros((n, R, 4))
+    marks = None
+    if subsets:
+        marks = [r["vis"] for r in run(jobs, dt, TN, timepoints, prec=prec, replicas=R, nthreads=nthreads,
+                                       cot_rec=cot_rec, cot_sig=cot_sig, boundary=False, marks=True)]
+    maxc = max(j["geom"].nclass for j in jobs)
+    for c in range(1, maxc):
+        todo = []
+        for k, jb in enumerate(jobs):
+            if jb["geom"].nclass <= c:
+                continue
+            sel = dict(jb)
+            if subsets:
+                mask = fd_walker_mask(jb["geom"], dt, c, h)
+                if mask is not None:
+                    idx = np.nonzero(marks[k] & np.uint32(mask))[0].astype(np.uint32)
+                    if idx.size == 0:
+                        continue  # entry 0: no walker can walk differently
+                    sel.update(nwalk=int(idx.size), wlist=idx)
+            todo.append((k, sel))
+        if not todo:
+            continue
+        F, Dv = [], []
+        for f in (1.0 + h, 1.0 - h):
+            pj = []
+            for k, sel in todo:
+                g = copy.copy(sel["geom"])
+                g.D = list(g.D)
+                g.D[c] = g.D[c] * f
+                pj.append(dict(sel, geom=g))
+            rs = run(pj, dt, TN, timepoints, prec=prec, replicas=R, nthreads=nthreads)
+            Fk = []
+            for (k, _), r in zip(todo, rs):
+                cs = None if cot_sig is None else cot_sig[k]
+                tot = contract(cot_rec[k], r["rec"], cs, r["sig"])
+                rep = [contract(cot_rec[k], r["rec_rep"][q], cs, None if r["sig_rep"] is None else r["sig_rep"][q])
+                       for q in range(R)]
+                Fk.append((tot, rep))
+            F.append(Fk)
+            Dv.append([p["geom"].D[c] for p in pj])
+        for m, (k, _) in enumerate(todo):
+            den = Dv[0][m] - Dv[1][m]
+            grad[k, c] = (F[0][m][0] - F[1][m][0]) / den
+            for q in range(R):
+                grad_rep[k, q, c] = (F[0][m][1][q] - F[1][m][1][q]) / den
+    return grad, grad_rep

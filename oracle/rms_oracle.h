@@ -70,6 +70,9 @@ typedef struct {
     int32_t ngrad;               /* PGSE table (0 = none) */
     const double *grad;          /* 6*ngrad rows: Delta, delta, |g|, gx, gy, gz */
     int64_t walker_base;         /* Philox id of the first walker (multiple of 64*replicas) */
+    /* walk a subset: walker k of the run is walker wlist[k] of the job
+     * (NULL: walker k); nwalk entries */
+    const uint32_t *wlist;
 } or_job;
 
 typedef struct {
@@ -100,6 +103,9 @@ typedef struct {
     double *fin_oi;       /* [3*nwalk] */
     int32_t *fin_c;       /* [nwalk] */
     uint32_t *fin_cnt;    /* [5*nwalk] per-walker counters */
+    /* adjoint pass: per walker, bit c = visited class c, bit 4 + a*4 + b =
+     * drew at an exchange face from class a into class b (NULL = skip) */
+    uint32_t *fin_vis;    /* [nwalk] */
 } or_out;
 
 /* ---- primitives (exported for known-answer tests) ---- */

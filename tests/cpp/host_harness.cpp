@@ -320,6 +320,31 @@ static void storage_variants()
     check(check_desc(D, SIG_PERIODIC | SIG_WRAP) != nullptr, "wrapped cell words without a periodic axis", "");
 }
 
+// Walkers of the central differences of D_c: the mask holds class c and the
+// draws at faces whose P moves with D_c.  With D1 = 2 > D2 = 0.8 and a
+// permeability, P_21 = min(1, sqrt(D1/D2)) pi0/(1 + pi0) with pi0 from the
+// smaller step (class 2's) does not depend on D1, P_12 does (the square root),
+// and both depend on D2 (pi0).  Without a permeability P_21 = 1 at every
+// point.  With D1 = 1, D2 = 1.02 and no permeability, P_12 = 1 at the base
+// point and below 1 at D_c (1 -+ h): no trustworthy mask.
+static void fd_masks()
+{
+    const double dt = 2e-4, h = 0.05;
+    const uint32_t b12 = 1u << (4 + 1 * NC + 2), b21 = 1u << (4 + 2 * NC + 1);
+    uint32_t m = 0;
+    rms_geometry g = geom(3, 2.0, 0.8, 0.2);
+    check(fd_walker_mask(g, dt, 1, h, m) && m == (2u | b12), "fd mask: D1 moves P_12 only", "");
+    check(fd_walker_mask(g, dt, 2, h, m) && m == (4u | b12 | b21), "fd mask: D2 moves both", "");
+    g = geom(3, 2.0, 0.8, -1.0);
+    check(fd_walker_mask(g, dt, 1, h, m) && m == (2u | b12), "fd mask: reference rule, D1", "");
+    check(fd_walker_mask(g, dt, 2, h, m) && m == (4u | b12), "fd mask: reference rule, D2", "");
+    g = geom(3, 1.0, 1.02, -1.0);
+    check(!fd_walker_mask(g, dt, 1, h, m), "fd mask: P_12 leaves 1 at D1 (1 + h)", "");
+    check(!fd_walker_mask(g, dt, 2, h, m), "fd mask: P_12 leaves 1 at D2 (1 - h)", "");
+    g = geom(2, 2.0, 0.0, -1.0);
+    check(fd_walker_mask(g, dt, 1, h, m) && m == 2u, "fd mask: one non-null class", "");
+}
+
 int main()
 {
     pgse_tables_vs_masks();
@@ -330,6 +355,7 @@ int main()
     wrap_rule();
     perturbed_physics();
     storage_variants();
+    fd_masks();
     std::printf("%d failures\n", nfail);
     return nfail;
 }

@@ -20,6 +20,16 @@ def random_blobs(shape, nclass, seed, p_null=0.15, dtype=np.uint8):
     return np.ascontiguousarray(lab.astype(dtype))
 
 
+def slabs_z(nlabel, nxy=6, depth=30, p_null=0.05):
+    """Slabs of `depth` voxels along z, label k in the k-th, a few null voxels
+    scattered: walkers deep in a slab do not reach another label for a while."""
+    lab = np.zeros((nxy, nxy, depth * (nlabel - 1)), np.uint8)
+    for k in range(1, nlabel):
+        lab[:, :, depth * (k - 1):depth * k] = k
+    lab[random_blobs(lab.shape, 2, seed=3, p_null=p_null) == 0] = 0
+    return lab
+
+
 def cylinder(n=24, nz=10, r=8.0, dtype=np.uint8):
     x = np.arange(n) + 0.5 - n / 2
     X, Y = np.meshgrid(x, x, indexing="ij")

@@ -6,7 +6,7 @@ central differences of the forward pass with common random numbers."""
 import numpy as np
 import pytest
 
-from helpers import fib_dirs, pgse_table, planes, random_blobs
+from helpers import fib_dirs, pgse_table, planes, random_blobs, slabs_z
 
 pytestmark = pytest.mark.gpu
 
@@ -60,10 +60,11 @@ def _contract(cot_rec, rec, cot_sig, sig):
                                                   ("f32", "both", "xyz")])
 def test_auto_adjoint_central_differences_match_oracle(rms, orc, engine, prec, tables, periodic):
     """The default D estimator on a geometry whose walk jumps in D (exchange
-    between two non-null classes, voxel edges): T2 and kappa entries
-    bit-identical to the oracle's adjoint; each D entry, total and per
-    replica, equal to the central difference of the oracle's forward runs at
-    D_c (1 +- h) with the same seeds, contracted in the same order."""
+    between two non-null classes, voxel edges), its central differences
+    walking every walker: T2 and kappa entries bit-identical to the oracle's
+    adjoint; each D entry, total and per replica, equal to the central
+    difference of the oracle's forward runs at D_c (1 +- h) with the same
+    seeds, contracted in the same order."""
     print(f"case auto adjoint {prec} {tables} periodic={periodic}", flush=True)
     lab = random_blobs((10, 11, 9), 3, seed=41)
     D, T2, kappa, h = [0.0, 2.0, 0.8], [np.inf, 70.0, 40.0], 0.2, 0.03
@@ -81,7 +82,7 @@ def test_auto_adjoint_central_differences_match_oracle(rms, orc, engine, prec, t
     plan = engine.plan([dict(geom=rms.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic), **job)],
                        dt, TN, T, precision=prec, replicas=R)
     try:
-        adj = plan.adjoint(cot_rec, cot_sig, want_replicas=True, fd_step=h)[0]
+        adj = plan.adjoint(cot_rec, cot_sig, want_replicas=True, fd_step=h, fd_all_walkers=True)[0]
     finally:
         plan.close()
     og = orc.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic)
@@ -105,6 +106,135 @@ def test_auto_adjoint_central_differences_match_oracle(rms, orc, engine, prec, t
             fr = _contract(cot_rec[0], rp["rec_rep"][r], cs, sp) - _contract(cot_rec[0], rm["rec_rep"][r], cs, sm)
             assert adj["grad_rep"][r, c] == fr / (dp - dm), (c, r)
     assert np.all(np.isfinite(adj["grad"][1:3])) and np.all(adj["grad"][1:3] != 0.0)
+
+
+def _fd_scale(cot_rec, rec, cot_sig, sig, h, Dc):
+    """Size of the rounding in a central difference of D_c: the objective's
+    absolute terms over the difference of the two D values, times 1e-12."""
+    f = np.sum(np.abs(cot_rec) * np.abs(rec))
+    if cot_sig is not None:
+        f += np.sum(np.abs(cot_sig) * np.abs(sig))
+    return 1e-12 * f / (2 * h * Dc)
+
+
+SUBSET_CASES = {
+    # two non-null classes with a permeability: the draws at faces into the
+    # class whose D moves mark their walkers
+    "two": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=0.2, prec="f32", tables="narrow"),
+    # three non-null classes, FP64, PGSE signal in the objective
+    "three": dict(labels=4, D=[0.0, 2.0, 0.7, 1.3], kappa=0.3, prec="f64", tables="pgse"),
+    # the reference rule P = min(1, sqrt(Dj/Di)) without a permeability:
+    # walkers cross into the faster class without a draw (their class change
+    # marks them)
+    "nokappa": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=None, prec="f32", tables="narrow"),
+    # D_1 and D_2 so close that D_c (1 +- h) moves P_12 off 1: a face that
+    # crosses without a draw at the base point draws at another, so every
+    # walker is walked and the entries equal those of the full walks exactly
+    "kink": dict(labels=3, D=[0.0, 1.0, 1.02], kappa=None, prec="f32", tables="narrow"),
+    # class 3 has no voxel: no walker can walk differently with D_3
+    "absent": dict(labels=3, nclass=4, D=[0.0, 2.0, 0.8, 1.1], kappa=0.2, prec="f32", tables="narrow"),
+}
+
+
+@pytest.mark.parametrize("case", sorted(SUBSET_CASES))
+def test_central_differences_on_walker_subsets(rms, engine, case):
+    """The central differences of D walk, for class c, only the walkers that
+    the tangent pass saw in class c or drawing at an exchange face whose P
+    depends on D_c: each D entry equals that of the walks of every walker to
+    rounding (the other walkers' walks are the same at both points), T2 and
+    kappa entries are untouched, and the replicas sum to the total.  A plan
+    with a second job whose D entries are pathwise (periodic planes) leaves
+    that job's entries bit-identical."""
+    cfg = SUBSET_CASES[case]
+    print(f"case central differences on walker subsets: {case}", flush=True)
+    lab = random_blobs((10, 11, 9), cfg["labels"], seed=52)
+    nclass = cfg.get("nclass", cfg["labels"])
+    D = cfg["D"]
+    T2 = [np.inf, 70.0, 40.0, 55.0][:nclass]
+    kw = {} if cfg["kappa"] is None else dict(kappa=cfg["kappa"])
+    lc = np.arange(cfg["labels"], dtype=np.int32) if nclass != cfg["labels"] else None
+    g = rms.Geometry(lab, 0.1, D, T2=T2, label_class=lc, **kw)
+    job = dict(geom=g, nwalk=1500, seed=77)
+    if cfg["tables"] == "narrow":
+        job.update(bval=np.array([0.7, 2.5]), bvec=fib_dirs(2))
+    else:
+        job.update(grad=pgse_table(fib_dirs(2), [1.0], [0.03], 0.01))
+    gp = rms.Geometry(planes(7, 6, 12), 0.1, [0.0, 1.8], T2=[50.0, 60.0], periodic="xyz")
+    jobs = [job, dict(job, geom=gp, seed=78, nwalk=700)]
+    dt, TN, T, R, h = 2e-4, 400, 16, 4, 0.05
+    nb = len(job["bval"]) if "bval" in job else 0
+    ng = job["grad"].shape[0] if "grad" in job else 0
+    rng = np.random.default_rng(13)
+    cot_rec = [rng.standard_normal((T, rms.nval(nclass, nb))), rng.standard_normal((T, rms.nval(2, nb)))]
+    cot_sig = [rng.standard_normal(ng), rng.standard_normal(ng)] if ng else None
+    plan = engine.plan(jobs, dt, TN, T, precision=cfg["prec"], replicas=R)
+    try:
+        fwd = plan.execute()
+        sub = plan.adjoint(cot_rec, cot_sig, want_replicas=True)
+        t_sub = plan.adj_timing["walk_ms"]
+        full = plan.adjoint(cot_rec, cot_sig, want_replicas=True, fd_all_walkers=True)
+        t_full = plan.adj_timing["walk_ms"]
+    finally:
+        plan.close()
+    print(f"adjoint walk ms: subsets {t_sub:.2f}, every walker {t_full:.2f}", flush=True)
+    a, b = sub[0], full[0]
+    np.testing.assert_array_equal(a["grad"][4:9], b["grad"][4:9])
+    np.testing.assert_array_equal(a["grad_rep"][:, 4:9], b["grad_rep"][:, 4:9])
+    cs = None if cot_sig is None else cot_sig[0]
+    sig = fwd[0]["sig"]
+    for c in range(1, nclass):
+        tol = _fd_scale(cot_rec[0], fwd[0]["rec"], cs, sig, h, D[c])
+        assert abs(a["grad"][c] - b["grad"][c]) <= tol, (c, a["grad"][c], b["grad"][c], tol)
+        assert abs(a["grad_rep"][:, c].sum() - a["grad"][c]) <= tol + 1e-12 * np.abs(a["grad_rep"][:, c]).sum()
+    if case == "kink":
+        np.testing.assert_array_equal(a["grad"], b["grad"])
+        np.testing.assert_array_equal(a["grad_rep"], b["grad_rep"])
+    if case == "absent":
+        assert a["grad"][3] == 0.0 and b["grad"][3] == 0.0
+        assert np.all(a["grad_rep"][:, 3] == 0.0)
+    assert np.all(np.isfinite(a["grad"][1:nclass])) and np.all(a["grad"][1:3] != 0.0)
+    # the pathwise job: no central differences at all
+    np.testing.assert_array_equal(sub[1]["grad"], full[1]["grad"])
+    np.testing.assert_array_equal(sub[1]["grad_rep"], full[1]["grad_rep"])
+
+
+@pytest.mark.parametrize("prec,case", [("f32", "strong"), ("f64", "three"), ("f32", "nokappa")])
+def test_walker_subset_differences_bitexact_against_oracle(rms, orc, engine, prec, case):
+    """The default central differences (walker subsets, h = 0.05) on slab
+    geometries where most walkers never meet another class: every D entry,
+    total and per replica, bit-identical to the oracle's central differences
+    on the walkers its own tangent pass marks (same packing into batches:
+    the marked walkers in index order), for two jobs in one plan."""
+    cfg = {"strong": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=5.0),
+           "three": dict(labels=4, D=[0.0, 2.0, 0.7, 1.3], kappa=0.3),
+           "nokappa": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=None)}[case]
+    print(f"case walker-subset central differences vs oracle: {prec} {case}", flush=True)
+    lab = slabs_z(cfg["labels"])
+    nc = len(cfg["D"])
+    kw = {} if cfg["kappa"] is None else dict(kappa=cfg["kappa"])
+    T2 = [np.inf, 70.0, 40.0, 55.0][:nc]
+    mk = lambda M: M.Geometry(lab, 0.1, cfg["D"], T2=T2, **kw)  # noqa: E731
+    tab = dict(bval=np.array([0.8, 2.0]), bvec=fib_dirs(2))
+    spec = [dict(nwalk=1500, seed=12, **tab), dict(nwalk=900, seed=13, **tab)]
+    dt, TN, T, R = 2e-4, 500, 10, 4
+    rng = np.random.default_rng(4)
+    cot = [rng.standard_normal((T, rms.nval(nc, 2))) for _ in spec]
+    plan = engine.plan([dict(geom=mk(rms), **j) for j in spec], dt, TN, T, precision=prec, replicas=R)
+    try:
+        adj = plan.adjoint(cot, None, want_replicas=True)
+        walked = plan.adj_timing["walker_steps"]
+    finally:
+        plan.close()
+    og = mk(orc)
+    g, gr = orc.central_differences([dict(geom=og, **j) for j in spec], dt, TN, T, cot, h=0.05, prec=prec,
+                                    replicas=R, subsets=True)
+    for k in range(len(spec)):
+        for c in range(1, nc):
+            assert adj[k]["grad"][c] == g[k, c], (k, c, adj[k]["grad"][c], g[k, c])
+            np.testing.assert_array_equal(adj[k]["grad_rep"][:, c], gr[k, :, c])
+    # the tangent pass walks every walker, the central differences fewer
+    full = sum(j["nwalk"] for j in spec) * TN
+    assert full < walked < full * (1 + 2 * (nc - 1)), (walked, full)
 
 
 def test_central_differences_nan_where_a_perturbation_needs_another_variant(rms, engine):

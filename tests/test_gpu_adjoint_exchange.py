@@ -170,6 +170,7 @@ def test_central_differences_on_walker_subsets(rms, engine, case):
     plan = engine.plan(jobs, dt, TN, T, precision=cfg["prec"], replicas=R)
     try:
         fwd = plan.execute()
+        assert plan.timing["walker_steps"] == sum(j["nwalk"] for j in jobs) * TN
         sub = plan.adjoint(cot_rec, cot_sig, want_replicas=True)
         t_sub = plan.adj_timing["walk_ms"]
         full = plan.adjoint(cot_rec, cot_sig, want_replicas=True, fd_all_walkers=True)

@@ -180,7 +180,8 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the walk shards by walker "
-              "ranges with one FP64 all-reduce at the end, which this build does not run.", file=sys.stderr)
+              "ranges with one FP64 all-reduce at the end; that multi-GPU driver is not part of this package.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -1001,11 +1001,13 @@ struct LayoutGuard {
 // from batch 0; jobs with none are left out (inc[j] = 0, zero[j] = 1: their
 // difference is exactly zero), and a job whose mask cannot be trusted walks
 // every walker.  Descriptors go to `hj`, the batch table, walker lists and
-// reductions to the plan's subset buffers, and the plan points at them, with
-// its own grid (grid0) cut to the batches.
+// reductions to the plan's subset buffers, and the plan points at them.  The
+// plan's own reductions (red0) and grid (grid0) are the starting point every
+// time: the plan may still point at an earlier class's subsets.
 static rms_status fd_subset_layout(rms_plan *p, const std::vector<char> &which, int c, double h,
-                                   const std::vector<uint32_t> &vis, int grid0, std::vector<JobDesc> &hj,
-                                   std::vector<char> &inc, std::vector<char> &zero)
+                                   const std::vector<uint32_t> &vis, const std::vector<ReduceDesc> &red0,
+                                   int grid0, std::vector<JobDesc> &hj, std::vector<char> &inc,
+                                   std::vector<char> &zero)
 {
     rms_ctx *ctx = p->ctx;
     hipStream_t s = ctx->stream;
@@ -1047,7 +1049,7 @@ static rms_status fd_subset_layout(rms_plan *p, const std::vector<char> &which, 
         D.wlist = all ? nullptr : p->d_wlist + at;
         for (int64_t b = 0; b < nbj; b++) bj.push_back(j);
         for (size_t r = r0; r < ri; r++) {
-            ReduceDesc e = p->red[r];
+            ReduceDesc e = red0[r];
             e.nb = nbj;
             rd.push_back(e);
         }
@@ -1110,7 +1112,8 @@ static rms_status fd_d_entries(rms_plan *p, const rms_job_adj *adj, const std::v
             std::vector<char> bad(nj, 0), inc(nj, 1), zero(nj, 0);
             std::vector<JobDesc> base;
             if (subsets) {
-                if ((st = fd_subset_layout(p, which, c, h, vis, keep.grid, base, inc, zero)) != RMS_OK) break;
+                if ((st = fd_subset_layout(p, which, c, h, vis, keep.red, keep.grid, base, inc, zero)) != RMS_OK)
+                    break;
             } else {
                 base = p->hjobs;
             }

@@ -199,36 +199,44 @@ def test_central_differences_on_walker_subsets(rms, engine, case):
     np.testing.assert_array_equal(sub[1]["grad_rep"], full[1]["grad_rep"])
 
 
-@pytest.mark.parametrize("prec,case", [("f32", "strong"), ("f64", "three"), ("f32", "nokappa")])
+@pytest.mark.parametrize("prec,case", [("f32", "strong"), ("f64", "three"), ("f32", "nokappa"), ("f32", "mixed")])
 def test_walker_subset_differences_bitexact_against_oracle(rms, orc, engine, prec, case):
     """The default central differences (walker subsets, h = 0.05) on slab
     geometries where most walkers never meet another class: every D entry,
     total and per replica, bit-identical to the oracle's central differences
     on the walkers its own tangent pass marks (same packing into batches:
-    the marked walkers in index order), for two jobs in one plan."""
+    the marked walkers in index order), for two jobs in one plan.  "mixed":
+    the first job's lattice has no voxel of class 1, so its walks for D_1
+    are left out while the second job's are not, and both walk for D_2."""
     cfg = {"strong": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=5.0),
            "three": dict(labels=4, D=[0.0, 2.0, 0.7, 1.3], kappa=0.3),
-           "nokappa": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=None)}[case]
+           "nokappa": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=None),
+           "mixed": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=5.0)}[case]
     print(f"case walker-subset central differences vs oracle: {prec} {case}", flush=True)
     lab = slabs_z(cfg["labels"])
+    labs = [lab, lab]
+    if case == "mixed":
+        labs[0] = np.where(lab == 1, 2, lab).astype(np.uint8)
     nc = len(cfg["D"])
     kw = {} if cfg["kappa"] is None else dict(kappa=cfg["kappa"])
     T2 = [np.inf, 70.0, 40.0, 55.0][:nc]
-    mk = lambda M: M.Geometry(lab, 0.1, cfg["D"], T2=T2, **kw)  # noqa: E731
+    mk = lambda M, L: M.Geometry(L, 0.1, cfg["D"], T2=T2, **kw)  # noqa: E731
     tab = dict(bval=np.array([0.8, 2.0]), bvec=fib_dirs(2))
     spec = [dict(nwalk=1500, seed=12, **tab), dict(nwalk=900, seed=13, **tab)]
     dt, TN, T, R = 2e-4, 500, 10, 4
     rng = np.random.default_rng(4)
     cot = [rng.standard_normal((T, rms.nval(nc, 2))) for _ in spec]
-    plan = engine.plan([dict(geom=mk(rms), **j) for j in spec], dt, TN, T, precision=prec, replicas=R)
+    plan = engine.plan([dict(geom=mk(rms, L), **j) for L, j in zip(labs, spec)], dt, TN, T, precision=prec,
+                       replicas=R)
     try:
         adj = plan.adjoint(cot, None, want_replicas=True)
         walked = plan.adj_timing["walker_steps"]
     finally:
         plan.close()
-    og = mk(orc)
-    g, gr = orc.central_differences([dict(geom=og, **j) for j in spec], dt, TN, T, cot, h=0.05, prec=prec,
-                                    replicas=R, subsets=True)
+    g, gr = orc.central_differences([dict(geom=mk(orc, L), **j) for L, j in zip(labs, spec)], dt, TN, T, cot,
+                                    h=0.05, prec=prec, replicas=R, subsets=True)
+    if case == "mixed":
+        assert adj[0]["grad"][1] == 0.0 and adj[0]["grad"][2] != 0.0 and adj[1]["grad"][1] != 0.0
     for k in range(len(spec)):
         for c in range(1, nc):
             assert adj[k]["grad"][c] == g[k, c], (k, c, adj[k]["grad"][c], g[k, c])

@@ -199,19 +199,23 @@ def test_central_differences_on_walker_subsets(rms, engine, case):
     np.testing.assert_array_equal(sub[1]["grad_rep"], full[1]["grad_rep"])
 
 
-@pytest.mark.parametrize("prec,case", [("f32", "strong"), ("f64", "three"), ("f32", "nokappa"), ("f32", "mixed")])
+@pytest.mark.parametrize("prec,case", [("f32", "strong"), ("f64", "three"), ("f32", "nokappa"), ("f32", "mixed"),
+                                       ("f64", "pgse")])
 def test_walker_subset_differences_bitexact_against_oracle(rms, orc, engine, prec, case):
     """The default central differences (walker subsets, h = 0.05) on slab
     geometries where most walkers never meet another class: every D entry,
     total and per replica, bit-identical to the oracle's central differences
     on the walkers its own tangent pass marks (same packing into batches:
-    the marked walkers in index order), for two jobs in one plan.  "mixed":
-    the first job's lattice has no voxel of class 1, so its walks for D_1
-    are left out while the second job's are not, and both walk for D_2."""
+    the marked walkers in index order), for two jobs in one plan, the second
+    a shard (walker_base: Philox ids from 64 R 3).  "mixed": the first job's
+    lattice has no voxel of class 1, so its walks for D_1 are left out while
+    the second job's are not, and both walk for D_2.  "pgse": a PGSE table,
+    its signal in the objective."""
     cfg = {"strong": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=5.0),
            "three": dict(labels=4, D=[0.0, 2.0, 0.7, 1.3], kappa=0.3),
            "nokappa": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=None),
-           "mixed": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=5.0)}[case]
+           "mixed": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=5.0),
+           "pgse": dict(labels=3, D=[0.0, 2.0, 0.8], kappa=5.0)}[case]
     print(f"case walker-subset central differences vs oracle: {prec} {case}", flush=True)
     lab = slabs_z(cfg["labels"])
     labs = [lab, lab]
@@ -222,19 +226,23 @@ def test_walker_subset_differences_bitexact_against_oracle(rms, orc, engine, pre
     T2 = [np.inf, 70.0, 40.0, 55.0][:nc]
     mk = lambda M, L: M.Geometry(L, 0.1, cfg["D"], T2=T2, **kw)  # noqa: E731
     tab = dict(bval=np.array([0.8, 2.0]), bvec=fib_dirs(2))
-    spec = [dict(nwalk=1500, seed=12, **tab), dict(nwalk=900, seed=13, **tab)]
+    if case == "pgse":
+        tab = dict(grad=pgse_table(fib_dirs(3), [1.0, 2.0], [0.03], 0.01))
     dt, TN, T, R = 2e-4, 500, 10, 4
+    spec = [dict(nwalk=1500, seed=12, **tab), dict(nwalk=900, seed=13, walker_base=64 * R * 3, **tab)]
     rng = np.random.default_rng(4)
-    cot = [rng.standard_normal((T, rms.nval(nc, 2))) for _ in spec]
+    nb = 0 if case == "pgse" else 2
+    cot = [rng.standard_normal((T, rms.nval(nc, nb))) for _ in spec]
+    cots = [rng.standard_normal(tab["grad"].shape[0]) for _ in spec] if case == "pgse" else None
     plan = engine.plan([dict(geom=mk(rms, L), **j) for L, j in zip(labs, spec)], dt, TN, T, precision=prec,
                        replicas=R)
     try:
-        adj = plan.adjoint(cot, None, want_replicas=True)
+        adj = plan.adjoint(cot, cots, want_replicas=True)
         walked = plan.adj_timing["walker_steps"]
     finally:
         plan.close()
     g, gr = orc.central_differences([dict(geom=mk(orc, L), **j) for L, j in zip(labs, spec)], dt, TN, T, cot,
-                                    h=0.05, prec=prec, replicas=R, subsets=True)
+                                    cots, h=0.05, prec=prec, replicas=R, subsets=True)
     if case == "mixed":
         assert adj[0]["grad"][1] == 0.0 and adj[0]["grad"][2] != 0.0 and adj[1]["grad"][1] != 0.0
     for k in range(len(spec)):

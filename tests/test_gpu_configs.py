@@ -15,9 +15,11 @@ quoted on are the paths checked here:
       the autotuned shape and order, bit-exact, each job equal to itself run
       alone
   C4  the axon with extra-axonal space at kappa = 0.01, 1e4 walkers x 5e4
-      steps, forward and adjoint bit-exact
+      steps, forward and adjoint bit-exact (the boundary-term estimator, and
+      the default one: its central differences on walker subsets against the
+      oracle's on its own marks)
   C5  8 of the 4096 sweep geometries (70x70x1000 uint16), 640 walkers each,
-      forward and adjoint bit-exact
+      forward and adjoint bit-exact, both estimators
 """
 import math
 
@@ -156,12 +158,20 @@ def _adjoint_pair(rms, orc, engine, W, gj, oj, TN, T, nclass):
     TD = A.diff_times(W.DT, TN, T)
     cots = [W.c4_cotangent(A, f["rec"], nclass, TD, int(j["bval"].size)) for f, j in zip(fwd, gj)]
     adj = plan.adjoint(cots, None, want_replicas=True, estimator="boundary")
+    # the default estimator: central differences of D on walker subsets
+    dflt = plan.adjoint(cots, None, want_replicas=True)
     plan.close()
     ref = orc.run(oj, W.DT, TN, T, prec="f32", replicas=32, cot_rec=cots)
     assert_same(fwd, ref)
     for a, b in zip(adj, ref):
         np.testing.assert_array_equal(a["grad"], b["grad"])
         np.testing.assert_array_equal(a["grad_rep"], b["grad_rep"])
+    print("case central differences on walker subsets against the oracle", flush=True)
+    g, gr = orc.central_differences(oj, W.DT, TN, T, cots, h=0.05, prec="f32", replicas=32, subsets=True)
+    for k, a in enumerate(dflt):
+        np.testing.assert_array_equal(a["grad"][1:nclass], g[k, 1:nclass])
+        np.testing.assert_array_equal(a["grad_rep"][:, 1:nclass], gr[k, :, 1:nclass])
+        np.testing.assert_array_equal(a["grad"][4:9], adj[k]["grad"][4:9])
     return fwd, adj
 
 

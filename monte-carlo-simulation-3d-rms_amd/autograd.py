@@ -15,9 +15,10 @@ Estimators used by the adjoint pass:
          length d s_c / d D_c = s_c / (2 D_c), reflections folded, plus the
          score of the exchange draws) where the walk's end moves continuously
          with D, and central differences with common random numbers (the
-         same seeds at D_c (1 +- h)) where it jumps: exchange faces,
-         lattice-edge mirrors, voxel edges whose face order decides the
-         outcome.  "boundary" adds the boundary terms of those jumps to the
+         same seeds at D_c (1 +- h), walking only the walkers whose walk can
+         change with D_c) where it jumps: exchange faces, lattice-edge
+         mirrors, voxel edges whose face order decides the outcome.
+         "boundary" adds the boundary terms of those jumps to the
          pathwise derivative instead (unbiased, slow, high variance; NaN with
          a PGSE table), "fd" differences every job, "pathwise" returns NaN
          where the walk jumps;

@@ -2,7 +2,9 @@
 per side, 2..4 classes (uint8 or uint16 labels), random D, T2, kappa (or the
 reference rule), periodic axes, FP32/FP64, narrow-pulse or PGSE tables,
 chunked launches -- each run by the HIP engine and the oracle and compared
-bit for bit (records, replicas, counters, final walker states, PGSE signal)."""
+bit for bit (records, replicas, counters, final walker states, PGSE signal);
+the adjoint on 100 of them, random multi-job plans, and the central
+differences of D on walker subsets in 100 random plans."""
 import numpy as np
 import pytest
 
@@ -117,3 +119,51 @@ def test_random_multi_job_plans_bitexact(rms, orc, engine):
         j = int(rng.integers(0, nj))
         alone = engine.run([gj[j]], 2e-4, TN, T, precision=prec, replicas=4)[0]
         np.testing.assert_array_equal(alone["rec"], a[j]["rec"], err_msg=f"alone job {j}: {desc}")
+
+
+def test_random_plans_subset_differences_bitexact(rms, orc, engine):
+    """100 seeded random plans of 1-4 random jobs (random launch shapes and
+    chunks), the central differences of D for every job (estimator "fd") on
+    the walkers the tangent pass marks: every D entry, total and per
+    replica, bit-identical to the oracle's differences on its own marks.
+    Entries the engine leaves NaN (a perturbation that needs another kernel
+    variant) are skipped; most are compared."""
+    rng = np.random.default_rng(20261207)
+    compared = skipped = 0
+    for k in range(100):
+        nj = int(rng.integers(1, 5))
+        prec = "f32" if rng.random() < 0.6 else "f64"
+        TN = int(rng.integers(20, 400))
+        T = int(rng.integers(1, min(TN, 40) + 1))
+        chunk = int(rng.choice([0, int(rng.integers(7, 200))]))
+        shape = dict(block_threads=int(rng.choice([0, 64, 128, 256])),
+                     grid_blocks=int(rng.choice([0, int(rng.integers(1, 3000))])), xcd_split=int(rng.choice([-1, 0, 1])))
+        gj, oj, cot_rec, cot_sig = [], [], [], []
+        for _ in range(nj):
+            lab, D, T2, kappa, periodic, _, _, _, job, _ = _case(rng)
+            gj.append(dict(geom=rms.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic), **job))
+            oj.append(dict(geom=orc.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic), **job))
+        nbs = [len(j["bval"]) if "bval" in j else 0 for j in gj]
+        ngs = [j["grad"].shape[0] if "grad" in j else 0 for j in gj]
+        cot_rec = [rng.standard_normal((T, rms.nval(len(j["geom"].D), n))) for j, n in zip(gj, nbs)]
+        cot_sig = [rng.standard_normal(ng) for ng in ngs] if any(ngs) else None
+        desc = (f"subset plan {k}: {nj} jobs {prec} TN={TN} T={T} chunk={chunk} launch={shape} "
+                f"shapes={[j['geom'].labels.shape for j in gj]} D={[np.round(j['geom'].D, 2).tolist() for j in gj]}")
+        print(desc, flush=True)
+        plan = engine.plan(gj, 2e-4, TN, T, precision=prec, replicas=4, chunk_steps=chunk, **shape)
+        try:
+            a = plan.adjoint(cot_rec, cot_sig, want_replicas=True, estimator="fd")
+        finally:
+            plan.close()
+        g, gr = orc.central_differences(oj, 2e-4, TN, T, cot_rec, cot_sig, h=0.05, prec=prec, replicas=4,
+                                        subsets=True)
+        for j in range(nj):
+            for c in range(1, len(gj[j]["geom"].D)):
+                if np.isnan(a[j]["grad"][c]):
+                    skipped += 1
+                    continue
+                compared += 1
+                assert a[j]["grad"][c] == g[j, c], f"job {j} D_{c}: {a[j]['grad'][c]} vs {g[j, c]}: {desc}"
+                np.testing.assert_array_equal(a[j]["grad_rep"][:, c], gr[j, :, c], err_msg=f"job {j} D_{c}: {desc}")
+    print(f"compared {compared} entries, skipped {skipped}", flush=True)
+    assert compared > 5 * skipped

@@ -86,6 +86,7 @@ struct Walker {
     int32_t lin;     // linear voxel index of g
     R tcur;          // steps spent in class c since the last flush
     R sc, isc;       // step of class c (voxels) and its reciprocal
+    int32_t hc;      // reflect-sure threshold of the first exchange draw's high half from class c
     double ts0, ts1, ts2, ts3;  // steps per class
     uint32_t ncross, nrefl, nmirror, ntrunc, ndraw;
 };
@@ -506,7 +507,7 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
         // or above the class's threshold), or that reflects without a draw, is
         // folded like a null face; other cases take the branch below
         const uint32_t om = (rels >> 1) & ~rels & 0x15u;  // CELL_OTHER slots
-        const int32_t hc = w.c == 1 ? L.xh12 : L.xh21;
+        const int32_t hc = w.hc;
         const bool easy = om == 0u || ((om & (om - 1u)) == 0u && (int32_t)xhi >= hc);
         rare = (rels & (rels >> 1) & 0x15u) != 0u || kx == ky || !easy;
         refl = (rels | (rels >> 1)) & 0x15u;
@@ -805,6 +806,7 @@ __device__ __forceinline__ void march_core(Walker<R> &w, const JobLocal<R> &L, c
                         if constexpr (NP > 0) T.vis |= 1u << cn;
                         w.sc = sel4(cn, L.st0, L.st1, L.st2, L.st3);
                         w.isc = sel4(cn, L.is0, L.is1, L.is2, L.is3);
+                        w.hc = cn == 1 ? L.xh12 : L.xh21;
                     }
                     w.ncross++;
                 } else {
@@ -1044,6 +1046,7 @@ __device__ bool jump_event(const Walker<R> &w, const JobLocal<R> &L, const doubl
     A1.c = cq;
     A1.sc = sel4(cq, L.st0, L.st1, L.st2, L.st3);
     A1.isc = sel4(cq, L.is0, L.is1, L.is2, L.is3);
+    A1.hc = cq == 1 ? L.xh12 : L.xh21;
     return true;
 }
 
@@ -1123,6 +1126,7 @@ __device__ __forceinline__ void load_cont(const uint32_t *lds, int m, int t, con
     a.lin = a.g.x * L.sx + a.g.y * L.nz + a.g.z;
     a.sc = sel4(a.c, L.st0, L.st1, L.st2, L.st3);
     a.isc = sel4(a.c, L.is0, L.is1, L.is2, L.is3);
+    a.hc = a.c == 1 ? L.xh12 : L.xh21;
     a.ps = {0.0, 0.0, 0.0};
     a.sg = {0, 0, 0};
     a.ncross = a.nrefl = a.nmirror = a.ntrunc = a.ndraw = 0u;
@@ -1470,11 +1474,13 @@ void walk_kernel(LaunchArgs a)
             }
             w.sc = sel4(w.c, L.st0, L.st1, L.st2, L.st3);
             w.isc = sel4(w.c, L.is0, L.is1, L.is2, L.is3);
+            w.hc = w.c == 1 ? L.xh12 : L.xh21;
         } else {
             load_state<R>(w, a.st, slot);
             w.lin = w.g.x * L.sx + w.g.y * L.nz + w.g.z;
             w.sc = sel4(w.c, L.st0, L.st1, L.st2, L.st3);
             w.isc = sel4(w.c, L.is0, L.is1, L.is2, L.is3);
+            w.hc = w.c == 1 ? L.xh12 : L.xh21;
             if (SIG & 2) {
                 w.ps = {a.st.ps[slot], a.st.ps[ws + slot], a.st.ps[2 * ws + slot]};
                 w.sg = {a.st.sg[slot], a.st.sg[ws + slot], a.st.sg[2 * ws + slot]};

@@ -110,6 +110,10 @@ typedef struct {
     int32_t block_threads;       /* walkers per workgroup, 0 = auto (256) */
     int32_t grid_blocks;         /* workgroups, 0 = auto (resident limit) */
     int32_t xcd_split;           /* 1 = give each XCD a contiguous batch range */
+    /* 1 = leave rms_job_out.counters at zero: the per-step diagnostic sums
+     * (crossings, reflections, mirrors, draws; the reference has none) cost
+     * about 3 % of the walk.  Final walker states always count. */
+    int32_t skip_counters;
 } rms_run_params;
 
 /* Per-job outputs (caller-owned host buffers; NULL pointers are skipped).

@@ -84,6 +84,7 @@ class RunParams(C.Structure):
         ("block_threads", C.c_int32),
         ("grid_blocks", C.c_int32),
         ("xcd_split", C.c_int32),
+        ("skip_counters", C.c_int32),
     ]
 
 

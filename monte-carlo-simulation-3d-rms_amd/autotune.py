@@ -72,14 +72,14 @@ def tune(engine, jobs, dt, timepoints=10, precision="f32", probe_steps=400, reps
         ordered = [jobs[k] for k in order]
         # resident grid from a probe plan at full occupancy
         p = engine.plan(ordered, dt, probe_steps, timepoints=min(timepoints, probe_steps), precision=precision,
-                        replicas=1, block_threads=cand["block_threads"], xcd_split=cand["xcd_split"])
+                        replicas=1, block_threads=cand["block_threads"], xcd_split=cand["xcd_split"], counters=False)
         p.execute(want_replicas=False)
         grid_full = p.timing["grid"]
         p.close()
         grid = max(8, int(grid_full * cand["occ_mult"]) // 8 * 8)
         p = engine.plan(ordered, dt, probe_steps, timepoints=min(timepoints, probe_steps), precision=precision,
                         replicas=1, block_threads=cand["block_threads"], grid_blocks=grid,
-                        xcd_split=cand["xcd_split"])
+                        xcd_split=cand["xcd_split"], counters=False)
         ts = []
         for _ in range(reps):
             p.execute(want_replicas=False)

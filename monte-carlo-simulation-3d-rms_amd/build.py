@@ -53,7 +53,8 @@ def _run(cmd):
 # the device assembly of every unit is scanned for the signature.
 JUMP_FLAGS = ["-mllvm", "-vgpr-regalloc=basic"]
 UNITS = (("rms_kernels.hip", []), ("rms_kernels_jump.hip", JUMP_FLAGS), ("rms_kernels_tangent.hip", []),
-         ("rms_kernels_bricks.hip", []), ("rms_kernels_wrap.hip", []), ("rms_api.cpp", []), ("rms_host.cpp", []))
+         ("rms_kernels_bricks.hip", []), ("rms_kernels_wrap.hip", []), ("rms_kernels_nocount.hip", []),
+         ("rms_api.cpp", []), ("rms_host.cpp", []))
 
 
 def spill_before_exec_restore(asm):

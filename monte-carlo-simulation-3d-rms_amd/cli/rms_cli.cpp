@@ -222,6 +222,7 @@ int run_body(int argc, char **argv, bool pgse)
     rp.timepoints = o.timepoints;
     rp.precision = o.precision;
     rp.replicas = 1;
+    rp.skip_counters = 1;  // the reference writes no diagnostic counters
     rms_plan *plan = nullptr;
     if (rms_plan_create(ctx, &job, 1, &rp, &plan) != RMS_OK) {
         std::string m = rms_last_error(ctx);

@@ -108,6 +108,7 @@ struct rms_plan {
     uint32_t *d_wvis = nullptr, *d_wlist = nullptr;
     int32_t *d_bj_fd = nullptr;
     ReduceDesc *d_red_fd = nullptr;
+    bool fd_walks = false;  // launches now are the central differences' walks
 };
 
 static const char *kVersion = "rms_amd 0.1 (gfx950)";
@@ -738,6 +739,10 @@ static LaunchArgs make_args(rms_plan *p, int64_t s0, int64_t s1)
     a.st = p->st;
     a.fin = p->fin;
     a.xcd_split = p->xcd_split;
+    // the diagnostic counters: unless skipped, and always with the final
+    // walker state (whose per-walker counters they are); never for the
+    // central differences' walks, which report only records
+    a.count = (!p->fd_walks && (p->prm.skip_counters == 0 || p->fin_alloc)) ? 1 : 0;
     return a;
 }
 
@@ -983,9 +988,11 @@ struct LayoutGuard {
     explicit LayoutGuard(rms_plan *q)
         : p(q), nbatch(q->nbatch), bj(q->d_batch_job), d_red(q->d_red), red(q->red), grid(q->grid)
     {
+        p->fd_walks = true;
     }
     ~LayoutGuard()
     {
+        p->fd_walks = false;
         p->nbatch = nbatch;
         p->d_batch_job = bj;
         p->d_red = d_red;
@@ -1255,6 +1262,7 @@ static rms_status plan_adjoint_impl(rms_plan *p, const rms_job_adj *adj, const r
     HIPCHK(ctx, hipEventRecord(p->ev[0], s));
     LaunchArgs a = make_args(p, 0, p->prm.TN);
     a.jobs = p->d_jobs_adj;
+    a.count = 0;  // the tangent pass reports no counters
     a.st.g = nullptr;  // single launch: no walker state is saved
     a.fin = FinalOut{};
     HIPCHK(ctx, launch_adjoint(a, p->prm.precision == RMS_PREC_F64, adj_sig, p->adj_np, grid, p->block, s));
@@ -1489,6 +1497,7 @@ static rms_status run_noalloc_impl(rms_ctx *ctx, const rms_job *job, const rms_r
     a.dt = P.dt;
     a.st.ws = nbatch * WAVE;
     a.xcd_split = xcd;
+    a.count = P.skip_counters == 0 ? 1 : 0;
     NA_HIP(hipEventRecord(ctx->ev[0], s));
     NA_HIP(launch_walk(a, f64, sig, (int)grid, block, s));
     NA_HIP(hipEventRecord(ctx->ev[1], s));

@@ -137,6 +137,7 @@ struct LaunchArgs {
     StateBuf st;
     FinalOut fin;
     int32_t xcd_split;         // 1: partition batches per XCD
+    int32_t count;             // 1: sum the per-walker diagnostic counters (crossings, reflections, ...)
 };
 
 // One ordered reduction: part[nb][nq] -> rep[R][nq] -> tot[nq].

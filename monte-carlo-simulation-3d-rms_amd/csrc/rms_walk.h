@@ -464,7 +464,7 @@ __device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLoca
 // and the straight move folded back at the reflecting faces.  Returns false
 // when the step needs the march (a crossing into another class, or three
 // comparisons that are not transitive: a near tie).
-template <typename R, int NP, bool EXCH, bool PER, bool WRAP>
+template <typename R, int NP, bool EXCH, bool PER, bool WRAP, bool CNT>
 __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t cell,
                                           const V3<R> &v, uint64_t wid, int32_t i, uint32_t xhi, int32_t npg,
                                           Tan<R, NP> &T)
@@ -617,11 +617,14 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
     w.g.y += gy;
     w.g.z += gz;
     w.lin += (gx == 0 ? 0 : (gx > 0 ? L.sx : -L.sx)) + (gy == 0 ? 0 : (gy > 0 ? L.nz : -L.nz)) + gz;
-    const uint32_t nref = (uint32_t)__builtin_popcount(refl);
-    w.nrefl += nref - nedge;
-    w.nmirror += nedge;
-    w.ndraw += ndraw;
-    w.ncross += (uint32_t)(__mul24(cx, cx) + __mul24(cy, cy) + __mul24(cz, cz)) - nref;
+    if constexpr (CNT) {
+        // diagnostic sums (about 3 % of the step; variants without them skip it)
+        const uint32_t nref = (uint32_t)__builtin_popcount(refl);
+        w.nrefl += nref - nedge;
+        w.nmirror += nedge;
+        w.ndraw += ndraw;
+        w.ncross += (uint32_t)(__mul24(cx, cx) + __mul24(cy, cy) + __mul24(cz, cz)) - nref;
+    }
     w.tcur = w.tcur + (R)1;
     return true;
 }
@@ -1407,6 +1410,10 @@ void walk_kernel(LaunchArgs a)
     constexpr bool PER = (SIG & 8) != 0;
     constexpr bool BR = (SIG & 32) != 0;
     constexpr bool WRAP = PER && (SIG & 64) != 0;
+    // the fast path's diagnostic counters: forward walks only (the tangent
+    // passes report none), and not in the variants built without them
+    // (bit 7, for launches that do not report counters)
+    constexpr bool CNT = NP == 0 && (SIG & 128) == 0;
     // sin/cos table of the FP32 direction draw, staged in LDS
     __shared__ float2 tab[256];
     // continuations of the kept boundary events (jump variants only)
@@ -1532,7 +1539,7 @@ void walk_kernel(LaunchArgs a)
                         uint64_t ca;
                         load_cont<R>(cont, m, tix, L, a, ca);
                         const int32_t l0 = a.lin;
-                        if (!fast_step<R, 0, EXCH, PER, WRAP>(a, L, J, ca, v, wid, i, xhi, 0, T0))
+                        if (!fast_step<R, 0, EXCH, PER, WRAP, false>(a, L, J, ca, v, wid, i, xhi, 0, T0))
                             march<R, 0, EXCH, PER>(a, L, J, wid, i, v, xhi, 0, T0);
                         if (a.lin != l0) ca = cell_of<R, BR>(L, a);
                         store_cont<R>(cont, m, tix, a, ca);
@@ -1589,7 +1596,7 @@ void walk_kernel(LaunchArgs a)
             }
             const int32_t lin0 = w.lin;
             const int32_t npg = i - i0;  // PGSE terms since the last flush
-            if (!fast_step<R, NP, EXCH, PER, WRAP>(w, L, J, cell, v, wid, i, xhi, npg, T))
+            if (!fast_step<R, NP, EXCH, PER, WRAP, CNT>(w, L, J, cell, v, wid, i, xhi, npg, T))
                 march<R, NP, EXCH, PER>(w, L, J, wid, i, v, xhi, npg, T);
             if (w.lin != lin0) cell = cell_of<R, BR>(L, w);  // only lanes that changed voxel read
             if (SIG & 2) {
@@ -1822,12 +1829,15 @@ void walk_kernel(LaunchArgs a)
             put(GT3, active, pg + 7, lane);
             put(GK, active, pg + 8, lane);
         } else if (a.step1 == a.TN) {
-            // diagnostics: exact integer sums
-            const uint32_t cs[5] = {w.ncross, w.nrefl, w.nmirror, w.ntrunc, w.ndraw};
+            // diagnostics: exact integer sums (only counted when the launch
+            // asks; the march's own increments are not reported otherwise)
+            if (a.count) {
+                const uint32_t cs[5] = {w.ncross, w.nrefl, w.nmirror, w.ntrunc, w.ndraw};
 #pragma unroll
-            for (int k = 0; k < 5; k++) {
-                const uint32_t t = wave_sum_u32(active ? cs[k] : 0u);
-                if (lane == 0) atomicAdd(J.counters + k, (unsigned long long)t);
+                for (int k = 0; k < 5; k++) {
+                    const uint32_t t = wave_sum_u32(active ? cs[k] : 0u);
+                    if (lane == 0) atomicAdd(J.counters + k, (unsigned long long)t);
+                }
             }
             if (a.fin.g && active) {
                 a.fin.g[slot] = w.g.x;

@@ -182,12 +182,14 @@ class Engine:
         return out + (bool(wc.value),) if wrap_cells else out
 
     def plan(self, jobs, dt, TN, timepoints=1000, precision="f32", replicas=32, chunk_steps=0, use_graph=False,
-             block_threads=0, grid_blocks=0, xcd_split=0):
+             block_threads=0, grid_blocks=0, xcd_split=0, counters=True):
+        """counters=False leaves the diagnostic counters (crossings,
+        reflections, mirrors, draws) at zero: about 3 % less work per step."""
         return Plan(self, jobs, dt, TN, timepoints, precision, replicas, chunk_steps, use_graph, block_threads,
-                    grid_blocks, xcd_split)
+                    grid_blocks, xcd_split, counters)
 
     def run_noalloc(self, job, dt, TN, timepoints=1000, precision="f32", replicas=32, block_threads=0, grid_blocks=0,
-                    xcd_split=0):
+                    xcd_split=0, counters=True):
         """One job through rms_run_noalloc: the walk of `run` with every device
         buffer in the library's static scratch array (no device allocation;
         the geometry is uploaded once, on first use, like `run` does)."""
@@ -205,7 +207,7 @@ class Engine:
         R = max(1, int(replicas))
         prm = capi.RunParams(dt=float(dt), TN=int(TN), timepoints=int(timepoints), precision=capi.PREC[precision],
                              replicas=R, chunk_steps=0, use_graph=0, block_threads=int(block_threads),
-                             grid_blocks=int(grid_blocks), xcd_split=int(xcd_split))
+                             grid_blocks=int(grid_blocks), xcd_split=int(xcd_split), skip_counters=int(not counters))
         nv = nval(geom.nclass, nb)
         r = {"rec": np.zeros((int(timepoints), nv)), "rec_rep": np.zeros((R, int(timepoints), nv)),
              "sig": np.zeros(ng) if ng else None, "sig_rep": np.zeros((R, ng)) if ng else None}
@@ -231,7 +233,7 @@ class Plan:
     """rms_plan_create / rms_plan_execute / rms_plan_adjoint."""
 
     def __init__(self, eng, jobs, dt, TN, timepoints, precision, replicas, chunk_steps, use_graph, block_threads,
-                 grid_blocks, xcd_split):
+                 grid_blocks, xcd_split, counters=True):
         self.eng = eng
         self.jobs = []
         n = len(jobs)
@@ -265,7 +267,8 @@ class Plan:
         self.params = capi.RunParams(
             dt=float(dt), TN=int(TN), timepoints=int(timepoints), precision=capi.PREC[precision],
             replicas=max(1, int(replicas)), chunk_steps=int(chunk_steps), use_graph=int(bool(use_graph)),
-            block_threads=int(block_threads), grid_blocks=int(grid_blocks), xcd_split=int(xcd_split))
+            block_threads=int(block_threads), grid_blocks=int(grid_blocks), xcd_split=int(xcd_split),
+            skip_counters=int(not counters))
         h = C.c_void_p()
         eng._check(capi.lib().rms_plan_create(eng._h, cj, n, C.byref(self.params), C.byref(h)), "rms_plan_create")
         self._h = h

@@ -39,6 +39,8 @@ def test_pure_host_entry_points(rms):
     np.testing.assert_array_equal(d["step"], np.sqrt(6 * 2e-4 * np.array([0, 2.0, 0.5])) / 0.1)
     assert d["P"][1, 2] == np.sqrt(0.25) and d["P"][2, 1] == 1.0
     assert C.sizeof(rms._capi.Geometry) == 4 * 4 + 8 + 8 + 4 + 4 + 8 + 8 * 4 * 2 + 8 * 16 + 4 + 4
+    # rms_run_params: dt, TN, three int32 (+4 padding), chunk_steps, five int32 (+4 padding)
+    assert C.sizeof(rms._capi.RunParams) == 8 + 8 + 3 * 4 + 4 + 8 + 5 * 4 + 4
     # rms_timing: three doubles, four int32, the int64 walker-step count
     assert C.sizeof(rms._capi.Timing) == 3 * 8 + 4 * 4 + 8
     # rms_adjoint_opts: estimator, fd_all_walkers, relative step

@@ -485,3 +485,52 @@ def test_periodic_sides_both_encodings_bitexact(rms, orc, engine, prec):
         np.testing.assert_array_equal(out["sig"], ref["sig"])
         np.testing.assert_array_equal(out["fin_gi"], ref["fin_gi"])
         assert np.any(np.abs(out["fin_gi"][2]) >= lab.shape[2] - 1) or not want  # walkers wrapped in z
+
+
+@pytest.mark.parametrize("prec,exch,chunk,graph", [("f32", False, 0, False), ("f32", True, 77, False),
+                                                  ("f64", True, 0, False), ("f32", True, 100, True)])
+def test_skipped_counters_leave_every_other_output_unchanged(rms, engine, prec, exch, chunk, graph):
+    """counters=False (rms_run_params.skip_counters): the diagnostic counters
+    are zero and records, replica sums and the PGSE signal bit-identical to
+    the counted run, single launch, chunks or a captured graph; asking for
+    the final walker state counts again (its per-walker counters)."""
+    print(f"case skipped counters {prec} exchange={exch} chunk={chunk} graph={graph}", flush=True)
+    lab = random_blobs((9, 8, 10), 3 if exch else 2, seed=8)
+    g = rms.Geometry(lab, 0.1, [0.0, 2.0, 0.9][:3 if exch else 2], T2=[np.inf, 70.0, 40.0][:3 if exch else 2],
+                     kappa=0.2 if exch else None, periodic="x")
+    jobs = [dict(geom=g, nwalk=700, seed=4, grad=pgse_table(fib_dirs(2), [1.0], [0.03], 0.01),
+                 bval=np.array([1.0, 2.0]), bvec=fib_dirs(2))]
+    kw = dict(precision=prec, replicas=4, chunk_steps=chunk, use_graph=graph)
+    on = engine.plan(jobs, 2e-4, 300, 12, **kw)
+    off = engine.plan(jobs, 2e-4, 300, 12, counters=False, **kw)
+    try:
+        a = on.execute()[0]
+        b = off.execute()[0]
+        b2 = off.execute()[0]  # a second (graph) replay
+        c = off.execute(final_state=True)[0]
+        d = on.execute(final_state=True)[0]
+    finally:
+        on.close()
+        off.close()
+    assert a["counters"][0] > 0 and a["counters"][1] > 0
+    for r in (b, b2):
+        assert np.all(r["counters"] == 0)
+        for key in ("rec", "rec_rep", "sig", "sig_rep"):
+            np.testing.assert_array_equal(r[key], a[key])
+    np.testing.assert_array_equal(c["counters"], a["counters"])
+    for key in ("rec", "fin_g", "fin_o", "fin_cnt", "fin_c"):
+        np.testing.assert_array_equal(c[key], d[key])
+
+
+def test_skipped_counters_allocation_free(rms, engine):
+    """The allocation-free run with counters=False: counters zero, records
+    equal to the counted allocation-free run."""
+    print("case skipped counters, allocation-free run", flush=True)
+    lab = random_blobs((8, 9, 7), 3, seed=12)
+    g = rms.Geometry(lab, 0.1, [0.0, 2.0, 0.9], T2=[np.inf, 70.0, 40.0], kappa=0.1)
+    job = dict(geom=g, nwalk=600, seed=2, bval=np.array([1.5]), bvec=fib_dirs(1))
+    a = engine.run_noalloc(job, 2e-4, 200, 10, replicas=4)
+    b = engine.run_noalloc(job, 2e-4, 200, 10, replicas=4, counters=False)
+    assert a["counters"][0] > 0 and np.all(b["counters"] == 0)
+    np.testing.assert_array_equal(a["rec"], b["rec"])
+    np.testing.assert_array_equal(a["rec_rep"], b["rec_rep"])

@@ -125,7 +125,8 @@ typedef struct {
     double *rec_rep;             /* [replicas * timepoints * nval] */
     double *sig;                 /* [ngrad] PGSE signal (sig.txt, main_PGSE.cu:648-651) */
     double *sig_rep;             /* [replicas * ngrad] */
-    /* crossings, reflections, edge mirrors, truncated steps, exchange draws */
+    /* crossings, reflections, edge mirrors, truncated steps, exchange draws
+     * (zero when rms_run_params.skip_counters is set) */
     uint64_t counters[5];
     /* optional final walker state, axis-major [3][nwalk] etc. */
     int32_t *fin_g;

@@ -44,9 +44,9 @@ const void *brick_walk_fn(int prec_f64, int sig, int np);
 // words are wrapped across periodic sides (sig bits 3 and 6, without bit 4)
 // (rms_kernels_wrap.hip).
 const void *wrap_walk_fn(int prec_f64, int sig, int np);
-// The FP32 forward variant without the fast path's diagnostic counters
-// (sig bit 7) for a plan variant sig, or null (rms_kernels_nocount.hip).
-const void *nocount_walk_fn(int sig);
+// The forward variant without the fast path's diagnostic counters (sig
+// bit 7) for a plan variant sig, or null (rms_kernels_nocount.hip).
+const void *nocount_walk_fn(int prec_f64, int sig);
 
 // Allocation-free runs: device address of this GPU's scratch array (the
 // current device's copy), and the two table kernels that fill it.

@@ -256,9 +256,8 @@ static hipError_t launch_fn(const void *fn, const LaunchArgs &a, int grid, int b
 
 hipError_t launch_walk(const LaunchArgs &a, int prec_f64, int sig, int grid, int block, hipStream_t s)
 {
-    // a launch that reports no counters takes the variant that does not sum
-    // them where one is built (FP32); the others count and do not report
-    const void *fn = (!a.count && !prec_f64) ? nocount_walk_fn(sig) : nullptr;
+    // a launch that reports no counters takes the variant that does not sum them
+    const void *fn = !a.count ? nocount_walk_fn(prec_f64, sig) : nullptr;
     return launch_fn(fn ? fn : walk_fn_np(prec_f64, sig, 0), a, grid, block, s);
 }
 

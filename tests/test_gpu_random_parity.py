@@ -89,8 +89,8 @@ def test_random_inputs_adjoint_bitexact(rms, orc, engine):
 def test_random_multi_job_plans_bitexact(rms, orc, engine):
     """60 seeded random plans of 2-5 random jobs each (mixed classes, tables,
     periodic axes and lattice sizes in one launch, random launch shapes and
-    chunks): every job bit-exact against the oracle and its records equal to
-    the job run alone."""
+    chunks, with or without the diagnostic counters): every job bit-exact
+    against the oracle and its records equal to the job run alone."""
     rng = np.random.default_rng(20261206)
     for k in range(60):
         nj = int(rng.integers(2, 6))
@@ -105,15 +105,21 @@ def test_random_multi_job_plans_bitexact(rms, orc, engine):
             lab, D, T2, kappa, periodic, _, _, _, job, _ = _case(rng)
             gj.append(dict(geom=rms.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic), **job))
             oj.append(dict(geom=orc.Geometry(lab, 0.1, D, T2=T2, kappa=kappa, periodic=periodic), **job))
-        desc = (f"plan {k}: {nj} jobs {prec} TN={TN} T={T} chunk={chunk} launch={shape} "
+        counted = bool(rng.random() < 0.5)
+        desc = (f"plan {k}: {nj} jobs {prec} TN={TN} T={T} chunk={chunk} launch={shape} counters={counted} "
                 f"shapes={[j['geom'].labels.shape for j in gj]} tables={[sorted(j) for j in gj]}")
         print(desc, flush=True)
-        a = engine.run(gj, 2e-4, TN, T, precision=prec, replicas=4, final_state=True, chunk_steps=chunk, **shape)
+        a = engine.run(gj, 2e-4, TN, T, precision=prec, replicas=4, final_state=counted, chunk_steps=chunk,
+                       counters=counted, **shape)
         b = orc.run(oj, 2e-4, TN, T, prec=prec, replicas=4, final_state=True)
+        keys = ("counters", "fin_g", "fin_gi", "fin_c", "fin_o", "fin_oi", "rec", "rec_rep") if counted else \
+            ("rec", "rec_rep")
         for j in range(nj):
-            for key in ("counters", "fin_g", "fin_gi", "fin_c", "fin_o", "fin_oi", "rec", "rec_rep"):
+            for key in keys:
                 np.testing.assert_array_equal(np.asarray(a[j][key]), np.asarray(b[j][key]),
                                               err_msg=f"job {j} {key}: {desc}")
+            if not counted:
+                assert np.all(a[j]["counters"] == 0), desc
             if b[j]["sig"] is not None:
                 np.testing.assert_array_equal(a[j]["sig"], b[j]["sig"], err_msg=f"job {j}: {desc}")
         j = int(rng.integers(0, nj))

@@ -109,6 +109,7 @@ struct rms_plan {
     int32_t *d_bj_fd = nullptr;
     ReduceDesc *d_red_fd = nullptr;
     bool fd_walks = false;  // launches now are the central differences' walks
+    bool lean = false;      // every job: T2 infinite, no exchange draw (lean_walk)
 };
 
 static const char *kVersion = "rms_amd 0.1 (gfx950)";
@@ -488,6 +489,7 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
     int sig = 0;
     int64_t nbatch = 0;
     int maxbound = 0;
+    bool lean = true;
     p->hjobs.resize(njobs);
     p->jobs.resize(njobs);
     for (int j = 0; j < njobs; j++) {
@@ -517,6 +519,7 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         D.nbz = G.nbz;
         H.exchange = exch;
         H.kappa_closed = kappa_closes_pair(G.meta, d);
+        lean = lean && lean_walk(G.meta, d);
         H.geom_id = J.geom_id;
         H.nwalk = J.nwalk;
         H.nbatch = (J.nwalk + WAVE - 1) / WAVE;
@@ -532,6 +535,7 @@ static rms_status plan_create_impl(rms_ctx *ctx, const rms_job *jobs, int32_t nj
         nbatch += H.nbatch;
     }
     p->sig = sig;
+    p->lean = lean;
     p->nbatch = nbatch;
     p->ws = nbatch * WAVE;
 
@@ -758,7 +762,10 @@ static rms_status enqueue(rms_plan *p, bool with_events)
     for (int64_t s0 = 0; s0 < TN; s0 += ch) {
         int64_t s1 = std::min(TN, s0 + ch);
         LaunchArgs a = make_args(p, s0, s1);
-        HIPCHK(ctx, launch_walk(a, p->prm.precision == RMS_PREC_F64, p->sig, p->grid, p->block, s));
+        // the lean variants only for the plan's own physics: a perturbed D
+        // (the central differences) may make a pair draw
+        const int sig = p->sig | ((p->lean && !p->fd_walks) ? SIG_LEAN : 0);
+        HIPCHK(ctx, launch_walk(a, p->prm.precision == RMS_PREC_F64, sig, p->grid, p->block, s));
     }
     if (with_events) HIPCHK(ctx, hipEventRecord(p->ev[1], s));
     HIPCHK(ctx, launch_reduce(p->d_red, (int32_t)p->red.size(), p->max_nq, p->prm.replicas, s));

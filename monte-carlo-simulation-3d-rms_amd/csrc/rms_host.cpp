@@ -175,6 +175,17 @@ bool wrap_cells_rule(const rms_geometry &g)
     return 256 * side > walk;
 }
 
+bool lean_walk(const rms_geometry &g, const Derived &d)
+{
+    if (!(d.T2s[0] > 0.0)) return false;
+    for (int c = 1; c < g.nclass; c++)
+        if (!std::isinf(d.T2s[c])) return false;
+    for (int a = 1; a < g.nclass; a++)
+        for (int b = 1; b < g.nclass; b++)
+            if (a != b && d.Pcat[a * NC + b] == 2) return false;
+    return true;
+}
+
 bool fd_walker_mask(const rms_geometry &g, double dt, int c, double h, uint32_t &mask)
 {
     mask = 1u << c;

@@ -16,7 +16,7 @@ namespace rms {
 
 // Kernel variant bits (the SIG template argument of walk_kernel).
 enum : int { SIG_NARROW = 1, SIG_PGSE = 2, SIG_EXCH = 4, SIG_PERIODIC = 8, SIG_JUMP = 16, SIG_BRICKS = 32,
-             SIG_WRAP = 64 };
+             SIG_WRAP = 64, SIG_NOCOUNT = 128, SIG_LEAN = 256 };
 
 // The adjoint variant of a plan: the boundary terms of the D derivative
 // (exchange faces, lattice-edge mirrors, voxel edges) whenever no job has a
@@ -65,6 +65,7 @@ bool wrap_cells_rule(const rms_geometry &g);
 // points (walkers that meet it leave no mark): then every walker is walked.
 bool fd_walker_mask(const rms_geometry &g, double dt, int c, double h, uint32_t &mask);
 
+
 // Steps must stay below one voxel by this margin: the straight move ends at
 // e = fma(s, v, o) with o in [0, 1], and e must round below 2 in FP32 and FP64.
 constexpr double STEP_MAX = 1.0 - 0x1p-20;
@@ -92,6 +93,13 @@ bool has_exchange(const rms_geometry &g, const Derived &d);
 // True when some pair of non-null classes is impermeable (P < 1e-6) only
 // because of its kappa (the reference rule would let walkers cross).
 bool kappa_closes_pair(const rms_geometry &g, const Derived &d);
+
+// A walk that needs no T2 times and no exchange draws: every non-null
+// class has an infinite T2 (class 0, never entered, a positive one), so the
+// weight exp(-sum t_c / T2_c) is exp(-0) = 1 exactly, and no pair of
+// non-null classes draws (P is 0 or 1 for each).  Such launches may take the
+// forward variants that keep neither (SIG_LEAN).
+bool lean_walk(const rms_geometry &g, const Derived &d);
 
 int32_t exchange_threshold(int32_t nclass, int cat, float P);
 int32_t pgse_flush_log2(int32_t nx, int32_t ny, int32_t nz);

@@ -45,7 +45,8 @@ const void *brick_walk_fn(int prec_f64, int sig, int np);
 // (rms_kernels_wrap.hip).
 const void *wrap_walk_fn(int prec_f64, int sig, int np);
 // The forward variant without the fast path's diagnostic counters (sig
-// bit 7) for a plan variant sig, or null (rms_kernels_nocount.hip).
+// bit 7) for a plan variant sig (with bit 8: every T2 infinite and no
+// exchange draw), or null (rms_kernels_nocount.hip).
 const void *nocount_walk_fn(int prec_f64, int sig);
 
 // Allocation-free runs: device address of this GPU's scratch array (the

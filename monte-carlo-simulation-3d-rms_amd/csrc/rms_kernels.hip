@@ -256,9 +256,10 @@ static hipError_t launch_fn(const void *fn, const LaunchArgs &a, int grid, int b
 
 hipError_t launch_walk(const LaunchArgs &a, int prec_f64, int sig, int grid, int block, hipStream_t s)
 {
-    // a launch that reports no counters takes the variant that does not sum them
+    // a launch that reports no counters takes the variant that does not sum
+    // them (and, with sig bit 8, no T2 times or draw bits); the others count
     const void *fn = !a.count ? nocount_walk_fn(prec_f64, sig) : nullptr;
-    return launch_fn(fn ? fn : walk_fn_np(prec_f64, sig, 0), a, grid, block, s);
+    return launch_fn(fn ? fn : walk_fn_np(prec_f64, sig & 255, 0), a, grid, block, s);
 }
 
 hipError_t walk_occupancy(int prec_f64, int sig, int block, int *blocks_per_cu)

@@ -356,7 +356,7 @@ __device__ __forceinline__ R exch_uniform(const JobLocal<R> &L, uint64_t wid, in
 // word and the azimuth bits 8..31 of the second; xhi returns the low bytes of
 // the two words (first word's in bits 8..15), the high half of the step's
 // first exchange draw.  f64 layout: xhi = 0 (unused).
-template <typename R>
+template <typename R, bool XHI = true>
 __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, int64_t i, bool fresh_block,
                                           uint32_t &b0, uint32_t &b1, const float2 *tab, uint32_t &xhi)
 {
@@ -372,7 +372,8 @@ __device__ __forceinline__ V3<R> draw_dir(const JobLocal<R> &L, uint64_t wid, in
             b0 = r.z;
             b1 = r.w;
         }
-        xhi = ((w0 & 0xFFu) << 8) | (w1 & 0xFFu);
+        // (launches whose geometries draw nothing leave the draw's bits out)
+        xhi = XHI ? ((w0 & 0xFFu) << 8) | (w1 & 0xFFu) : 0u;
         u1 = u01_f32(w0);
         const R cth = (R)1 - (R)2 * u1;
         const R sth = (R)2 * sqrt_rn_mid(u1 * ((R)1 - u1));
@@ -464,7 +465,7 @@ __device__ __forceinline__ bool exchange_faces(const Walker<R> &w, const JobLoca
 // and the straight move folded back at the reflecting faces.  Returns false
 // when the step needs the march (a crossing into another class, or three
 // comparisons that are not transitive: a near tie).
-template <typename R, int NP, bool EXCH, bool PER, bool WRAP, bool CNT>
+template <typename R, int NP, bool EXCH, bool PER, bool WRAP, bool CNT, bool T2W>
 __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, const JobDesc &J, uint64_t cell,
                                           const V3<R> &v, uint64_t wid, int32_t i, uint32_t xhi, int32_t npg,
                                           Tan<R, NP> &T)
@@ -625,7 +626,7 @@ __device__ __forceinline__ bool fast_step(Walker<R> &w, const JobLocal<R> &L, co
         w.ndraw += ndraw;
         w.ncross += (uint32_t)(__mul24(cx, cx) + __mul24(cy, cy) + __mul24(cz, cz)) - nref;
     }
-    w.tcur = w.tcur + (R)1;
+    if constexpr (T2W) w.tcur = w.tcur + (R)1;  // time in the class for its T2 weight
     return true;
 }
 
@@ -1414,6 +1415,10 @@ void walk_kernel(LaunchArgs a)
     // passes report none), and not in the variants built without them
     // (bit 7, for launches that do not report counters)
     constexpr bool CNT = NP == 0 && (SIG & 128) == 0;
+    // bit 8 (forward walks): every class's T2 is infinite and no class pair
+    // draws, so the T2 weight is exp(-0) = 1 exactly and no step needs the
+    // bits of an exchange draw
+    constexpr bool LEAN = NP == 0 && (SIG & 256) != 0;
     // sin/cos table of the FP32 direction draw, staged in LDS
     __shared__ float2 tab[256];
     // continuations of the kept boundary events (jump variants only)
@@ -1528,7 +1533,7 @@ void walk_kernel(LaunchArgs a)
         for (int32_t i = step0; i < step1; i++) {
             const bool fresh = ((i & 1) == 0) || (i == step0);
             uint32_t xhi;
-            const V3<R> v = draw_dir<R>(L, wid, i, fresh, b0, b1, tab, xhi);
+            const V3<R> v = draw_dir<R, !LEAN>(L, wid, i, fresh, b0, b1, tab, xhi);
             if constexpr (JUMP) {
                 // the continuations walk with the walker's direction and draws
                 if (has) {
@@ -1539,7 +1544,7 @@ void walk_kernel(LaunchArgs a)
                         uint64_t ca;
                         load_cont<R>(cont, m, tix, L, a, ca);
                         const int32_t l0 = a.lin;
-                        if (!fast_step<R, 0, EXCH, PER, WRAP, false>(a, L, J, ca, v, wid, i, xhi, 0, T0))
+                        if (!fast_step<R, 0, EXCH, PER, WRAP, false, true>(a, L, J, ca, v, wid, i, xhi, 0, T0))
                             march<R, 0, EXCH, PER>(a, L, J, wid, i, v, xhi, 0, T0);
                         if (a.lin != l0) ca = cell_of<R, BR>(L, a);
                         store_cont<R>(cont, m, tix, a, ca);
@@ -1596,7 +1601,7 @@ void walk_kernel(LaunchArgs a)
             }
             const int32_t lin0 = w.lin;
             const int32_t npg = i - i0;  // PGSE terms since the last flush
-            if (!fast_step<R, NP, EXCH, PER, WRAP, CNT>(w, L, J, cell, v, wid, i, xhi, npg, T))
+            if (!fast_step<R, NP, EXCH, PER, WRAP, CNT, !LEAN>(w, L, J, cell, v, wid, i, xhi, npg, T))
                 march<R, NP, EXCH, PER>(w, L, J, wid, i, v, xhi, npg, T);
             if (w.lin != lin0) cell = cell_of<R, BR>(L, w);  // only lanes that changed voxel read
             if (SIG & 2) {
@@ -1647,7 +1652,7 @@ void walk_kernel(LaunchArgs a)
             if (i == rnext && i < rend) {
                 // records, main.cu:333-378
                 rnext += Tstep;
-                const double s0 = t2_weight<R>(w, L);
+                const double s0 = LEAN ? 1.0 : t2_weight<R>(w, L);
                 const V3<double> d = displacement<R>(w, L.res);
                 const double x = d.x, y = d.y, z = d.z;
                 if constexpr (NP == 0) {
@@ -1743,7 +1748,7 @@ void walk_kernel(LaunchArgs a)
         }
         if ((SIG & 2) && a.step1 == a.TN) {
             // final PGSE signal, main_PGSE.cu:348-357
-            const double s0 = t2_weight<R>(w, L);
+            const double s0 = LEAN ? 1.0 : t2_weight<R>(w, L);
             double *psig = J.psig + lb * (int64_t)J.ngrad;
             double Fs = 0.0, H[NP > 0 ? NP : 1];
 #pragma unroll

@@ -345,6 +345,31 @@ static void fd_masks()
     check(fd_walker_mask(g, dt, 1, h, m) && m == 2u, "fd mask: one non-null class", "");
 }
 
+// The lean rule: infinite T2 in every non-null class and no drawing pair.
+static void lean_rule()
+{
+    Derived d;
+    rms_geometry g = geom(2, 2.0, 0.0, -1.0);
+    derive(g, 2e-4, d);
+    check(lean_walk(g, d), "lean: one class, T2 infinite", "");
+    g.T2[1] = 80.0;
+    derive(g, 2e-4, d);
+    check(!lean_walk(g, d), "lean: a finite T2", "");
+    g = geom(3, 1.5, 1.5, -1.0);  // P = 1 both ways: crossings without draws
+    derive(g, 2e-4, d);
+    check(lean_walk(g, d), "lean: two classes that cross without a draw", "");
+    g = geom(3, 2.0, 1.5, -1.0);  // P_21 = 1, P_12 = sqrt(0.75): draws
+    derive(g, 2e-4, d);
+    check(!lean_walk(g, d), "lean: a drawing pair", "");
+    g = geom(3, 2.0, 1.5, 0.0);  // kappa = 0: both pairs reflect
+    derive(g, 2e-4, d);
+    check(lean_walk(g, d), "lean: pairs closed by kappa = 0", "");
+    g = geom(2, 2.0, 0.0, -1.0);
+    g.T2[0] = 0.0;
+    derive(g, 2e-4, d);
+    check(!lean_walk(g, d), "lean: class 0 with T2 = 0 (0/0 in the weight)", "");
+}
+
 int main()
 {
     pgse_tables_vs_masks();
@@ -356,6 +381,7 @@ int main()
     perturbed_physics();
     storage_variants();
     fd_masks();
+    lean_rule();
     std::printf("%d failures\n", nfail);
     return nfail;
 }

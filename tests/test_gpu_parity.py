@@ -534,3 +534,37 @@ def test_skipped_counters_allocation_free(rms, engine):
     assert a["counters"][0] > 0 and np.all(b["counters"] == 0)
     np.testing.assert_array_equal(a["rec"], b["rec"])
     np.testing.assert_array_equal(a["rec_rep"], b["rec_rep"])
+
+
+@pytest.mark.parametrize("prec,case", [("f32", "axon"), ("f64", "axon"), ("f32", "cross"), ("f32", "closed"),
+                                       ("f32", "bricks")])
+def test_lean_walks_equal_the_full_variant(rms, orc, engine, prec, case):
+    """Plans whose classes all have infinite T2 and whose pairs never draw
+    take, without counters, the forward variants that keep no time-in-class
+    sums and no draw bits (weight exp(-0) = 1): records, replica sums and
+    the PGSE signal bit-identical to the counted run (the full variant) and
+    to the oracle.  "cross": two classes with equal D and no permeability
+    (P = 1 both ways: class changes without draws); "closed": kappa = 0."""
+    print(f"case lean walk {prec} {case}", flush=True)
+    if case == "axon":
+        lab = cylinder(20, 12, 6.0)
+        D, kw = [0.0, 2.0], {}
+    else:
+        lab = random_blobs((10, 9, 11), 3, seed=31)
+        D, kw = [0.0, 1.5, 1.5 if case != "closed" else 0.8], ({} if case == "cross" else dict(kappa=0.0))
+    T2 = [np.inf] * len(D)
+    st = "bricks" if case == "bricks" else "auto"
+    if case == "bricks":
+        lab = random_blobs((10, 9, 11), 3, seed=31)
+        D, kw = [0.0, 1.5, 1.5], {}
+    grad = pgse_table(fib_dirs(2), [1.0], [0.03], 0.01)
+    mk = lambda M, **x: M.Geometry(lab, 0.1, D, T2=T2, periodic="xz", **kw, **x)  # noqa: E731
+    job = dict(nwalk=900, seed=6, grad=grad, bval=np.array([1.0, 2.0]), bvec=fib_dirs(2))
+    full = engine.run([dict(geom=mk(rms, storage=st), **job)], 2e-4, 400, 20, precision=prec, replicas=4)[0]
+    lean = engine.run([dict(geom=mk(rms, storage=st), **job)], 2e-4, 400, 20, precision=prec, replicas=4,
+                      counters=False)[0]
+    ref = orc.run([dict(geom=mk(orc), **job)], 2e-4, 400, 20, prec=prec, replicas=4)[0]
+    for key in ("rec", "rec_rep", "sig", "sig_rep"):
+        np.testing.assert_array_equal(lean[key], full[key], err_msg=key)
+        np.testing.assert_array_equal(lean[key], ref[key], err_msg=key)
+    assert np.all(lean["counters"] == 0) and full["counters"][1] > 0

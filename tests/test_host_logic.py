@@ -21,7 +21,7 @@ def test_host_descriptor_builder(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
-    assert r.stdout.count("ok ") >= 40
+    assert r.stdout.count("ok ") >= 46
 
 
 def test_oracle_wrap_rule_matches_the_library_rule(orc):

@@ -564,7 +564,12 @@ def test_lean_walks_equal_the_full_variant(rms, orc, engine, prec, case):
     lean = engine.run([dict(geom=mk(rms, storage=st), **job)], 2e-4, 400, 20, precision=prec, replicas=4,
                       counters=False)[0]
     ref = orc.run([dict(geom=mk(orc), **job)], 2e-4, 400, 20, prec=prec, replicas=4)[0]
+    # the same lean walk in launches of 130 steps (walker state saved between
+    # them), replayed as a captured graph
+    chunked = engine.run([dict(geom=mk(rms, storage=st), **job)], 2e-4, 400, 20, precision=prec, replicas=4,
+                         counters=False, chunk_steps=130, use_graph=True)[0]
     for key in ("rec", "rec_rep", "sig", "sig_rep"):
         np.testing.assert_array_equal(lean[key], full[key], err_msg=key)
         np.testing.assert_array_equal(lean[key], ref[key], err_msg=key)
+        np.testing.assert_array_equal(chunked[key], ref[key], err_msg="chunked " + key)
     assert np.all(lean["counters"] == 0) and full["counters"][1] > 0

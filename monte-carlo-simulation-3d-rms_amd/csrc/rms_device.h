@@ -237,6 +237,28 @@ __device__ __forceinline__ double wave_sum(double x)
     return x;
 }
 
+// The value of lane (lane ^ M), for a full wavefront: DPP quad permutations
+// for M = 1, 2 (VALU), ds_swizzle's xor mode inside each 32-lane half for
+// M = 4, 8, 16, ds_bpermute across the halves for M = 32.
+template <int M>
+__device__ __forceinline__ int xlane_i32(int v)
+{
+    static_assert(M == 1 || M == 2 || M == 4 || M == 8 || M == 16 || M == 32, "lane distance");
+    if constexpr (M == 1) return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+    else if constexpr (M == 2) return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // 2,3,0,1
+    else if constexpr (M == 32) return __shfl_xor(v, 32, 64);
+    else return __builtin_amdgcn_ds_swizzle(v, 0x1F | (M << 10));  // and 0x1F, or 0, xor M
+}
+
+template <int M>
+__device__ __forceinline__ double xlane(double x)
+{
+    const uint64_t u = (uint64_t)__double_as_longlong(x);
+    const uint32_t lo = (uint32_t)xlane_i32<M>((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)xlane_i32<M>((int)(uint32_t)(u >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t x)
 {
 #pragma unroll

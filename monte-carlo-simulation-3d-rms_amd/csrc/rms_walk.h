@@ -1272,6 +1272,73 @@ __device__ __forceinline__ void put(double v, bool active, double *dst, int lane
     if (lane == 0) *dst = t;
 }
 
+// The fixed part of a record row (main.cu:333-372): position 0 the weight,
+// 1..4 the class indicators of classes 0..3, 5..25 the 21 weighted second and
+// fourth moments in the row's order, 26..31 zero.  Products are taken left to
+// right from s0.
+constexpr int8_t REC_MOM[21][4] = {
+    {0, 0, -1, -1}, {0, 1, -1, -1}, {0, 2, -1, -1}, {1, 1, -1, -1}, {1, 2, -1, -1}, {2, 2, -1, -1},
+    {0, 0, 0, 0}, {0, 0, 0, 1}, {0, 0, 0, 2}, {0, 0, 1, 1}, {0, 0, 1, 2}, {0, 0, 2, 2},
+    {0, 1, 1, 1}, {0, 1, 1, 2}, {0, 1, 2, 2}, {0, 2, 2, 2},
+    {1, 1, 1, 1}, {1, 1, 1, 2}, {1, 1, 2, 2}, {1, 2, 2, 2}, {2, 2, 2, 2}};
+
+struct RecRow {
+    double s0, d[3];
+    int32_t c;
+    bool active;
+};
+
+template <int P>
+__device__ __forceinline__ double rec_value(const RecRow &r)
+{
+    double v = 0.0;
+    if constexpr (P == 0) {
+        v = r.s0;
+    } else if constexpr (P <= 4) {
+        v = r.c == P - 1 ? 1.0 : 0.0;
+    } else if constexpr (P <= 25) {
+        v = r.s0;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (REC_MOM[P - 5][k] >= 0) v = v * r.d[REC_MOM[P - 5][k]];
+    }
+    return r.active ? v : 0.0;
+}
+
+// Reduce-scatter of the 32 positions over lane bits 0..LV-1: slot Q of level
+// LV holds, in a lane whose bit LV-1 is b, the sum over lane pairs at distance
+// 2^(LV-1) of slot Q + b*H of level LV-1 (H = 32 >> LV).  Each position meets
+// exactly the pairs of wave_sum's butterfly in the same order, so its total is
+// bit-identical to wave_sum of that value.
+template <int LV, int Q>
+__device__ __forceinline__ double rec_tree(const RecRow &r, int lane)
+{
+    if constexpr (LV == 0) {
+        return rec_value<Q>(r);
+    } else {
+        constexpr int H = 32 >> LV;
+        const double lo = rec_tree<LV - 1, Q>(r, lane);
+        const double hi = rec_tree<LV - 1, Q + H>(r, lane);
+        const bool up = ((lane >> (LV - 1)) & 1) != 0;
+        return (up ? hi : lo) + xlane<1 << (LV - 1)>(up ? lo : hi);
+    }
+}
+
+// Wave-sum the fixed part of a record row and store it: lane L < 32 ends with
+// the total of position bitrev5(L), written to its place in the row (the class
+// indicators only for the geometry's classes).
+__device__ __forceinline__ void put_row(const RecRow &r, int nclass, double *dst, int lane)
+{
+    // an opaque copy of the lane index: the lane-bit masks of the tree are
+    // formed here, not hoisted out of the step loop into scalar registers
+    asm volatile("" : "+v"(lane));
+    double t = rec_tree<5, 0>(r, lane);
+    t = t + xlane<32>(t);
+    const int p = (int)(__builtin_bitreverse32((uint32_t)lane) >> 27);
+    const int off = p == 0 ? 0 : p <= 4 ? (p - 1 < nclass ? p : -1) : p <= 25 ? 1 + nclass + (p - 5) : -1;
+    if (lane < 32 && off >= 0) dst[off] = t;
+}
+
 template <typename R>
 __device__ __forceinline__ void load_state(Walker<R> &w, const StateBuf &st, int64_t s)
 {
@@ -1657,30 +1724,8 @@ void walk_kernel(LaunchArgs a)
                 const double x = d.x, y = d.y, z = d.z;
                 if constexpr (NP == 0) {
                     double *dst = part + (int64_t)r * nval;
-                    put(s0, active, dst + 0, lane);
-                    for (int cc = 0; cc < L.nclass; cc++) put(cc == w.c ? 1.0 : 0.0, active, dst + 1 + cc, lane);
+                    put_row(RecRow{s0, {x, y, z}, w.c, active}, L.nclass, dst, lane);
                     double *m = dst + 1 + L.nclass;
-                    put(s0 * x * x, active, m + 0, lane);
-                    put(s0 * x * y, active, m + 1, lane);
-                    put(s0 * x * z, active, m + 2, lane);
-                    put(s0 * y * y, active, m + 3, lane);
-                    put(s0 * y * z, active, m + 4, lane);
-                    put(s0 * z * z, active, m + 5, lane);
-                    put(s0 * x * x * x * x, active, m + 6, lane);
-                    put(s0 * x * x * x * y, active, m + 7, lane);
-                    put(s0 * x * x * x * z, active, m + 8, lane);
-                    put(s0 * x * x * y * y, active, m + 9, lane);
-                    put(s0 * x * x * y * z, active, m + 10, lane);
-                    put(s0 * x * x * z * z, active, m + 11, lane);
-                    put(s0 * x * y * y * y, active, m + 12, lane);
-                    put(s0 * x * y * y * z, active, m + 13, lane);
-                    put(s0 * x * y * z * z, active, m + 14, lane);
-                    put(s0 * x * z * z * z, active, m + 15, lane);
-                    put(s0 * y * y * y * y, active, m + 16, lane);
-                    put(s0 * y * y * y * z, active, m + 17, lane);
-                    put(s0 * y * y * z * z, active, m + 18, lane);
-                    put(s0 * y * z * z * z, active, m + 19, lane);
-                    put(s0 * z * z * z * z, active, m + 20, lane);
                     if (SIG & 1) {
                         const double *qs = J.qscale + (int64_t)r * nbvec;
                         for (int jb = 0; jb < nbvec; jb++) {

@@ -15,7 +15,9 @@
 // other lanes run the face march in the same step.
 //
 // Per-record moments are summed across the wavefront with a fixed butterfly
-// and written as per-batch partials; a second kernel sums the partials in a
+// (the fixed part of a row by a reduce-scatter that adds, per value, the same
+// lane pairs in the same order) and written as per-batch partials; a second
+// kernel sums the partials in a
 // fixed order, so results are bit-reproducible and identical to the host
 // restatement.
 //
